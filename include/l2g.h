@@ -1,0 +1,158 @@
+/* l2g.h — C ABI of the MI355X (gfx950) two-electron Coulomb integral library.
+ *
+ * This is the drop-in boundary for libint's
+ *   libint2::Engine(Operator::coulomb, max_nprim, max_l) / Engine::compute(s1,s2,s3,s4)
+ * path (include/libint2/engine.h:502-526, :747-749; implementation
+ * include/libint2/engine.impl.h:1151-2113).  Every entry point takes plain
+ * pointers and sizes, returns 0 on success and a negative L2G_E* code on
+ * failure (message in l2g_last_error()), and never lets a C++ exception
+ * cross the boundary.  Results are bitwise laid out exactly as libint lays
+ * out Engine::results()[0]: row-major (s1 s2|s3 s4) in the caller's shell
+ * order, Cartesian functions in libint's standard order
+ * (include/libint2/cgshell_ordering.h:125-130), real solid harmonics in the
+ * Standard (m=-l..l) or Gaussian order (include/libint2/shgshell_ordering.h).
+ *
+ * Host-only utilities (no GPU needed): l2g_shell_renorm,
+ * l2g_solid_harmonics_coeff, l2g_boys_table, l2g_version.
+ */
+#ifndef L2G_H
+#define L2G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define L2G_OK 0
+#define L2G_E_INVALID -1      /* bad argument (std::invalid_argument in libint) */
+#define L2G_E_LMAX -2         /* angular momentum above the compiled limit (Engine::lmax_exceeded) */
+#define L2G_E_NODEVICE -3     /* no HIP device / HIP runtime error */
+#define L2G_E_OOM -4          /* device or host allocation failed */
+#define L2G_E_LAUNCH -5       /* kernel launch or execution error */
+#define L2G_E_STATE -6        /* object used in the wrong state (Engine::using_default_initialized) */
+
+/* screening methods, values as libint2::ScreeningMethod (include/libint2/shell.h:1041-1059) */
+#define L2G_SCREEN_ORIGINAL 0x0001
+#define L2G_SCREEN_CONSERVATIVE 0x0010
+#define L2G_SCREEN_SCHWARZ 0x0100
+#define L2G_SCREEN_SCHWARZINF 0x1000
+
+/* solid-harmonic orderings (include/libint2/shgshell_ordering.h) */
+#define L2G_SHG_STANDARD 1
+#define L2G_SHG_GAUSSIAN 2
+
+typedef struct l2g_context_s* l2g_context;
+typedef struct l2g_basis_s* l2g_basis;
+typedef struct l2g_eri_plan_s* l2g_eri_plan;
+
+/* ---- library / device ------------------------------------------------- */
+
+/* Version string. */
+const char* l2g_version(void);
+
+/* Message of the last failed call on this thread. */
+const char* l2g_last_error(void);
+
+/* Largest angular momentum per shell the compiled kernels cover
+ * (libint: LIBINT2_MAX_AM_eri, engine.impl.h:620-622). */
+int l2g_max_l(void);
+
+/* Opens a device context: selects the HIP device, creates a stream, builds
+ * and uploads the Boys-function table.  Replaces libint2::initialize()
+ * (include/libint2/initialize.h:76-90) for this path. */
+int l2g_init(int device, l2g_context* ctx);
+
+/* Releases the context (libint2::finalize, initialize.h:105-113). */
+int l2g_finalize(l2g_context ctx);
+
+/* Solid-harmonic ordering of pure shells for results produced after the call
+ * (libint2::set_solid_harmonics_ordering, initialize.h:117-119). */
+int l2g_set_solid_harmonics_ordering(l2g_context ctx, int ordering);
+
+/* ---- host utilities ---------------------------------------------------- */
+
+/* Embeds primitive normalization into contraction coefficients and, if
+ * enforce_unit, rescales the contraction to unit self-overlap; also returns
+ * ln|c| per primitive.  Replaces Shell::renorm + update_max_ln_coeff
+ * (include/libint2/shell.h:958-1011).  coeff is updated in place. */
+int l2g_shell_renorm(int l, int nprim, const double* alpha, double* coeff, double* max_ln_coeff,
+                     int enforce_unit);
+
+/* Coefficient of Cartesian (lx,ly,lz) in real solid harmonic (l,m)
+ * (SolidHarmonicsCoefficients::coeff, include/libint2/solidharmonics.h:114-180). */
+double l2g_solid_harmonics_coeff(int l, int m, int lx, int ly, int lz);
+
+/* Copies the host Boys table, layout [m][interval][8] (monomial coefficients
+ * of xd = 7T - iv - 1/2), m in [0, *mmax], interval in [0, *nint).
+ * Pass out = NULL to query the sizes. */
+int l2g_boys_table(double* out, int* nint, int* mmax);
+
+/* ---- basis ------------------------------------------------------------- */
+
+/* Uploads a basis.  Arrays are per shell (l, pure, nprim, origin[3*nshell],
+ * bohr) and per primitive in shell order (alpha, coeff).  Coefficients must
+ * already be renormalised (l2g_shell_renorm), exactly as libint2::Shell
+ * stores them.  Only single-contraction shells (libint asserts the same,
+ * engine.impl.h:1168-1171). */
+int l2g_basis_create(l2g_context ctx, int nshell, const int* l, const int* pure, const int* nprim,
+                     const double* alpha, const double* coeff, const double* origin, l2g_basis* basis);
+int l2g_basis_destroy(l2g_basis basis);
+
+/* ---- ERI batches ------------------------------------------------------- */
+
+/* Plans a batch of shell quartets given as caller-order index 4-tuples
+ * (quartets[4*q+0..3]).  Each quartet is canonicalised as
+ * Engine::compute2 does (engine.impl.h:1181-1208), bucketed by class and
+ * contraction depth, its shell-pair data is built with the requested
+ * precision and primitive screening (Original or Conservative, as Engine
+ * allows without precomputed pairs, engine.impl.h:1212-1216), and the
+ * work lists are uploaded.  scale = Engine::prescale_by (engine.h:857),
+ * uniform_cart = CartesianShellNormalization::uniform. */
+int l2g_eri_plan_create(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision,
+                        int screening, double scale, int uniform_cart, l2g_eri_plan* plan);
+
+/* Number of doubles the batch produces. */
+int l2g_eri_plan_size(l2g_eri_plan plan, int64_t* total);
+
+/* Per-quartet offsets (in doubles) into the batch output, caller order. */
+int l2g_eri_plan_offsets(l2g_eri_plan plan, int64_t* offsets);
+
+/* Runs the batch on the context stream.  d_out = device buffer of
+ * l2g_eri_plan_size doubles, or NULL to use a buffer owned by the plan.
+ * Asynchronous: returns after the launches are queued. */
+int l2g_eri_plan_execute(l2g_eri_plan plan, double* d_out);
+
+/* Waits for the context stream. */
+int l2g_synchronize(l2g_context ctx);
+
+/* Copies the plan-owned output to the host; nonzero (optional, nquartets
+ * bytes) receives 0 for quartets whose every primitive was screened out,
+ * where Engine::compute returns results()[0] == nullptr
+ * (engine.impl.h:1781-1784). */
+int l2g_eri_plan_download(l2g_eri_plan plan, double* host_out, int8_t* nonzero);
+
+/* Device pointer of the plan-owned output (valid until destroy). */
+int l2g_eri_plan_device_output(l2g_eri_plan plan, double** d_out);
+
+/* Per-class breakdown of the plan: for each of the *nclasses classes
+ * (up to cap), the class (la,lb,lc,ld) packed as la*1000+lb*100+lc*10+ld,
+ * the number of quartets, the number of primitive quartets (before the
+ * per-primitive screen) and the duration in ms of its last execution
+ * (HIP events on the context stream). */
+int l2g_eri_plan_stats(l2g_eri_plan plan, int cap, int* nclasses, int* class_code, int64_t* nquartets,
+                       int64_t* nprim_quartets, float* ms);
+
+int l2g_eri_plan_destroy(l2g_eri_plan plan);
+
+/* One-shot convenience: plan, execute, download, destroy.  offsets and
+ * nonzero may be NULL. */
+int l2g_eri_compute(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision, int screening,
+                    double* host_out, int64_t* offsets, int8_t* nonzero);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* L2G_H */

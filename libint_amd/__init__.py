@@ -1,0 +1,358 @@
+"""libint_amd — MI355X (gfx950) evaluator for libint's two-electron Coulomb path.
+
+Python host bindings over the C ABI in include/l2g.h (libint_amd/lib/libl2g.so).
+The GPU work is done by the generated HIP kernels inside that library; this
+module only moves arrays across the boundary.  There is no CPU fallback: if
+the library is missing or no GPU is visible, the calls raise.
+
+The objects mirror libint's C++ API for this path:
+
+* :class:`Shell` — ``libint2::Shell`` (include/libint2/shell.h:720-1012):
+  exponents, one contraction ``(l, pure, coeff)``, origin; coefficients are
+  renormalised on construction (``Shell::renorm``, shell.h:958-999).
+* :class:`Engine` — ``libint2::Engine(Operator::coulomb, max_nprim, max_l)``
+  (include/libint2/engine.h:502-526) with ``compute(s1, s2, s3, s4)``
+  returning the libint-ordered buffer, or ``None`` where libint returns
+  ``results()[0] == nullptr`` (engine.impl.h:1781-1784), plus the batched
+  ``compute_batch``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "lib", "libl2g.so")
+
+SCREEN_ORIGINAL, SCREEN_CONSERVATIVE, SCREEN_SCHWARZ, SCREEN_SCHWARZINF = 0x0001, 0x0010, 0x0100, 0x1000
+SHG_STANDARD, SHG_GAUSSIAN = 1, 2
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes) of every symbol in include/l2g.h
+ABI = {
+    "l2g_version": (ctypes.c_char_p, []),
+    "l2g_last_error": (ctypes.c_char_p, []),
+    "l2g_max_l": (ctypes.c_int, []),
+    "l2g_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "l2g_finalize": (ctypes.c_int, [_vp]),
+    "l2g_set_solid_harmonics_ordering": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_shell_renorm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _f64p, _f64p, _vp, ctypes.c_int]),
+    "l2g_solid_harmonics_coeff": (ctypes.c_double, [ctypes.c_int] * 5),
+    "l2g_boys_table": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "l2g_basis_create": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                        ctypes.POINTER(_vp)]),
+    "l2g_basis_destroy": (ctypes.c_int, [_vp]),
+    "l2g_eri_plan_create": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "l2g_eri_plan_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "l2g_eri_plan_offsets": (ctypes.c_int, [_vp, _i64p]),
+    "l2g_eri_plan_execute": (ctypes.c_int, [_vp, _vp]),
+    "l2g_synchronize": (ctypes.c_int, [_vp]),
+    "l2g_eri_plan_download": (ctypes.c_int, [_vp, _vp, _vp]),
+    "l2g_eri_plan_device_output": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "l2g_eri_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
+    "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
+    "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class L2gError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[l2g {code}] {msg}")
+        self.code = code
+
+
+class LmaxExceeded(L2gError):
+    """libint2::Engine::lmax_exceeded (include/libint2/engine.h:893-916)."""
+
+
+def build() -> None:
+    """Generate and compile the gfx950 kernels and the C ABI library."""
+    jobs = str(min(16, os.cpu_count() or 8))
+    subprocess.run(["make", "-s", "-C", _HERE, "-j", jobs], check=True)
+
+
+def lib():
+    """Load libl2g.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} not built; run libint_amd.build() (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in ABI.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().l2g_last_error().decode(errors="replace")
+        if rc == -2:
+            raise LmaxExceeded(rc, msg)
+        raise L2gError(rc, msg)
+
+
+def max_l() -> int:
+    return lib().l2g_max_l()
+
+
+def shell_renorm(l: int, alpha, coeff, enforce_unit: bool = True):
+    a = np.ascontiguousarray(alpha, np.float64)
+    c = np.array(coeff, np.float64)
+    mlc = np.zeros_like(c)
+    _check(lib().l2g_shell_renorm(l, a.size, a, c, mlc.ctypes.data, 1 if enforce_unit else 0))
+    return c, mlc
+
+
+def solid_harmonics_coeff(l, m, lx, ly, lz) -> float:
+    return lib().l2g_solid_harmonics_coeff(l, m, lx, ly, lz)
+
+
+def boys_table() -> np.ndarray:
+    """Host copy of the Boys table, shape (mmax+1, nint, 8)."""
+    nint, mmax = ctypes.c_int(), ctypes.c_int()
+    _check(lib().l2g_boys_table(None, ctypes.byref(nint), ctypes.byref(mmax)))
+    out = np.zeros((mmax.value + 1, nint.value, 8))
+    _check(lib().l2g_boys_table(out.ctypes.data, ctypes.byref(nint), ctypes.byref(mmax)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# shells and basis arrays
+# ---------------------------------------------------------------------------
+
+
+class Shell:
+    """Single-contraction Gaussian shell, coefficients renormalised as
+    libint2::Shell does (shell.h:767-777 with renorm :958-999).
+
+    ``Shell(l, [(alpha, coeff), ...], origin=(0,0,0), pure=...)`` mirrors the
+    python binding's ``Shell(L, primitives, center, pure)``
+    (python/src/libint2/libint2.cc:155-156)."""
+
+    def __init__(self, l: int, primitives: Sequence, origin=(0.0, 0.0, 0.0), pure: bool = True,
+                 renorm: bool = True):
+        self.l = int(l)
+        self.pure = bool(pure)
+        self.alpha = np.array([p[0] for p in primitives], np.float64)
+        c = np.array([p[1] for p in primitives], np.float64)
+        if renorm:
+            c, _ = shell_renorm(self.l, self.alpha, c)
+        self.coeff = c
+        self.O = np.array(origin, np.float64).reshape(3)
+
+    def size(self) -> int:
+        return 2 * self.l + 1 if self.pure else (self.l + 1) * (self.l + 2) // 2
+
+    def nprim(self) -> int:
+        return self.alpha.size
+
+
+class BasisArrays:
+    """Flat per-shell / per-primitive arrays, the form the C ABI takes."""
+
+    def __init__(self, l, pure, nprim, alpha, coeff, origin):
+        self.l = np.ascontiguousarray(l, np.int32)
+        self.pure = np.ascontiguousarray(pure, np.int32)
+        self.nprim = np.ascontiguousarray(nprim, np.int32)
+        self.alpha = np.ascontiguousarray(alpha, np.float64)
+        self.coeff = np.ascontiguousarray(coeff, np.float64)
+        self.origin = np.ascontiguousarray(np.asarray(origin, np.float64).reshape(-1))
+
+    @staticmethod
+    def from_shells(shells: Iterable[Shell]) -> "BasisArrays":
+        shells = list(shells)
+        return BasisArrays([s.l for s in shells], [int(s.pure) for s in shells], [s.nprim() for s in shells],
+                           np.concatenate([s.alpha for s in shells]) if shells else np.zeros(0),
+                           np.concatenate([s.coeff for s in shells]) if shells else np.zeros(0),
+                           np.concatenate([s.O for s in shells]) if shells else np.zeros(0))
+
+    @property
+    def nshell(self) -> int:
+        return int(self.l.size)
+
+    def sizes(self) -> np.ndarray:
+        return np.where(self.pure != 0, 2 * self.l + 1, (self.l + 1) * (self.l + 2) // 2)
+
+
+# ---------------------------------------------------------------------------
+# device objects
+# ---------------------------------------------------------------------------
+
+
+class Context:
+    """HIP device context (libint2::initialize for this path)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().l2g_init(device, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_solid_harmonics_ordering(self, ordering: int) -> None:
+        _check(lib().l2g_set_solid_harmonics_ordering(self._h, ordering))
+
+    def synchronize(self) -> None:
+        _check(lib().l2g_synchronize(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _check(lib().l2g_finalize(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Basis:
+    def __init__(self, ctx: Context, arrays: BasisArrays):
+        self.ctx = ctx
+        self.arrays = arrays
+        h = _vp()
+        a = arrays
+        _check(lib().l2g_basis_create(ctx.handle, a.nshell, a.l, a.pure, a.nprim, a.alpha, a.coeff, a.origin,
+                                      ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().l2g_basis_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class EriPlan:
+    """A planned batch of shell quartets (caller order)."""
+
+    def __init__(self, basis: Basis, quartets, precision: float = np.finfo(float).eps,
+                 screening: int = SCREEN_ORIGINAL, scale: float = 1.0, uniform_cart: bool = False):
+        self.basis = basis
+        q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+        self.nq = q.shape[0]
+        h = _vp()
+        _check(lib().l2g_eri_plan_create(basis.handle, self.nq, q.reshape(-1), precision, screening, scale,
+                                         int(uniform_cart), ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        _check(lib().l2g_eri_plan_size(h, ctypes.byref(n)))
+        self.size = n.value
+
+    def offsets(self) -> np.ndarray:
+        o = np.zeros(self.nq, np.int64)
+        if self.nq:
+            _check(lib().l2g_eri_plan_offsets(self._h, o))
+        return o
+
+    def execute(self, d_out: int | None = None) -> None:
+        _check(lib().l2g_eri_plan_execute(self._h, d_out))
+
+    def device_output(self) -> int:
+        p = _vp()
+        _check(lib().l2g_eri_plan_device_output(self._h, ctypes.byref(p)))
+        return p.value
+
+    def download(self):
+        out = np.zeros(max(self.size, 1))
+        nz = np.zeros(max(self.nq, 1), np.int8)
+        _check(lib().l2g_eri_plan_download(self._h, out.ctypes.data, nz.ctypes.data))
+        return out[: self.size], nz[: self.nq]
+
+    def stats(self):
+        cap = 256
+        n = ctypes.c_int()
+        code = np.zeros(cap, np.int32)
+        nq = np.zeros(cap, np.int64)
+        npq = np.zeros(cap, np.int64)
+        ms = np.zeros(cap, np.float32)
+        _check(lib().l2g_eri_plan_stats(self._h, cap, ctypes.byref(n), code.ctypes.data, nq.ctypes.data,
+                                        npq.ctypes.data, ms.ctypes.data))
+        k = n.value
+        return [dict(cls=f"{c:04d}", nquartets=int(a), nprim=int(b), ms=float(t))
+                for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().l2g_eri_plan_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,
+                     scale=1.0, uniform_cart=False):
+    """Evaluate a batch; returns (values, offsets, nonzero)."""
+    p = EriPlan(basis, quartets, precision, screening, scale, uniform_cart)
+    offs = p.offsets()
+    p.execute()
+    vals, nz = p.download()
+    return vals, offs, nz
+
+
+class Engine:
+    """Coulomb ``libint2::Engine`` for this path (engine.h:502-526).
+
+    ``compute(s1, s2, s3, s4)`` evaluates one quartet on the GPU and returns
+    a flat array in libint's layout, or None where libint returns nullptr.
+    One Engine per thread, as libint requires (engine.h:560-579)."""
+
+    def __init__(self, max_nprim: int, max_l: int, precision: float = np.finfo(float).eps,
+                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None):
+        if max_nprim <= 0:
+            raise ValueError("max_nprim must be positive")
+        if max_l > globals()["max_l"]():
+            raise LmaxExceeded(-2, f"max_l {max_l} exceeds the compiled limit {globals()['max_l']()}")
+        self.max_nprim = max_nprim
+        self.max_l_ = max_l
+        self.ctx = ctx or Context(0)
+        self.set_precision(precision)
+        self.screening = screening
+        self.scale = 1.0
+        self.uniform_cart = False
+
+    def set_precision(self, prec: float) -> "Engine":
+        self.precision = max(0.0, float(prec))
+        return self
+
+    def prescale_by(self, s: float) -> "Engine":
+        self.scale = float(s)
+        return self
+
+    def compute(self, s1: Shell, s2: Shell, s3: Shell, s4: Shell):
+        for s in (s1, s2, s3, s4):
+            if s.l > self.max_l_:
+                raise LmaxExceeded(-2, "the angular momentum limit is exceeded")
+            if s.nprim() > self.max_nprim:
+                raise ValueError("shell has more primitives than max_nprim")
+        b = Basis(self.ctx, BasisArrays.from_shells([s1, s2, s3, s4]))
+        vals, _, nz = compute_quartets(b, [[0, 1, 2, 3]], self.precision, self.screening, self.scale,
+                                       self.uniform_cart)
+        return vals if nz[0] else None
+
+    def compute_batch(self, basis: Basis, quartets):
+        return compute_quartets(basis, quartets, self.precision, self.screening, self.scale, self.uniform_cart)

@@ -1,0 +1,720 @@
+"""Generator of the gfx950 ERI kernels (one kernel per canonical class).
+
+Each generated kernel evaluates contracted (ab|cd) Coulomb integrals for one
+canonical angular-momentum class (la>=lb, lc>=ld, la+lb<=lc+ld, the class
+predicate of libint's generator, src/bin/libint/build_libint.cc:79-83) with
+the Head-Gordon-Pople scheme libint uses by default
+(src/bin/libint/strategy.cc:67-80: VRR on a and c, HRR on b and d):
+
+  1. Obara-Saika VRR builds [e0|f0]^(0), e in [la, la+lb], f in [lc, lc+ld]
+     (src/bin/libint/vrr_11_twoprep_11.h:152-224), accumulated over the
+     primitive quartets (src/bin/libint/dg.cc:1177-1189);
+  2. HRR (ab| <- (a+1 b-1| + AB (a b-1|, and the same on the ket with CD
+     (src/bin/libint/hrr.h:290-330);
+  3. Cartesian -> real solid harmonics with the coefficients of
+     include/libint2/solidharmonics.h:114-180;
+  4. a strided write that undoes the canonical reorder
+     (include/libint2/engine.impl.h:1987-2067).
+
+Mapping to the hardware: one lane owns one shell quartet and keeps every
+recurrence intermediate in VGPRs.  The contracted target set of a large class
+does not fit one lane's registers ((dd|dd): 961 contracted values plus ~330
+live intermediates), so the targets are partitioned by their bra Cartesian
+component into groups whose recursion sub-graph fits a register budget.  Each
+group runs the whole primitive loop on its own (recomputing the shared part of
+the recursion), transforms its rows through the ket HRR + ket spherical
+transform, and parks them in a lane-private scratch column in HBM; a final
+pass reads the rows back column by column for the bra HRR, bra spherical
+transform and the permuted store.  Classes whose targets fit in one group keep
+everything in registers.
+
+Run as a script: writes the kernels and the dispatch table into the output
+directory given on the command line.
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+from dataclasses import dataclass, field
+
+# ---------------------------------------------------------------------------
+# shell conventions
+# ---------------------------------------------------------------------------
+
+
+def cart(l):
+    """Cartesian components of level l, libint standard order
+    (include/libint2/cgshell_ordering.h:125-130)."""
+    return [(i, j, l - i - j) for i in range(l, -1, -1) for j in range(l - i, -1, -1)]
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def cart_index(t):
+    x, y, z = t
+    l = x + y + z
+    return ((l - x + 1) * (l - x)) // 2 + l - x - y
+
+
+def _fac(n):
+    return float(math.factorial(n)) if n >= 0 else 1.0
+
+
+def _df_km1(k):
+    r = 1.0
+    i = k - 1
+    while i > 1:
+        r *= i
+        i -= 2
+    return r
+
+
+def _binom(n, k):
+    if k < 0 or k > n:
+        return 0.0
+    return float(math.comb(n, k))
+
+
+def _parity(i):
+    return -1 if i % 2 else 1
+
+
+def sh_coeff(l, m, lx, ly, lz):
+    """Coefficient of Cartesian (lx,ly,lz) in real solid harmonic (l,m),
+    IJQC 54, 83 (1995) eq. 15 — the formula libint evaluates in
+    include/libint2/solidharmonics.h:114-180."""
+    abs_m = abs(m)
+    if (lx + ly - abs_m) % 2:
+        return 0.0
+    j = (lx + ly - abs_m) // 2
+    if j < 0:
+        return 0.0
+    comp = 1 if m >= 0 else -1
+    i = abs_m - lx
+    if comp != _parity(abs(i)):
+        return 0.0
+    pfac = math.sqrt(((_fac(2 * lx) * _fac(2 * ly) * _fac(2 * lz)) / _fac(2 * l)) *
+                     (_fac(l - abs_m) / _fac(l)) * (1.0 / _fac(l + abs_m)) *
+                     (1.0 / (_fac(lx) * _fac(ly) * _fac(lz))))
+    pfac /= float(1 << l)
+    # C integer division truncates toward zero
+    pfac *= _parity(int((i - 1) / 2)) if m < 0 else _parity(int(i / 2))
+    s = 0.0
+    for ii in range(j, (l - abs_m) // 2 + 1):
+        p1 = _binom(l, ii) * _binom(ii, j)
+        p1 *= (_parity(ii) * _fac(2 * (l - ii))) / _fac(l - abs_m - 2 * ii)
+        s1 = 0.0
+        for k in range(max((lx - abs_m) // 2, 0), min(j, lx // 2) + 1):
+            if lx - 2 * k <= abs_m:
+                s1 += _binom(j, k) * _binom(abs_m, lx - 2 * k) * _parity(k)
+        s += p1 * s1
+    s *= math.sqrt(_df_km1(2 * l) / (_df_km1(2 * lx) * _df_km1(2 * ly) * _df_km1(2 * lz)))
+    return pfac * s if m == 0 else math.sqrt(2.0) * pfac * s
+
+
+def sh_rows(l):
+    """Standard-order (m=-l..l) rows: list of [(cart index, coeff)]."""
+    rows = []
+    for m in range(-l, l + 1):
+        r = []
+        for ic, c in enumerate(cart(l)):
+            v = sh_coeff(l, m, *c)
+            if v != 0.0:
+                r.append((ic, v))
+        rows.append(r)
+    return rows
+
+
+def build_dir(t):
+    """Axis used to lower a Cartesian triple: the one with the largest quantum
+    number (ties -> x, y, z order)."""
+    best = 0
+    for i in (1, 2):
+        if t[i] > t[best]:
+            best = i
+    return best
+
+
+def sub1(t, i):
+    u = list(t)
+    u[i] -= 1
+    return tuple(u)
+
+
+def add1(t, i):
+    u = list(t)
+    u[i] += 1
+    return tuple(u)
+
+
+# ---------------------------------------------------------------------------
+# Obara-Saika VRR graph
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class VNode:
+    key: tuple
+    kind: str          # 'base' | 'bra' | 'ket'
+    dir: int = -1
+    deps: list = field(default_factory=list)   # [(coef_expr, node_key)]
+
+
+class VrrGraph:
+    def __init__(self):
+        self.nodes: dict = {}
+
+    def get(self, e, f, m):
+        k = (e, f, m)
+        if k in self.nodes:
+            return k
+        if sum(f) > 0:
+            i = build_dir(f)
+            f1 = sub1(f, i)
+            n = VNode(k, 'ket', i)
+            n.deps.append(('QC', self.get(e, f1, m)))
+            n.deps.append(('WQ', self.get(e, f1, m + 1)))
+            if f1[i] > 0:
+                f2 = sub1(f1, i)
+                n.deps.append((('f', f1[i]), self.get(e, f2, m)))
+                n.deps.append((('fr', f1[i]), self.get(e, f2, m + 1)))
+            if e[i] > 0:
+                n.deps.append((('x', e[i]), self.get(sub1(e, i), f1, m + 1)))
+        elif sum(e) > 0:
+            i = build_dir(e)
+            e1 = sub1(e, i)
+            n = VNode(k, 'bra', i)
+            n.deps.append(('PA', self.get(e1, f, m)))
+            n.deps.append(('WP', self.get(e1, f, m + 1)))
+            if e1[i] > 0:
+                e2 = sub1(e1, i)
+                n.deps.append((('e', e1[i]), self.get(e2, f, m)))
+                n.deps.append((('er', e1[i]), self.get(e2, f, m + 1)))
+        else:
+            n = VNode(k, 'base')
+        self.nodes[k] = n
+        return k
+
+    def order(self, targets):
+        """DFS post-order over the sub-graph reaching `targets`; children are
+        visited deepest-first, which keeps the live set small."""
+        depth = {}
+
+        def d(k):
+            if k in depth:
+                return depth[k]
+            n = self.nodes[k]
+            depth[k] = 1 + max((d(c) for _, c in n.deps), default=0)
+            return depth[k]
+
+        seen = set()
+        out = []
+        sys.setrecursionlimit(1000000)
+
+        def visit(k):
+            if k in seen:
+                return
+            n = self.nodes[k]
+            for _, c in sorted(n.deps, key=lambda x: -d(x[1])):
+                visit(c)
+            seen.add(k)
+            out.append(k)
+
+        for t in targets:
+            visit(t)
+        return out
+
+
+def peak_live(order, graph, targets):
+    tset = set(targets)
+    last = {}
+    for idx, k in enumerate(order):
+        for _, c in graph.nodes[k].deps:
+            last[c] = idx
+    cur = set()
+    peak = 0
+    for idx, k in enumerate(order):
+        cur.add(k)
+        for _, c in graph.nodes[k].deps:
+            if last.get(c) == idx:
+                cur.discard(c)
+        if k not in last:
+            cur.discard(k)
+        peak = max(peak, len(cur))
+    return peak
+
+
+# ---------------------------------------------------------------------------
+# class description
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class ClassPlan:
+    la: int
+    lb: int
+    lc: int
+    ld: int
+    ecomps: list
+    fcomps: list
+    groups: list            # list of lists of e indices
+    group_nodes: list       # node count per group
+    group_peak: list
+
+    @property
+    def L(self):
+        return self.la + self.lb + self.lc + self.ld
+
+    @property
+    def name(self):
+        return f"{self.la}{self.lb}{self.lc}{self.ld}"
+
+    @property
+    def ncol_cart(self):
+        return ncart(self.lc) * ncart(self.ld)
+
+
+def plan_class(la, lb, lc, ld, budget=96):
+    ecomps = [c for l in range(la, la + lb + 1) for c in cart(l)]
+    fcomps = [c for l in range(lc, lc + ld + 1) for c in cart(l)]
+    nf = len(fcomps)
+    g = VrrGraph()
+    # greedy grouping of bra components; components sorted so that groups
+    # share sub-components
+    order_e = sorted(range(len(ecomps)), key=lambda i: ecomps[i], reverse=True)
+    groups = []
+    cur = []
+
+    def cost(es):
+        tg = [g.get(ecomps[i], f, 0) for i in es for f in fcomps]
+        o = g.order(tg)
+        return len(o), peak_live(o, g, tg) + len(tg)
+
+    for i in order_e:
+        trial = cur + [i]
+        n, pk = cost(trial)
+        if cur and pk > budget:
+            groups.append(cur)
+            cur = [i]
+        else:
+            cur = trial
+    if cur:
+        groups.append(cur)
+    gn, gp = [], []
+    for grp in groups:
+        n, pk = cost(grp)
+        gn.append(n)
+        gp.append(pk)
+    return ClassPlan(la, lb, lc, ld, ecomps, fcomps, groups, gn, gp)
+
+
+# ---------------------------------------------------------------------------
+# code emission helpers
+# ---------------------------------------------------------------------------
+
+AX = 'xyz'
+
+
+def fmt(v):
+    r = repr(float(v))
+    if 'e' not in r and '.' not in r and 'inf' not in r and 'nan' not in r:
+        r += '.0'
+    return r
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.ind = 0
+
+    def __call__(self, s=''):
+        self.lines.append('  ' * self.ind + s if s else '')
+
+    def block(self, head):
+        self(head + ' {')
+        self.ind += 1
+
+    def end(self, tail=''):
+        self.ind -= 1
+        self('}' + tail)
+
+    def text(self):
+        return '\n'.join(self.lines) + '\n'
+
+
+def vrr_group_order(plan, graph, grp_idx):
+    es = plan.groups[grp_idx]
+    targets = [graph.get(plan.ecomps[i], f, 0) for i in es for f in plan.fcomps]
+    return targets, graph.order(targets)
+
+
+def vrr_coef_names(plan, graph, grp_idx):
+    """Integer multiplicities used by the group's recursion, per kind."""
+    targets, order = vrr_group_order(plan, graph, grp_idx)
+    need = {'e': set(), 'f': set(), 'x': set()}
+    for key in order:
+        for c, _ in graph.nodes[key].deps:
+            if isinstance(c, tuple) and c[0] in ('e', 'f', 'x'):
+                need[c[0]].add(c[1])
+    return need
+
+
+def emit_vrr_coefs(E, need):
+    """Per-primitive multiples k/(2 zeta), k rho/(2 zeta^2), ... so that every
+    recursion term is a single fused multiply-add."""
+    for k in sorted(need['e']):
+        E(f"const double ez{k} = {fmt(k)} * oo2z, ezr{k} = ez{k} * roz;")
+    for k in sorted(need['f']):
+        E(f"const double fe{k} = {fmt(k)} * oo2e, fer{k} = fe{k} * roe;")
+    for k in sorted(need['x']):
+        E(f"const double xz{k} = {fmt(k)} * oo2ze;")
+
+
+def emit_vrr_group(E, plan, graph, grp_idx, acc_names):
+    """Emit the VRR sub-graph for group `grp_idx`; accumulate targets."""
+    targets, order = vrr_group_order(plan, graph, grp_idx)
+    name = {}
+    tpos = {t: k for k, t in enumerate(targets)}
+    for k, key in enumerate(order):
+        n = graph.nodes[key]
+        v = f"v{k}"
+        name[key] = v
+        if n.kind == 'base':
+            expr = f"F[{key[2]}]"
+        elif n.kind == 'bra':
+            i = AX[n.dir]
+            expr = f"PA{i} * {name[n.deps[0][1]]} + WP{i} * {name[n.deps[1][1]]}"
+            if len(n.deps) > 2:
+                (ce, k2), (_, k3) = n.deps[2], n.deps[3]
+                expr += f" + ez{ce[1]} * {name[k2]} - ezr{ce[1]} * {name[k3]}"
+        else:
+            i = AX[n.dir]
+            expr = f"QC{i} * {name[n.deps[0][1]]} + WQ{i} * {name[n.deps[1][1]]}"
+            rest = n.deps[2:]
+            j = 0
+            while j < len(rest):
+                c, kk = rest[j]
+                if c[0] == 'f':
+                    expr += f" + fe{c[1]} * {name[kk]} - fer{c[1]} * {name[rest[j + 1][1]]}"
+                    j += 2
+                else:
+                    expr += f" + xz{c[1]} * {name[kk]}"
+                    j += 1
+        E(f"const double {v} = {expr};")
+        if key in tpos:
+            E(f"{acc_names[tpos[key]]} += {v};")
+    return len(order)
+
+
+def hrr_program(l1, l2):
+    """HRR from blocks e in [l1, l1+l2] to (a in l1, b in l2).
+    Returns (ops, outputs): ops is a list of (dst_key, src0_key, src1_key, dir)
+    in evaluation order; keys are (a, b) with b=(0,0,0) meaning an input."""
+    nodes = {}
+    order = []
+
+    def get(a, b):
+        k = (a, b)
+        if k in nodes:
+            return k
+        if sum(b) == 0:
+            nodes[k] = None
+            return k
+        i = build_dir(b)
+        s0 = get(add1(a, i), sub1(b, i))
+        s1 = get(a, sub1(b, i))
+        nodes[k] = (s0, s1, i)
+        order.append(k)
+        return k
+
+    outs = [get(a, b) for a in cart(l1) for b in cart(l2)]
+    ops = [(k, nodes[k][0], nodes[k][1], nodes[k][2]) for k in order]
+    return ops, outs
+
+
+def emit_hrr(E, l1, l2, inp, prefix, dvec):
+    """inp: dict e-triple -> C expression. Returns list of output expressions
+    (a-major, b-minor, Cartesian order).  dvec: name of the distance vector
+    (AB or CD) components, e.g. 'AB' -> ABx, ABy, ABz."""
+    ops, outs = hrr_program(l1, l2)
+    name = {}
+    for (a, b) in [(a, (0, 0, 0)) for a in inp]:
+        name[(a, b)] = inp[a]
+    for n, (k, s0, s1, i) in enumerate(ops):
+        v = f"{prefix}{n}"
+        E(f"const double {v} = {name[s0]} + {dvec}{AX[i]} * {name[s1]};")
+        name[k] = v
+    return [name[k] for k in outs]
+
+
+def emit_sph_2(E, l1, l2, pure1, pure2, vals, prefix):
+    """vals: Cartesian (a-major, b-minor) expressions; apply solid-harmonic
+    transforms (standard order) on the pure indices.  Returns (n1, n2, list)."""
+    n1c, n2c = ncart(l1), ncart(l2)
+    cur = vals
+    d1, d2 = n1c, n2c
+    cnt = [0]
+
+    def newv(expr):
+        v = f"{prefix}{cnt[0]}"
+        cnt[0] += 1
+        E(f"const double {v} = {expr};")
+        return v
+
+    if pure1 and l1 > 0:
+        rows = sh_rows(l1)
+        nxt = []
+        for r in rows:
+            for j in range(d2):
+                terms = [f"{fmt(c)} * {cur[ic * d2 + j]}" for ic, c in r]
+                nxt.append(newv(' + '.join(terms)))
+        cur = nxt
+        d1 = 2 * l1 + 1
+    if pure2 and l2 > 0:
+        rows = sh_rows(l2)
+        nxt = []
+        for i in range(d1):
+            for r in rows:
+                terms = [f"{fmt(c)} * {cur[i * d2 + ic]}" for ic, c in r]
+                nxt.append(newv(' + '.join(terms)))
+        cur = nxt
+        d2 = 2 * l2 + 1
+    return d1, d2, cur
+
+
+def pure_variants(l1, l2):
+    """Purity combinations that change the code: purity of an s shell is
+    irrelevant."""
+    v1 = [0, 1] if l1 > 0 else [0]
+    v2 = [0, 1] if l2 > 0 else [0]
+    return [(p1, p2) for p1 in v1 for p2 in v2]
+
+
+# ---------------------------------------------------------------------------
+# kernel emission
+# ---------------------------------------------------------------------------
+
+
+def emit_class(plan: ClassPlan, block=128):
+    la, lb, lc, ld = plan.la, plan.lb, plan.lc, plan.ld
+    graph = VrrGraph()
+    ne, nf = len(plan.ecomps), len(plan.fcomps)
+    G = len(plan.groups)
+    E = Emitter()
+    E(f"// class ({la}{lb}|{lc}{ld}): L={plan.L}, {ne}x{nf} contracted [e0|f0] targets, "
+      f"{G} target group(s)")
+    E(f"// group sizes (bra components): {[len(g) for g in plan.groups]}")
+    E(f"// group graph nodes: {plan.group_nodes}; estimated peak live doubles: {plan.group_peak}")
+    kname = f"l2g_eri_{plan.name}"
+    E(f"__global__ void __launch_bounds__({block}) {kname}(const L2gEriArgs A)")
+    E("{")
+    E.ind += 1
+    E("const int tid = blockIdx.x * blockDim.x + threadIdx.x;")
+    E("const int nthr = gridDim.x * blockDim.x;")
+    E.block("for (int qi = tid; qi < A.n; qi += nthr)")
+    E("const int bp = A.bra_pair[qi];")
+    E("const int kp = A.ket_pair[qi];")
+    E("const int meta = A.meta[qi];")
+    E("const int pb0 = A.pair_first[bp], npb = A.pair_np[bp];")
+    E("const int pk0 = A.pair_first[kp], npk = A.pair_np[kp];")
+    E("const double ABx = A.pair_AB[3 * bp], ABy = A.pair_AB[3 * bp + 1], ABz = A.pair_AB[3 * bp + 2];")
+    E("const double CDx = A.pair_AB[3 * kp], CDy = A.pair_AB[3 * kp + 1], CDz = A.pair_AB[3 * kp + 2];")
+    E(f"const double* __restrict__ boysL = A.boys + (size_t){plan.L} * L2G_BOYS_NINT * 8;")
+    E("const double npbk = (double)npb * (double)npk;")
+    E("(void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz; (void)npbk;")
+    E("// lane-private scratch column; the empty asm keeps the compiler from")
+    E("// hoisting one address per row/column out of the quartet loop")
+    E("double* yb = A.ybuf + tid;")
+    E("asm volatile(\"\" : \"+v\"(yb));")
+    E("(void)yb;")
+
+    # storage of ket-transformed rows: registers when the whole target fits
+    # one group and the row block is small, else a lane-private HBM column
+    ncol_max = ncart(lc) * ncart(ld)
+    in_regs = (G == 1) and ne * ncol_max <= 48
+    if in_regs:
+        E(f"double Y[{ne}][{ncol_max}];")
+    E("int nsurv = 0;  // primitive quartets that pass screening")
+    for gi, grp in enumerate(plan.groups):
+        accs = [f"acc{gi}_{t}" for t in range(len(grp) * nf)]
+        E(f"// ---- target group {gi}: bra components {[plan.ecomps[i] for i in grp]}")
+        E.block("")
+        for a in accs:
+            E(f"double {a} = 0.0;")
+        E.block("for (int ib = 0; ib < npb; ++ib)")
+        E("const double4 Pg = A.pp_P[pb0 + ib];")
+        E("const double4 PAk = A.pp_PA[pb0 + ib];")
+        E("const double2 xb = A.pp_aux[pb0 + ib];")
+        E.block("for (int ik = 0; ik < npk; ++ik)")
+        E("const double2 xk = A.pp_aux[pk0 + ik];")
+        E("if (!(xb.x + xk.x > A.ln_precision)) continue;")
+        E("const double4 Qg = A.pp_P[pk0 + ik];")
+        E("const double4 QCk = A.pp_PA[pk0 + ik];")
+        E("const double gp = Pg.w, gq = Qg.w;")
+        E("const double rs = rsqrt(gp + gq);")
+        E("const double oos = rs * rs;")
+        E("const double pfac = PAk.w * QCk.w * rs * A.scale;")
+        E.block("if (A.method == L2G_SCREEN_ORIGINAL)")
+        E("if (fabs(pfac) < A.precision) continue;")
+        E.end()
+        E.block("else if (A.method == L2G_SCREEN_CONSERVATIVE)")
+        E("const double nsk = A.pp_nonsph[pk0 + ik], nsb = A.pp_nonsph[pb0 + ib];")
+        E("if (fabs(pfac) * fmax(1.0, nsb * nsk) * npbk < A.precision) continue;")
+        E.end()
+        if gi == 0:
+            E("++nsurv;")
+        E("const double PQx = Pg.x - Qg.x, PQy = Pg.y - Qg.y, PQz = Pg.z - Qg.z;")
+        E("const double T = gp * gq * oos * (PQx * PQx + PQy * PQy + PQz * PQz);")
+        E(f"double F[{plan.L + 1}];")
+        E(f"boys_fm<{plan.L}>(T, boysL, pfac, F);")
+        if plan.L > 0:
+            E("const double PAx = PAk.x, PAy = PAk.y, PAz = PAk.z;")
+            E("const double QCx = QCk.x, QCy = QCk.y, QCz = QCk.z;")
+            E("const double goq = gq * oos, gop = gp * oos;")
+            E("const double WPx = -goq * PQx, WPy = -goq * PQy, WPz = -goq * PQz;")
+            E("const double WQx = gop * PQx, WQy = gop * PQy, WQz = gop * PQz;")
+            E("const double oo2z = xb.y, oo2e = xk.y, oo2ze = 0.5 * oos;")
+            E("const double roz = goq, roe = gop;")
+            E("(void)PAx; (void)PAy; (void)PAz; (void)QCx; (void)QCy; (void)QCz;")
+            E("(void)WPx; (void)WPy; (void)WPz; (void)WQx; (void)WQy; (void)WQz;")
+            E("(void)oo2z; (void)oo2e; (void)oo2ze; (void)roz; (void)roe;")
+            emit_vrr_coefs(E, vrr_coef_names(plan, graph, gi))
+        emit_vrr_group(E, plan, graph, gi, accs)
+        E.end()  # ik
+        E.end()  # ib
+        # ket HRR + ket spherical transform for each row of the group
+        for r, ie in enumerate(grp):
+            E.block("")
+            inp = {f: accs[r * nf + jf] for jf, f in enumerate(plan.fcomps)}
+            cvals = emit_hrr(E, lc, ld, inp, "h", "CD")
+            E.block("switch ((meta >> 5) & 3)")
+            for (pc, pd) in pure_variants(lc, ld):
+                E.block(f"case {pc | (pd << 1)}:")
+                d1, d2, vals = emit_sph_2(E, lc, ld, pc, pd, cvals, "k")
+                for col, v in enumerate(vals):
+                    if in_regs:
+                        E(f"Y[{ie}][{col}] = {v};")
+                    else:
+                        E(f"yb[(size_t)({ie * ncol_max + col}) * A.ystride] = {v};")
+                E("break;")
+                E.end()
+            E("default: break;")
+            E.end()
+            E.end()
+        E.end()  # group block
+    # bra HRR + bra sph + permuted store, column by column over the ket
+    E("if (A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
+    E("// ---- bra HRR, bra spherical transform and permuted store")
+    E(f"const L2gStore st = make_store(meta, A, qi, {la}, {lb}, {lc}, {ld});")
+    E("const int ncol = st.nk1 * st.nk2;")
+
+    def bra_column(colexpr, ysrc):
+        E("const int ic = (" + colexpr + ") / st.nk2, id = (" + colexpr + ") - ic * st.nk2;")
+        E(f"const size_t colofs = st.base + (size_t)st.pos3(ic) * st.s3 + (size_t)st.pos4(id) * st.s4;")
+        E("const double colscale = st.uniform ? st.scale3(ic) * st.scale4(id) : 1.0;")
+        inp = {e: ysrc(ie) for ie, e in enumerate(plan.ecomps)}
+        bvals = emit_hrr(E, la, lb, inp, "g", "AB")
+        E.block("switch ((meta >> 3) & 3)")
+        for (pa, pb) in pure_variants(la, lb):
+            E.block(f"case {pa | (pb << 1)}:")
+            d1, d2, vals = emit_sph_2(E, la, lb, pa, pb, bvals, "b")
+            for i1 in range(d1):
+                for i2 in range(d2):
+                    E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {vals[i1 * d2 + i2]});")
+            E("break;")
+            E.end()
+        E("default: break;")
+        E.end()
+
+    if in_regs:
+        for col in range(ncol_max):
+            E.block(f"if ({col} < ncol)")
+            bra_column(str(col), lambda ie, c=col: f"Y[{ie}][{c}]")
+            E.end()
+    else:
+        E.block("for (int col = 0; col < ncol; ++col)")
+        bra_column("col", lambda ie: f"yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
+        E.end()
+    E.end()  # quartet loop
+    E.ind -= 1
+    E("}")
+    return kname, E.text(), in_regs
+
+
+def canonical_classes(lmax):
+    out = []
+    pairs = [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+    for (la, lb) in pairs:
+        for (lc, ld) in pairs:
+            if la + lb <= lc + ld:
+                out.append((la, lb, lc, ld))
+    return out
+
+
+def generate(outdir, lmax=2, budget=96, nfiles=8):
+    os.makedirs(outdir, exist_ok=True)
+    classes = canonical_classes(lmax)
+    plans = [plan_class(*c, budget=budget) for c in classes]
+    # distribute classes over files by estimated size (node count)
+    weights = sorted(((sum(p.group_nodes), i) for i, p in enumerate(plans)), reverse=True)
+    files = [[] for _ in range(nfiles)]
+    load = [0] * nfiles
+    for w, i in weights:
+        j = load.index(min(load))
+        files[j].append(i)
+        load[j] += w
+    table = []
+    written = []
+    for fi, idxs in enumerate(files):
+        if not idxs:
+            continue
+        fn = os.path.join(outdir, f"eri_gen_{fi}.hip")
+        parts = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
+                 '#include "../eri_device.h"', '']
+        for i in sorted(idxs):
+            p = plans[i]
+            kname, code, in_regs = emit_class(p)
+            parts.append(code)
+            parts.append(f"void {kname}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s) {{")
+            parts.append(f"  hipLaunchKernelGGL({kname}, dim3(grid), dim3(block), 0, s, A);")
+            parts.append("}")
+            parts.append("")
+            table.append((p, kname, in_regs))
+        new = '\n'.join(parts)
+        if not (os.path.exists(fn) and open(fn).read() == new):
+            with open(fn, 'w') as f:
+                f.write(new)
+        written.append(fn)
+    # dispatch table
+    lines = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
+             '#include "../eri_device.h"', '']
+    for p, k, _ in table:
+        lines.append(f"void {k}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s);")
+    lines.append('')
+    lines.append(f"extern const int l2g_gen_lmax = {lmax};")
+    lines.append("extern const L2gClassInfo l2g_gen_classes[] = {")
+    for p, k, in_regs in table:
+        ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld)
+        lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
+                     f"{sum(p.group_nodes)}, &{k}_launch}},")
+    lines.append("};")
+    lines.append(f"extern const int l2g_gen_nclasses = {len(table)};")
+    fn = os.path.join(outdir, "eri_table.cpp")
+    new = '\n'.join(lines) + '\n'
+    if not (os.path.exists(fn) and open(fn).read() == new):
+        with open(fn, 'w') as f:
+            f.write(new)
+    written.append(fn)
+    return written, plans
+
+
+if __name__ == '__main__':
+    out = sys.argv[1]
+    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    budget = int(sys.argv[3]) if len(sys.argv) > 3 else 96
+    files, plans = generate(out, lmax, budget)
+    for p in plans:
+        print(p.name, 'groups', len(p.groups), 'nodes', sum(p.group_nodes), 'peak', max(p.group_peak))

@@ -1,0 +1,194 @@
+// Device-side definitions shared by the generated ERI kernels and the host
+// launcher: kernel argument block, Boys function, permuted store.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define L2G_BOYS_NINT 820   // 7 intervals per unit T on [0, 117 + 1/7)
+#define L2G_BOYS_MMAX 40    // tabulated m range, as FmEval_Chebyshev7 (boys.h:298-306)
+#define L2G_BOYS_TCRIT 117.0
+
+#define L2G_SCREEN_ORIGINAL 0x0001
+#define L2G_SCREEN_CONSERVATIVE 0x0010
+
+// Work list of one canonical class plus everything a kernel reads.
+// Primitive-pair records are stored in the canonical orientation of the pair
+// (first shell = the one with the larger angular momentum):
+//   pp_P   = (Px, Py, Pz, gamma)                 gamma = alpha1 + alpha2
+//   pp_PA  = (PAx, PAy, PAz, Kc)                 Kc = K * c1 * c2,
+//            K = sqrt(2) pi^(5/4) exp(-rho |AB|^2) / gamma (shell.h:1244)
+//   pp_aux = (ln_scr, 1 / (2 gamma))
+//   pp_nonsph = conservative-screening factor (shell.h:1208-1227)
+struct L2gEriArgs {
+  const int* __restrict__ bra_pair;
+  const int* __restrict__ ket_pair;
+  const long long* __restrict__ out_off;
+  const int* __restrict__ meta;  // bit0 swap_tbra, bit1 swap_tket, bit2 swap_braket,
+                                 // bits3..6 pure flags of canonical b1,b2,k1,k2
+  int n;
+  const int* __restrict__ pair_first;
+  const int* __restrict__ pair_np;
+  const double* __restrict__ pair_AB;
+  const double4* __restrict__ pp_P;
+  const double4* __restrict__ pp_PA;
+  const double2* __restrict__ pp_aux;
+  const double* __restrict__ pp_nonsph;
+  const double* __restrict__ boys;  // [m][iv][8]
+  double precision, ln_precision, scale;
+  int method;
+  int shg_gaussian;
+  int uniform_cart;
+  double* __restrict__ out;
+  double* __restrict__ ybuf;
+  long long ystride;
+  signed char* __restrict__ nz;          // optional: 0 if every primitive was screened out
+  const long long* __restrict__ qidx;    // caller index of each work item (for nz)
+};
+
+typedef void (*L2gLaunchFn)(const L2gEriArgs&, int grid, int block, hipStream_t s);
+
+struct L2gClassInfo {
+  int la, lb, lc, ld;
+  int ngroups;
+  int ybuf_doubles;  // per lane; 0 = rows kept in registers
+  int nodes;         // VRR graph nodes per primitive quartet (all groups)
+  L2gLaunchFn launch;
+};
+
+// F_m(T) * pfac for m = 0..L.  Below T_crit: degree-7 Chebyshev interpolant
+// of F_L on the interval holding T, then downward recursion
+// F_{m-1} = (2T F_m + e^{-T}) / (2m - 1).  Above T_crit: the asymptotic
+// F_0 = sqrt(pi/(4T)) and upward recursion without the e^{-T} term, as
+// FmEval_Chebyshev7::eval does (boys.h:348-358).
+template <int L>
+__device__ __forceinline__ void boys_fm(double T, const double* __restrict__ tabL, double pfac, double* F) {
+  if (T > L2G_BOYS_TCRIT) {
+    const double oox = 1.0 / T;
+    double f = 0.88622692545275801365 * sqrt(oox);
+    F[0] = f * pfac;
+#pragma unroll
+    for (int m = 1; m <= L; ++m) {
+      f = f * ((double)m - 0.5) * oox;
+      F[m] = f * pfac;
+    }
+  } else {
+    const double x7 = T * 7.0;
+    const int iv = (int)x7;
+    const double xd = x7 - (double)iv - 0.5;
+    const double4* c4 = reinterpret_cast<const double4*>(tabL + (size_t)iv * 8);
+    const double4 lo = c4[0], hi = c4[1];
+    double f = hi.w;
+    f = fma(f, xd, hi.z);
+    f = fma(f, xd, hi.y);
+    f = fma(f, xd, hi.x);
+    f = fma(f, xd, lo.w);
+    f = fma(f, xd, lo.z);
+    f = fma(f, xd, lo.y);
+    f = fma(f, xd, lo.x);
+    double fm[L + 1];
+    fm[L] = f;
+    if (L > 0) {
+      const double e = exp(-T);
+      const double t2 = T + T;
+#pragma unroll
+      for (int m = L; m > 0; --m) fm[m - 1] = (t2 * fm[m] + e) * (1.0 / (double)(2 * m - 1));
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) F[m] = fm[m] * pfac;
+  }
+}
+
+__device__ __forceinline__ int l2g_ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// sqrt((2l-1)!! / ((2i-1)!! (2j-1)!! (2k-1)!!)) for Cartesian index c of
+// level l (cartesian.h:96-109); used only with uniform normalization.
+__device__ __forceinline__ double l2g_uniform_cart_scale(int l, int c) {
+  // components in standard order; compute exponents from the index
+  int x = l, rem = c;
+  while (rem > l - x) {
+    rem -= (l - x + 1);
+    --x;
+  }
+  const int y = l - x - rem, z = l - x - y;
+  double num = 1, den = 1;
+  for (int i = 2 * l - 1; i > 1; i -= 2) num *= i;
+  for (int i = 2 * x - 1; i > 1; i -= 2) den *= i;
+  for (int i = 2 * y - 1; i > 1; i -= 2) den *= i;
+  for (int i = 2 * z - 1; i > 1; i -= 2) den *= i;
+  return sqrt(num / den);
+}
+
+// Caller-order store of one quartet.  Canonical index k (b1,b2,k1,k2) lands
+// at caller position pos[k] (engine.impl.h:1181-1208 swaps undone as in
+// :1987-2067); within a pure shell, standard index s (m = s - l) is moved to
+// the Gaussian position when that ordering is selected (shgshell_ordering.h).
+struct L2gStore {
+  size_t base;
+  int nk1, nk2;
+  int s1, s2, s3, s4;
+  int l1, l2, l3, l4;
+  int p1, p2, p3, p4;
+  int gauss, uniform;
+
+  __device__ __forceinline__ int pos(int l, int pure, int s) const {
+    if (!pure || !gauss) return s;
+    const int m = s - l;
+    return m == 0 ? 0 : (m > 0 ? 2 * m - 1 : -2 * m);
+  }
+  __device__ __forceinline__ int pos3(int i) const { return pos(l3, p3, i); }
+  __device__ __forceinline__ int pos4(int i) const { return pos(l4, p4, i); }
+  __device__ __forceinline__ double scale3(int i) const { return p3 ? 1.0 : l2g_uniform_cart_scale(l3, i); }
+  __device__ __forceinline__ double scale4(int i) const { return p4 ? 1.0 : l2g_uniform_cart_scale(l4, i); }
+  __device__ __forceinline__ void put(double* __restrict__ out, size_t colofs, int i1, int i2, double colscale,
+                                      double v) const {
+    double val = v;
+    if (uniform) {
+      val *= colscale * (p1 ? 1.0 : l2g_uniform_cart_scale(l1, i1)) * (p2 ? 1.0 : l2g_uniform_cart_scale(l2, i2));
+    }
+    out[colofs + (size_t)pos(l1, p1, i1) * s1 + (size_t)pos(l2, p2, i2) * s2] = val;
+  }
+};
+
+__device__ __forceinline__ L2gStore make_store(int meta, const L2gEriArgs& A, int qi, int l1, int l2, int l3,
+                                               int l4) {
+  L2gStore st;
+  st.base = (size_t)A.out_off[qi];
+  st.l1 = l1, st.l2 = l2, st.l3 = l3, st.l4 = l4;
+  st.p1 = (meta >> 3) & 1;
+  st.p2 = (meta >> 4) & 1;
+  st.p3 = (meta >> 5) & 1;
+  st.p4 = (meta >> 6) & 1;
+  const int n1 = st.p1 ? 2 * l1 + 1 : l2g_ncart(l1);
+  const int n2 = st.p2 ? 2 * l2 + 1 : l2g_ncart(l2);
+  const int n3 = st.p3 ? 2 * l3 + 1 : l2g_ncart(l3);
+  const int n4 = st.p4 ? 2 * l4 + 1 : l2g_ncart(l4);
+  st.nk1 = n3;
+  st.nk2 = n4;
+  const int swap_tbra = meta & 1, swap_tket = (meta >> 1) & 1, swap_braket = (meta >> 2) & 1;
+  int pos[4];
+  if (!swap_braket) {
+    pos[0] = swap_tbra ? 1 : 0;
+    pos[1] = swap_tbra ? 0 : 1;
+    pos[2] = swap_tket ? 3 : 2;
+    pos[3] = swap_tket ? 2 : 3;
+  } else {
+    pos[0] = swap_tket ? 3 : 2;
+    pos[1] = swap_tket ? 2 : 3;
+    pos[2] = swap_tbra ? 1 : 0;
+    pos[3] = swap_tbra ? 0 : 1;
+  }
+  int csz[4];
+  csz[pos[0]] = n1;
+  csz[pos[1]] = n2;
+  csz[pos[2]] = n3;
+  csz[pos[3]] = n4;
+  const int cs[4] = {csz[1] * csz[2] * csz[3], csz[2] * csz[3], csz[3], 1};
+  st.s1 = cs[pos[0]];
+  st.s2 = cs[pos[1]];
+  st.s3 = cs[pos[2]];
+  st.s4 = cs[pos[3]];
+  st.gauss = A.shg_gaussian;
+  st.uniform = A.uniform_cart;
+  return st;
+}

@@ -1,0 +1,718 @@
+// Host side of the C ABI declared in include/l2g.h: device context, Boys
+// table, basis upload, shell-pair data, batch planning and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/l2g.h"
+#include "eri_device.h"
+
+extern const L2gClassInfo l2g_gen_classes[];
+extern const int l2g_gen_nclasses;
+extern const int l2g_gen_lmax;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct L2gError : std::runtime_error {
+  int code;
+  L2gError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess)                                                                           \
+      throw L2gError(e_ == hipErrorOutOfMemory ? L2G_E_OOM : L2G_E_NODEVICE,                      \
+                     std::string(#x) + ": " + hipGetErrorString(e_));                              \
+  } while (0)
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return L2G_OK;
+  } catch (const L2gError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return L2G_E_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return L2G_E_INVALID;
+  }
+}
+
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+int nfun(int l, int pure) { return pure ? 2 * l + 1 : ncart(l); }
+
+// ---------------------------------------------------------------------------
+// Boys table: for every interval [iv/7, (iv+1)/7] and m, the degree-7
+// polynomial in xd = 7T - iv - 1/2 that interpolates F_m(T) at the 8
+// Chebyshev points of the interval.  F_m at the points comes from the
+// Maclaurin series of F_40 (all terms positive, long double) and downward
+// recursion.  Same construction as FmEval_Chebyshev7's table
+// (include/libint2/boys.h:252-482, parameters in
+// include/libint2/util/boys_cheb.nb: delta = 1/7, order 7, m <= 40).
+// ---------------------------------------------------------------------------
+struct BoysTable {
+  std::vector<double> t;  // [m][iv][8]
+  BoysTable() {
+    const int NI = L2G_BOYS_NINT, MM = L2G_BOYS_MMAX, NC = 8;
+    t.assign((size_t)(MM + 1) * NI * NC, 0.0);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    long double node[NC];
+    for (int k = 0; k < NC; ++k) node[k] = cosl(pi * (2 * k + 1) / (2.0L * NC));  // in [-1, 1]
+    // Chebyshev polynomial monomial coefficients, T_j(u) = sum_i P[j][i] u^i
+    long double P[NC][NC];
+    std::memset(P, 0, sizeof(P));
+    P[0][0] = 1;
+    P[1][1] = 1;
+    for (int j = 2; j < NC; ++j)
+      for (int i = 0; i < NC; ++i) P[j][i] = (i ? 2 * P[j - 1][i - 1] : 0.0L) - P[j - 2][i];
+    long double vals[MM + 1][NC];
+    for (int iv = 0; iv < NI; ++iv) {
+      for (int k = 0; k < NC; ++k) {
+        const long double T = ((long double)iv + 0.5L + 0.5L * node[k]) / 7.0L;
+        // F_MM(T) = e^{-T} sum_i T^i / prod_{j=0..i} (MM + j + 1/2) / 2
+        long double term = 1.0L / (MM + 0.5L), sum = term;
+        for (int i = 1; i < 2000; ++i) {
+          term *= T / (MM + i + 0.5L);
+          sum += term;
+          if (term < sum * 1e-21L) break;
+        }
+        const long double eT = expl(-T);
+        long double f = 0.5L * eT * sum;
+        vals[MM][k] = f;
+        for (int m = MM; m > 0; --m) {
+          f = (2 * T * f + eT) / (2 * m - 1);
+          vals[m - 1][k] = f;
+        }
+      }
+      for (int m = 0; m <= MM; ++m) {
+        long double c[NC];
+        for (int j = 0; j < NC; ++j) {
+          long double s = 0;
+          for (int k = 0; k < NC; ++k) s += vals[m][k] * cosl(pi * j * (2 * k + 1) / (2.0L * NC));
+          c[j] = s * (j ? 2.0L : 1.0L) / NC;
+        }
+        double* out = &t[((size_t)m * NI + iv) * NC];
+        long double scale = 1;  // u = 2 xd
+        for (int i = 0; i < NC; ++i) {
+          long double s = 0;
+          for (int j = 0; j < NC; ++j) s += c[j] * P[j][i];
+          out[i] = (double)(s * scale);
+          scale *= 2;
+        }
+      }
+    }
+  }
+};
+const BoysTable& boys_table() {
+  static BoysTable tab;
+  return tab;
+}
+
+// ---------------------------------------------------------------------------
+// normalization and solid harmonics
+// ---------------------------------------------------------------------------
+double double_factorial_km1(int k) {  // (k-1)!!
+  double r = 1;
+  for (int i = k - 1; i > 1; i -= 2) r *= i;
+  return r;
+}
+double factorial(int n) {
+  double r = 1;
+  for (int i = 2; i <= n; ++i) r *= i;
+  return r;
+}
+double choose(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  return factorial(n) / (factorial(k) * factorial(n - k));
+}
+int sgn_parity(int i) { return (i & 1) ? -1 : 1; }
+
+// ---------------------------------------------------------------------------
+// objects behind the opaque handles
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p && bytes >= b) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return;
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(std::max<size_t>(h.size() * sizeof(T), 16));
+  if (!h.empty()) HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+struct l2g_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf boys;
+  int shg_gaussian = 0;
+  int ncu = 256;
+  int class_of[8][8][8][8];
+};
+
+struct l2g_basis_s {
+  l2g_context_s* ctx;
+  int nshell;
+  std::vector<int> l, pure, nprim, off;
+  std::vector<double> alpha, coeff, lnc, origin;
+};
+
+struct ClassWork {
+  int cls;  // index into l2g_gen_classes
+  int64_t first, count;
+  int64_t nprimq;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float ms = 0;
+};
+
+struct l2g_eri_plan_s {
+  l2g_basis_s* basis;
+  int64_t nq = 0;
+  int64_t total = 0;
+  std::vector<int64_t> offsets;
+  std::vector<ClassWork> classes;
+  double precision, ln_precision, scale;
+  int method, uniform;
+  // device arrays
+  DevBuf bra, ket, off, meta, qidx;
+  DevBuf pair_first, pair_np, pair_AB, ppP, ppPA, ppAux, ppNs;
+  DevBuf out, ybuf, nz;
+  int64_t ystride = 0;
+  int grid = 0, block = 128;
+  ~l2g_eri_plan_s() {
+    for (auto& c : classes) {
+      if (c.ev0) (void)hipEventDestroy(c.ev0);
+      if (c.ev1) (void)hipEventDestroy(c.ev1);
+    }
+  }
+};
+
+namespace {
+
+// Shell-pair data for ordered pair (i, j), restating ShellPair::init
+// (include/libint2/shell.h:1137-1256) for Original and Conservative
+// screening, with the pair's contraction coefficients folded into K.
+struct PairBuilder {
+  const l2g_basis_s& B;
+  double ln_prec;
+  int method;
+  std::vector<int> first, np;
+  std::vector<double> AB;
+  std::vector<double4> P, PA;
+  std::vector<double2> aux;
+  std::vector<double> ns;
+  std::unordered_map<uint64_t, int> id;
+
+  PairBuilder(const l2g_basis_s& b, double lp, int m) : B(b), ln_prec(lp), method(m) {}
+
+  int get(int i, int j) {
+    const uint64_t key = (uint64_t)(uint32_t)i << 32 | (uint32_t)j;
+    auto it = id.find(key);
+    if (it != id.end()) return it->second;
+    const int pid = (int)first.size();
+    id.emplace(key, pid);
+    first.push_back((int)P.size());
+    const double* A = &B.origin[3 * i];
+    const double* Bo = &B.origin[3 * j];
+    double ab[3], AB2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      ab[k] = A[k] - Bo[k];
+      AB2 += ab[k] * ab[k];
+      AB.push_back(ab[k]);
+    }
+    const int n1 = B.nprim[i], n2 = B.nprim[j];
+    const int l1 = B.l[i], l2 = B.l[j];
+    int cnt = 0;
+    for (int p1 = 0; p1 < n1; ++p1)
+      for (int p2 = 0; p2 < n2; ++p2) {
+        const double a1 = B.alpha[B.off[i] + p1], a2 = B.alpha[B.off[j] + p2];
+        const double gamma = a1 + a2, oog = 1.0 / gamma;
+        const double rho = a1 * a2 * oog;
+        const double mr = -rho * AB2;
+        double ln_scr = mr + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
+        if (method == L2G_SCREEN_ORIGINAL && ln_scr < ln_prec) continue;
+        double Pp[3];
+        for (int k = 0; k < 3; ++k) Pp[k] = (AB2 == 0.0) ? A[k] : (a1 * A[k] + a2 * Bo[k]) * oog;
+        double nonsph = 0;
+        if (method == L2G_SCREEN_CONSERVATIVE) {
+          double mpa = 0, mpb = 0;
+          for (int k = 0; k < 3; ++k) {
+            mpa = std::max(mpa, std::fabs(Pp[k] - A[k]));
+            mpb = std::max(mpb, std::fabs(Pp[k] - Bo[k]));
+          }
+          nonsph = std::max(std::pow(mpa, l1) * std::pow(mpb, l2),
+                            factorial(l1) * factorial(l2) * std::pow(oog, l1 + l2));
+          ln_scr += 1.777485947591722872387900 + std::log(oog) + std::log(std::max(nonsph, 1.0)) +
+                    std::log((double)(n1 * n2));
+          if (ln_scr < ln_prec) continue;
+        }
+        const double K = 5.9149671727956128778 * std::exp(mr) * oog;
+        const double c = B.coeff[B.off[i] + p1] * B.coeff[B.off[j] + p2];
+        P.push_back(double4{Pp[0], Pp[1], Pp[2], gamma});
+        PA.push_back(double4{Pp[0] - A[0], Pp[1] - A[1], Pp[2] - A[2], K * c});
+        aux.push_back(double2{ln_scr, 0.5 * oog});
+        ns.push_back(nonsph);
+        ++cnt;
+      }
+    np.push_back(cnt);
+    return pid;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* l2g_version(void) { return "libint_amd 0.1 (gfx950)"; }
+
+const char* l2g_last_error(void) { return g_err.c_str(); }
+
+int l2g_max_l(void) { return l2g_gen_lmax; }
+
+int l2g_shell_renorm(int l, int nprim, const double* alpha, double* coeff, double* max_ln_coeff, int enforce_unit) {
+  return guarded([&] {
+    if (l < 0 || nprim <= 0 || !alpha || !coeff) throw std::invalid_argument("l2g_shell_renorm: bad arguments");
+    const double pi32 = 5.56832799683170784528481798212;  // pi^(3/2)
+    const double dfl = double_factorial_km1(2 * l);
+    for (int p = 0; p < nprim; ++p) {
+      if (alpha[p] < 0) throw std::invalid_argument("l2g_shell_renorm: negative exponent");
+      if (alpha[p] != 0) {
+        const double ta = 2 * alpha[p];
+        coeff[p] *= std::sqrt(std::pow(2.0, l) * std::pow(ta, l + 1) * std::sqrt(ta) / (pi32 * dfl));
+      }
+    }
+    if (enforce_unit) {
+      double s = 0;
+      for (int p = 0; p < nprim; ++p)
+        for (int q = 0; q <= p; ++q) {
+          const double g = alpha[p] + alpha[q];
+          s += (p == q ? 1 : 2) * dfl * pi32 * coeff[p] * coeff[q] / (std::pow(2.0, l) * std::pow(g, l + 1) * std::sqrt(g));
+        }
+      const double f = 1.0 / std::sqrt(s);
+      for (int p = 0; p < nprim; ++p) coeff[p] *= f;
+    }
+    if (max_ln_coeff)
+      for (int p = 0; p < nprim; ++p) max_ln_coeff[p] = std::log(std::fabs(coeff[p]));
+  });
+}
+
+double l2g_solid_harmonics_coeff(int l, int m, int lx, int ly, int lz) {
+  const int am = std::abs(m);
+  if ((lx + ly - am) % 2) return 0.0;
+  const int j = (lx + ly - am) / 2;
+  if (j < 0) return 0.0;
+  const int comp = (m >= 0) ? 1 : -1;
+  const int i = am - lx;
+  if (comp != sgn_parity(std::abs(i))) return 0.0;
+  double pfac = std::sqrt(((factorial(2 * lx) * factorial(2 * ly) * factorial(2 * lz)) / factorial(2 * l)) *
+                          (factorial(l - am) / factorial(l)) * (1.0 / factorial(l + am)) *
+                          (1.0 / (factorial(lx) * factorial(ly) * factorial(lz))));
+  pfac /= (double)(1L << l);
+  pfac *= (m < 0) ? sgn_parity((i - 1) / 2) : sgn_parity(i / 2);
+  double sum = 0;
+  for (int ii = j; ii <= (l - am) / 2; ++ii) {
+    double t = choose(l, ii) * choose(ii, j) * (sgn_parity(ii) * factorial(2 * (l - ii))) / factorial(l - am - 2 * ii);
+    double s1 = 0;
+    for (int k = std::max((lx - am) / 2, 0); k <= std::min(j, lx / 2); ++k)
+      if (lx - 2 * k <= am) s1 += choose(j, k) * choose(am, lx - 2 * k) * sgn_parity(k);
+    sum += t * s1;
+  }
+  sum *= std::sqrt(double_factorial_km1(2 * l) /
+                   (double_factorial_km1(2 * lx) * double_factorial_km1(2 * ly) * double_factorial_km1(2 * lz)));
+  return (m == 0) ? pfac * sum : std::sqrt(2.0) * pfac * sum;
+}
+
+int l2g_boys_table(double* out, int* nint, int* mmax) {
+  return guarded([&] {
+    if (nint) *nint = L2G_BOYS_NINT;
+    if (mmax) *mmax = L2G_BOYS_MMAX;
+    if (out) {
+      const auto& t = boys_table().t;
+      std::memcpy(out, t.data(), t.size() * sizeof(double));
+    }
+  });
+}
+
+int l2g_init(int device, l2g_context* out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("l2g_init: null output");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw L2gError(L2G_E_NODEVICE, "no HIP device available");
+    if (device < 0 || device >= n) throw std::invalid_argument("l2g_init: device index out of range");
+    auto ctx = std::make_unique<l2g_context_s>();
+    ctx->device = device;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    ctx->ncu = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    const auto& t = boys_table().t;
+    ctx->boys.alloc(t.size() * sizeof(double));
+    HIPCHK(hipMemcpy(ctx->boys.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (auto& a : ctx->class_of)
+      for (auto& b : a)
+        for (auto& c : b)
+          for (auto& d : c) d = -1;
+    for (int i = 0; i < l2g_gen_nclasses; ++i) {
+      const auto& c = l2g_gen_classes[i];
+      ctx->class_of[c.la][c.lb][c.lc][c.ld] = i;
+    }
+    *out = ctx.release();
+  });
+}
+
+int l2g_finalize(l2g_context ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int l2g_set_solid_harmonics_ordering(l2g_context ctx, int ordering) {
+  return guarded([&] {
+    if (!ctx) throw L2gError(L2G_E_STATE, "null context");
+    if (ordering != L2G_SHG_STANDARD && ordering != L2G_SHG_GAUSSIAN)
+      throw std::invalid_argument("solid harmonics ordering value not recognized");
+    ctx->shg_gaussian = (ordering == L2G_SHG_GAUSSIAN);
+  });
+}
+
+int l2g_basis_create(l2g_context ctx, int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                     const double* coeff, const double* origin, l2g_basis* out) {
+  return guarded([&] {
+    if (!ctx) throw L2gError(L2G_E_STATE, "null context");
+    if (!out || nshell < 0 || (nshell > 0 && (!l || !pure || !nprim || !alpha || !coeff || !origin)))
+      throw std::invalid_argument("l2g_basis_create: bad arguments");
+    auto b = std::make_unique<l2g_basis_s>();
+    b->ctx = ctx;
+    b->nshell = nshell;
+    b->l.assign(l, l + nshell);
+    b->pure.assign(pure, pure + nshell);
+    b->nprim.assign(nprim, nprim + nshell);
+    b->off.resize(nshell + 1);
+    b->off[0] = 0;
+    for (int s = 0; s < nshell; ++s) {
+      if (l[s] < 0) throw std::invalid_argument("negative angular momentum");
+      if (l[s] > l2g_gen_lmax)
+        throw L2gError(L2G_E_LMAX, "angular momentum " + std::to_string(l[s]) + " exceeds the compiled limit " +
+                                       std::to_string(l2g_gen_lmax));
+      if (nprim[s] <= 0) throw std::invalid_argument("shell without primitives");
+      b->off[s + 1] = b->off[s] + nprim[s];
+    }
+    const int np = b->off[nshell];
+    b->alpha.assign(alpha, alpha + np);
+    b->coeff.assign(coeff, coeff + np);
+    b->lnc.resize(np);
+    for (int p = 0; p < np; ++p) b->lnc[p] = std::log(std::fabs(b->coeff[p]));
+    b->origin.assign(origin, origin + 3 * nshell);
+    *out = b.release();
+  });
+}
+
+int l2g_basis_destroy(l2g_basis b) {
+  delete b;
+  return L2G_OK;
+}
+
+int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double scale,
+                        int uniform_cart, l2g_eri_plan* out) {
+  return guarded([&] {
+    if (!B) throw L2gError(L2G_E_STATE, "null basis");
+    if (!out || nq < 0 || (nq > 0 && !quartets)) throw std::invalid_argument("l2g_eri_plan_create: bad arguments");
+    if (screening != L2G_SCREEN_ORIGINAL && screening != L2G_SCREEN_CONSERVATIVE)
+      throw std::invalid_argument(
+          "without precomputed shell pair data only the Original or Conservative screening methods are available");
+    l2g_context_s* ctx = B->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    auto plan = std::make_unique<l2g_eri_plan_s>();
+    plan->basis = B;
+    plan->nq = nq;
+    plan->scale = scale;
+    plan->method = screening;
+    plan->uniform = uniform_cart ? 1 : 0;
+    if (precision <= 0) {
+      plan->precision = 0;
+      plan->ln_precision = std::numeric_limits<double>::lowest();
+    } else {
+      plan->precision = precision;
+      plan->ln_precision = std::log(precision);
+    }
+    PairBuilder pb(*B, plan->ln_precision, screening);
+    // canonicalise every quartet and bucket by class
+    struct Item {
+      int bp, kp, meta;
+      int64_t off, q;
+      int64_t w;
+    };
+    std::vector<std::vector<Item>> buckets(l2g_gen_nclasses);
+    plan->offsets.resize(nq);
+    int64_t total = 0;
+    for (int64_t q = 0; q < nq; ++q) {
+      const int32_t* s = quartets + 4 * q;
+      for (int k = 0; k < 4; ++k)
+        if (s[k] < 0 || s[k] >= B->nshell) throw std::invalid_argument("shell index out of range");
+      const int l1 = B->l[s[0]], l2 = B->l[s[1]], l3 = B->l[s[2]], l4 = B->l[s[3]];
+      // engine.impl.h:1181-1208
+      const bool stb = l1 < l2, stk = l3 < l4, sbk = (l1 + l2) > (l3 + l4);
+      const int b1 = sbk ? (stk ? s[3] : s[2]) : (stb ? s[1] : s[0]);
+      const int b2 = sbk ? (stk ? s[2] : s[3]) : (stb ? s[0] : s[1]);
+      const int k1 = sbk ? (stb ? s[1] : s[0]) : (stk ? s[3] : s[2]);
+      const int k2 = sbk ? (stb ? s[0] : s[1]) : (stk ? s[2] : s[3]);
+      const int cls = ctx->class_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+      if (cls < 0) throw L2gError(L2G_E_LMAX, "no kernel for this angular momentum class");
+      auto purebit = [&](int sh) { return (B->pure[sh] && B->l[sh] > 0) ? 1 : 0; };
+      const int meta = (stb ? 1 : 0) | (stk ? 2 : 0) | (sbk ? 4 : 0) | (purebit(b1) << 3) | (purebit(b2) << 4) |
+                       (purebit(k1) << 5) | (purebit(k2) << 6);
+      Item it;
+      it.bp = pb.get(b1, b2);
+      it.kp = pb.get(k1, k2);
+      it.meta = meta;
+      it.off = total;
+      it.q = q;
+      it.w = (int64_t)pb.np[it.bp] * pb.np[it.kp];
+      plan->offsets[q] = total;
+      total += (int64_t)nfun(l1, B->pure[s[0]]) * nfun(l2, B->pure[s[1]]) * nfun(l3, B->pure[s[2]]) *
+               nfun(l4, B->pure[s[3]]);
+      buckets[cls].push_back(it);
+    }
+    plan->total = total;
+    // flatten: classes in table order, each sorted by contraction depth
+    std::vector<int> bra, ket, meta;
+    std::vector<long long> off, qidx;
+    bra.reserve(nq);
+    ket.reserve(nq);
+    meta.reserve(nq);
+    off.reserve(nq);
+    qidx.reserve(nq);
+    int64_t ymax = 0;
+    for (int c = 0; c < l2g_gen_nclasses; ++c) {
+      auto& v = buckets[c];
+      if (v.empty()) continue;
+      std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
+        if (a.w != b.w) return a.w > b.w;
+        if (a.bp != b.bp) return a.bp < b.bp;
+        return a.kp < b.kp;
+      });
+      ClassWork cw;
+      cw.cls = c;
+      cw.first = (int64_t)bra.size();
+      cw.count = (int64_t)v.size();
+      cw.nprimq = 0;
+      for (const auto& it : v) {
+        bra.push_back(it.bp);
+        ket.push_back(it.kp);
+        meta.push_back(it.meta);
+        off.push_back(it.off);
+        qidx.push_back(it.q);
+        cw.nprimq += it.w;
+      }
+      HIPCHK(hipEventCreate(&cw.ev0));
+      HIPCHK(hipEventCreate(&cw.ev1));
+      plan->classes.push_back(cw);
+      ymax = std::max<int64_t>(ymax, l2g_gen_classes[c].ybuf_doubles);
+    }
+    hipStream_t st = ctx->stream;
+    upload(plan->bra, bra, st);
+    upload(plan->ket, ket, st);
+    upload(plan->meta, meta, st);
+    upload(plan->off, off, st);
+    upload(plan->qidx, qidx, st);
+    upload(plan->pair_first, pb.first, st);
+    upload(plan->pair_np, pb.np, st);
+    upload(plan->pair_AB, pb.AB, st);
+    upload(plan->ppP, pb.P, st);
+    upload(plan->ppPA, pb.PA, st);
+    upload(plan->ppAux, pb.aux, st);
+    upload(plan->ppNs, pb.ns, st);
+    // persistent grid: enough blocks to fill every CU several times over
+    plan->block = 128;
+    const int64_t maxblocks = (int64_t)ctx->ncu * 8;
+    int64_t need = 0;
+    for (const auto& c : plan->classes) need = std::max(need, (c.count + plan->block - 1) / plan->block);
+    plan->grid = (int)std::max<int64_t>(1, std::min(need, maxblocks));
+    plan->ystride = (int64_t)plan->grid * plan->block;
+    plan->ybuf.alloc(std::max<int64_t>(ymax * plan->ystride, 1) * sizeof(double));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = plan.release();
+  });
+}
+
+int l2g_eri_plan_size(l2g_eri_plan p, int64_t* total) {
+  return guarded([&] {
+    if (!p || !total) throw std::invalid_argument("null argument");
+    *total = p->total;
+  });
+}
+
+int l2g_eri_plan_offsets(l2g_eri_plan p, int64_t* offsets) {
+  return guarded([&] {
+    if (!p || !offsets) throw std::invalid_argument("null argument");
+    std::memcpy(offsets, p->offsets.data(), p->offsets.size() * sizeof(int64_t));
+  });
+}
+
+int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    l2g_context_s* ctx = p->basis->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    double* out = d_out;
+    if (!out) {
+      p->out.alloc(std::max<int64_t>(p->total, 1) * sizeof(double));
+      out = p->out.as<double>();
+    }
+    L2gEriArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.pair_first = p->pair_first.as<int>();
+    A.pair_np = p->pair_np.as<int>();
+    A.pair_AB = p->pair_AB.as<double>();
+    A.pp_P = p->ppP.as<double4>();
+    A.pp_PA = p->ppPA.as<double4>();
+    A.pp_aux = p->ppAux.as<double2>();
+    A.pp_nonsph = p->ppNs.as<double>();
+    A.boys = ctx->boys.as<double>();
+    A.precision = p->precision;
+    A.ln_precision = p->ln_precision;
+    A.scale = p->scale;
+    A.method = p->method;
+    A.shg_gaussian = ctx->shg_gaussian;
+    A.uniform_cart = p->uniform;
+    A.out = out;
+    A.ybuf = p->ybuf.as<double>();
+    A.ystride = p->ystride;
+    p->nz.alloc(std::max<int64_t>(p->nq, 1));
+    A.nz = p->nz.as<signed char>();
+    for (auto& c : p->classes) {
+      const L2gClassInfo& ci = l2g_gen_classes[c.cls];
+      A.bra_pair = p->bra.as<int>() + c.first;
+      A.ket_pair = p->ket.as<int>() + c.first;
+      A.meta = p->meta.as<int>() + c.first;
+      A.out_off = p->off.as<long long>() + c.first;
+      A.qidx = p->qidx.as<long long>() + c.first;
+      A.n = (int)c.count;
+      const int grid = (int)std::min<int64_t>(p->grid, (c.count + p->block - 1) / p->block);
+      HIPCHK(hipEventRecord(c.ev0, ctx->stream));
+      ci.launch(A, grid, p->block, ctx->stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c.ev1, ctx->stream));
+    }
+  });
+}
+
+int l2g_synchronize(l2g_context ctx) {
+  return guarded([&] {
+    if (!ctx) throw L2gError(L2G_E_STATE, "null context");
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("kernel execution failed: ") + hipGetErrorString(e));
+  });
+}
+
+int l2g_eri_plan_download(l2g_eri_plan p, double* host_out, int8_t* nonzero) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    l2g_context_s* ctx = p->basis->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("kernel execution failed: ") + hipGetErrorString(e));
+    if (host_out && p->total > 0) {
+      if (!p->out.p) throw L2gError(L2G_E_STATE, "plan has not been executed into its own buffer");
+      HIPCHK(hipMemcpy(host_out, p->out.p, p->total * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (nonzero && p->nq > 0) {
+      if (!p->nz.p) throw L2gError(L2G_E_STATE, "plan has not been executed");
+      HIPCHK(hipMemcpy(nonzero, p->nz.p, p->nq, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int l2g_eri_plan_device_output(l2g_eri_plan p, double** d_out) {
+  return guarded([&] {
+    if (!p || !d_out) throw std::invalid_argument("null argument");
+    p->out.alloc(std::max<int64_t>(p->total, 1) * sizeof(double));
+    *d_out = p->out.as<double>();
+  });
+}
+
+int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* nquart, int64_t* nprimq, float* ms) {
+  return guarded([&] {
+    if (!p || !ncls) throw std::invalid_argument("null argument");
+    *ncls = (int)p->classes.size();
+    for (int i = 0; i < (int)p->classes.size() && i < cap; ++i) {
+      auto& c = p->classes[i];
+      const auto& ci = l2g_gen_classes[c.cls];
+      if (code) code[i] = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld;
+      if (nquart) nquart[i] = c.count;
+      if (nprimq) nprimq[i] = c.nprimq;
+      if (ms) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, c.ev0, c.ev1) != hipSuccess) t = -1;
+        ms[i] = t;
+      }
+    }
+  });
+}
+
+int l2g_eri_plan_destroy(l2g_eri_plan p) {
+  if (p) {
+    (void)hipSetDevice(p->basis->ctx->device);
+    (void)hipStreamSynchronize(p->basis->ctx->stream);
+  }
+  delete p;
+  return L2G_OK;
+}
+
+int l2g_eri_compute(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double* host_out,
+                    int64_t* offsets, int8_t* nonzero) {
+  l2g_eri_plan p = nullptr;
+  int rc = l2g_eri_plan_create(B, nq, quartets, precision, screening, 1.0, 0, &p);
+  if (rc) return rc;
+  if (offsets) rc = l2g_eri_plan_offsets(p, offsets);
+  if (!rc) rc = l2g_eri_plan_execute(p, nullptr);
+  if (!rc) rc = l2g_eri_plan_download(p, host_out, nonzero);
+  l2g_eri_plan_destroy(p);
+  return rc;
+}
+
+}  // extern "C"

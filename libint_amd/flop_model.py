@@ -1,0 +1,92 @@
+"""Canonical Head-Gordon-Pople FLOP model for one contracted shell quartet.
+
+This is the algorithmic work the roofline figure divides by time; it does not
+depend on how the kernels are organised (their target grouping recomputes
+part of the recursion, which is therefore NOT counted).
+
+Per primitive quartet (counted for every primitive quartet that survives the
+shell-pair screen):
+  * prefactors + Boys:  60 + 16 (L + 1)
+  * Obara-Saika VRR over the [e0|f0]^(m) set that the HGP targets need
+    (e in [la, la+lb], f in [lc, lc+ld]), at libint's per-term costs
+    (src/bin/libint/vrr_11_twoprep_11.h:157-220): 3 FLOP for the
+    PA/WP (QC/WQ) pair, +5 for the a-1 (c-1) pair, +3 for the cross term;
+  * contraction: 1 add per contracted [e0|f0] intermediate.
+Per quartet:
+  * HRR: 2 FLOP per produced element per step, bra first over every f, then
+    the ket over every Cartesian (ab) (src/bin/libint/hrr.h:290-330);
+  * Cartesian -> solid harmonics: 2 * nnz(l) per transformed slice, index by
+    index (include/libint2/solidharmonics.h transform_first/inner/last).
+
+For (dd|dd) with all shells pure this gives 23,912 + 11,256 + 10,736 =
+45,904 FLOP at one primitive quartet.
+"""
+from __future__ import annotations
+
+import functools
+
+from .codegen.gen_eri import VrrGraph, cart, ncart, sh_rows
+
+
+def _sh_nnz(l: int) -> int:
+    return sum(len(r) for r in sh_rows(l)) if l >= 2 else 0
+
+
+@functools.lru_cache(maxsize=None)
+def vrr_flops(la: int, lb: int, lc: int, ld: int) -> int:
+    """VRR + contraction FLOPs per primitive quartet (canonical class)."""
+    g = VrrGraph()
+    targets = [g.get(e, f, 0) for le in range(la, la + lb + 1) for e in cart(le)
+               for lf in range(lc, lc + ld + 1) for f in cart(lf)]
+    order = g.order(targets)
+    fl = 0
+    for k in order:
+        n = g.nodes[k]
+        if n.kind == 'base':
+            continue
+        fl += 3
+        for c, _ in n.deps[2:]:
+            if isinstance(c, tuple) and c[0] in ('e', 'f'):
+                fl += 5
+            elif isinstance(c, tuple) and c[0] == 'x':
+                fl += 3
+    return fl + len(targets)
+
+
+def _hrr_elems(l1: int, l2: int) -> int:
+    """Elements produced by the HRR transferring l2 quanta onto l1."""
+    tot = 0
+    for bl in range(1, l2 + 1):
+        for al in range(l1, l1 + l2 - bl + 1):
+            tot += ncart(al) * ncart(bl)
+    return tot
+
+
+@functools.lru_cache(maxsize=None)
+def quartet_flops(la: int, lb: int, lc: int, ld: int, pure=(True, True, True, True)) -> int:
+    """HRR + cart->sph FLOPs per contracted quartet (canonical class)."""
+    nf = sum(ncart(l) for l in range(lc, lc + ld + 1))
+    hrr = 2 * _hrr_elems(la, lb) * nf + 2 * _hrr_elems(lc, ld) * ncart(la) * ncart(lb)
+    dims = [ncart(la), ncart(lb), ncart(lc), ncart(ld)]
+    ls = [la, lb, lc, ld]
+    sph = 0
+    for k in range(4):
+        if not pure[k] or ls[k] < 2:
+            continue
+        other = 1
+        for j in range(4):
+            if j != k:
+                other *= dims[j]
+        sph += 2 * _sh_nnz(ls[k]) * other
+        dims[k] = 2 * ls[k] + 1
+    return hrr + sph
+
+
+def prim_flops(la: int, lb: int, lc: int, ld: int) -> int:
+    L = la + lb + lc + ld
+    return 60 + 16 * (L + 1) + vrr_flops(la, lb, lc, ld)
+
+
+def class_flops(la, lb, lc, ld, nquartets: int, nprim_quartets: int, pure=(True, True, True, True)) -> float:
+    """Model FLOPs of a batch of one class."""
+    return float(nprim_quartets) * prim_flops(la, lb, lc, ld) + float(nquartets) * quartet_flops(la, lb, lc, ld, pure)

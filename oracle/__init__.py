@@ -1,0 +1,164 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes front-end for ``oracle/lib/liboracle.so`` (built from
+``oracle/eri_oracle.cpp`` by ``oracle/Makefile``).  The oracle is a CPU
+restatement of libint's ``Engine::compute2<Operator::coulomb, BraKet::xx_xx, 0>``
+(include/libint2/engine.impl.h:1151-2113) plus an independent
+McMurchie-Davidson brute force and long-double Boys reference.  Only
+``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module; the shipped library never does.
+
+Parity status per function (how each is pinned is listed in the tests):
+  * boys_cheb7 / boys_reference: pinned against mpmath goldens
+    (tests/golden/boys_golden.json), tolerance of
+    tests/unit/test-core-ints.cc:79-84.  Bit content of libint's own
+    Chebyshev table is UNPINNED (the table is absent from the reference tree).
+  * shell_renorm: known answer 1.64592278064949 (tests/unit/test-core.cc:49-55)
+    and the unit self-overlap invariant.
+  * solidharm_coeff: known answers of tests/unit/test-core.cc:26-37 and the
+    p-shell mapping of tests/unit/test-shell-order.cc:52-72.
+  * eri_quartets: brute force (eri_brute), the Python goldens of
+    python/tests/test_libint2.py:36-45, 8-fold permutational symmetry.
+  * eri_brute: closed forms ((ss|ss) analytic), the same Python goldens.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "lib", "liboracle.so")
+_lib = None
+
+ORIGINAL, CONSERVATIVE, SCHWARZ, SCHWARZ_INF = 0x0001, 0x0010, 0x0100, 0x1000
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile the oracle library (g++, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            build()
+        L = ctypes.CDLL(_LIBPATH)
+        L.oracle_set_shg_ordering.argtypes = [ctypes.c_int]
+        L.oracle_boys_cheb7.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p]
+        L.oracle_boys_reference.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p]
+        L.oracle_boys_table.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.oracle_boys_table.restype = ctypes.POINTER(ctypes.c_double)
+        L.oracle_shell_renorm.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p, ctypes.c_int]
+        L.oracle_solidharm_coeff.argtypes = [ctypes.c_int] * 5
+        L.oracle_solidharm_coeff.restype = ctypes.c_double
+        basis_args = [ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.oracle_eri_quartets.argtypes = basis_args + [
+            ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+            ctypes.c_void_p, _i64p, ctypes.c_void_p, ctypes.c_int]
+        L.oracle_eri_quartets.restype = ctypes.c_int64
+        L.oracle_eri_brute.argtypes = basis_args + [ctypes.c_int64, _i32p, ctypes.c_void_p, ctypes.c_int]
+        L.oracle_eri_brute.restype = ctypes.c_int64
+        if hasattr(L, "oracle_fock_2e"):
+            L.oracle_fock_2e.argtypes = basis_args + [
+                _f64p, ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int,
+                ctypes.POINTER(ctypes.c_int64)]
+            L.oracle_fock_2e.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+@dataclass
+class OBasis:
+    """Flat shell arrays (coefficients already renormalised, bohr)."""
+    l: np.ndarray
+    pure: np.ndarray
+    nprim: np.ndarray
+    alpha: np.ndarray
+    coeff: np.ndarray
+    origin: np.ndarray
+
+    @staticmethod
+    def of(b) -> "OBasis":
+        return OBasis(np.ascontiguousarray(b.l, np.int32), np.ascontiguousarray(b.pure, np.int32),
+                      np.ascontiguousarray(b.nprim, np.int32), np.ascontiguousarray(b.alpha, np.float64),
+                      np.ascontiguousarray(b.coeff, np.float64),
+                      np.ascontiguousarray(np.asarray(b.origin, np.float64).reshape(-1)))
+
+    def args(self):
+        return (len(self.l), self.l, self.pure, self.nprim, self.alpha, self.coeff, self.origin)
+
+
+def set_shg_ordering(gaussian: bool) -> None:
+    lib().oracle_set_shg_ordering(1 if gaussian else 0)
+
+
+def boys_cheb7(T, mmax: int) -> np.ndarray:
+    T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
+    out = np.zeros((T.size, mmax + 1))
+    lib().oracle_boys_cheb7(T.size, T, mmax, out)
+    return out
+
+
+def boys_reference(T, mmax: int) -> np.ndarray:
+    T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
+    out = np.zeros((T.size, mmax + 1))
+    lib().oracle_boys_reference(T.size, T, mmax, out)
+    return out
+
+
+def boys_table() -> np.ndarray:
+    nint, mmax = ctypes.c_int(), ctypes.c_int()
+    p = lib().oracle_boys_table(ctypes.byref(nint), ctypes.byref(mmax))
+    n = nint.value * (mmax.value + 1) * 8
+    return np.ctypeslib.as_array(p, shape=(n,)).reshape(nint.value, mmax.value + 1, 8).copy()
+
+
+def shell_renorm(l: int, alpha, coeff, enforce_unit: bool = True):
+    a = np.ascontiguousarray(alpha, np.float64)
+    c = np.array(coeff, np.float64)
+    mlc = np.zeros_like(c)
+    lib().oracle_shell_renorm(l, a.size, a, c, mlc, 1 if enforce_unit else 0)
+    return c, mlc
+
+
+def solidharm_coeff(l, m, lx, ly, lz) -> float:
+    return lib().oracle_solidharm_coeff(l, m, lx, ly, lz)
+
+
+def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGINAL,
+                 uniform_cart=False, nthreads=0):
+    """Restated Engine::compute2 over quartets (caller order).
+
+    Returns (flat values, offsets[nq], nonzero[nq])."""
+    ob = OBasis.of(basis)
+    q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+    nq = q.shape[0]
+    offs = np.zeros(nq, np.int64)
+    L = lib()
+    total = L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening,
+                                  int(uniform_cart), None, offs, None, nthreads)
+    out = np.zeros(max(total, 1))
+    nz = np.zeros(nq, np.int8)
+    L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart),
+                          out.ctypes.data, offs, nz.ctypes.data, nthreads)
+    return out[:total], offs, nz
+
+
+def eri_brute(basis, quartets, nthreads=0):
+    ob = OBasis.of(basis)
+    q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+    L = lib()
+    total = L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), None, nthreads)
+    out = np.zeros(max(total, 1))
+    L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), out.ctypes.data, nthreads)
+    return out[:total]
