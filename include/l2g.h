@@ -147,6 +147,13 @@ int l2g_eri_plan_stats(l2g_eri_plan plan, int cap, int* nclasses, int* class_cod
 
 int l2g_eri_plan_destroy(l2g_eri_plan plan);
 
+/* Schwarz factors K(s1,s2) = sqrt(max|(s1 s2|s1 s2)|) (norm = 0) or
+ * sqrt(||(s1 s2|s1 s2)||_2) (norm = 1) for npairs shell pairs
+ * (pairs[2*i], pairs[2*i+1]), on the GPU.  Replaces the per-pair loop of
+ * compute_schwarz_ints (tests/hartree-fock/hartree-fock++.cc:1231-1298),
+ * which uses precision 0 (no primitive screening); pass 0 for the same. */
+int l2g_schwarz(l2g_basis basis, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K);
+
 /* One-shot convenience: plan, execute, download, destroy.  offsets and
  * nonzero may be NULL. */
 int l2g_eri_compute(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision, int screening,

@@ -26,7 +26,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "lib", "libl2g.so")
+LIBPATH = os.environ.get("LIBINT_AMD_LIB") or os.path.join(_HERE, "lib", "libl2g.so")
 
 SCREEN_ORIGINAL, SCREEN_CONSERVATIVE, SCREEN_SCHWARZ, SCREEN_SCHWARZINF = 0x0001, 0x0010, 0x0100, 0x1000
 SHG_STANDARD, SHG_GAUSSIAN = 1, 2
@@ -61,6 +61,7 @@ ABI = {
     "l2g_eri_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
     "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
+    "l2g_schwarz": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _f64p]),
 }
 
 _lib = None
@@ -302,6 +303,16 @@ class EriPlan:
                 self._h = None
         except Exception:
             pass
+
+
+def schwarz(basis: Basis, pairs, precision: float = 0.0, norm: str = "inf") -> np.ndarray:
+    """Schwarz factors for shell pairs (n, 2): sqrt(max|(ab|ab)|) ("inf") or
+    sqrt(||(ab|ab)||_2) ("2"), as compute_schwarz_ints
+    (tests/hartree-fock/hartree-fock++.cc:1231-1298)."""
+    p = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+    K = np.zeros(max(p.shape[0], 1))
+    _check(lib().l2g_schwarz(basis.handle, p.shape[0], p.reshape(-1), precision, 0 if norm == "inf" else 1, K))
+    return K[: p.shape[0]]
 
 
 def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,

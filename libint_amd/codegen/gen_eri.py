@@ -498,7 +498,7 @@ def pure_variants(l1, l2):
 # ---------------------------------------------------------------------------
 
 
-def emit_class(plan: ClassPlan, block=128):
+def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     la, lb, lc, ld = plan.la, plan.lb, plan.lc, plan.ld
     graph = VrrGraph()
     ne, nf = len(plan.ecomps), len(plan.fcomps)
@@ -508,15 +508,20 @@ def emit_class(plan: ClassPlan, block=128):
       f"{G} target group(s)")
     E(f"// group sizes (bra components): {[len(g) for g in plan.groups]}")
     E(f"// group graph nodes: {plan.group_nodes}; estimated peak live doubles: {plan.group_peak}")
-    kname = f"l2g_eri_{plan.name}"
-    E(f"__global__ void __launch_bounds__({block}) {kname}(const L2gEriArgs A)")
+    kname = f"l2g_{mode}_{plan.name}"
+    attr = f" __attribute__((amdgpu_waves_per_eu({waves}, {waves})))" if waves else ""
+    if mode == 'schwarz':
+        E("// Schwarz-bound variant: work item = shell pair p, evaluates (p|p) and")
+        E("// stores sqrt(max|.|) (norm 0) or sqrt(||.||_2) (norm 1) at out[out_off]")
+        E("// (tests/hartree-fock/hartree-fock++.cc:1231-1298).")
+    E(f"__global__ void __launch_bounds__({block}){attr} {kname}(const L2gEriArgs A)")
     E("{")
     E.ind += 1
     E("const int tid = blockIdx.x * blockDim.x + threadIdx.x;")
     E("const int nthr = gridDim.x * blockDim.x;")
     E.block("for (int qi = tid; qi < A.n; qi += nthr)")
     E("const int bp = A.bra_pair[qi];")
-    E("const int kp = A.ket_pair[qi];")
+    E("const int kp = " + ("bp;" if mode == 'schwarz' else "A.ket_pair[qi];"))
     E("const int meta = A.meta[qi];")
     E("const int pb0 = A.pair_first[bp], npb = A.pair_np[bp];")
     E("const int pk0 = A.pair_first[kp], npk = A.pair_np[kp];")
@@ -606,7 +611,10 @@ def emit_class(plan: ClassPlan, block=128):
             E.end()
         E.end()  # group block
     # bra HRR + bra sph + permuted store, column by column over the ket
-    E("if (A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
+    if mode == 'eri':
+        E("if (A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
+    else:
+        E("double smax = 0.0, ssq = 0.0;")
     E("// ---- bra HRR, bra spherical transform and permuted store")
     E(f"const L2gStore st = make_store(meta, A, qi, {la}, {lb}, {lc}, {ld});")
     E("const int ncol = st.nk1 * st.nk2;")
@@ -623,7 +631,11 @@ def emit_class(plan: ClassPlan, block=128):
             d1, d2, vals = emit_sph_2(E, la, lb, pa, pb, bvals, "b")
             for i1 in range(d1):
                 for i2 in range(d2):
-                    E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {vals[i1 * d2 + i2]});")
+                    v = vals[i1 * d2 + i2]
+                    if mode == 'eri':
+                        E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {v});")
+                    else:
+                        E(f"smax = fmax(smax, fabs({v})); ssq += {v} * {v};")
             E("break;")
             E.end()
         E("default: break;")
@@ -638,6 +650,8 @@ def emit_class(plan: ClassPlan, block=128):
         E.block("for (int col = 0; col < ncol; ++col)")
         bra_column("col", lambda ie: f"yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
         E.end()
+    if mode == 'schwarz':
+        E("A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(ssq)) : sqrt(smax);")
     E.end()  # quartet loop
     E.ind -= 1
     E("}")
@@ -654,54 +668,75 @@ def canonical_classes(lmax):
     return out
 
 
-def generate(outdir, lmax=2, budget=96, nfiles=8):
+def class_waves(plan, budget, waves):
+    """waves_per_eu hint: explicit, or 0 (compiler default)."""
+    return waves
+
+
+def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=None):
+    """Write the kernels and the dispatch tables.  `budgets` maps a class name
+    ('2222') to its own (budget, waves)."""
     os.makedirs(outdir, exist_ok=True)
     classes = canonical_classes(lmax)
-    plans = [plan_class(*c, budget=budget) for c in classes]
-    # distribute classes over files by estimated size (node count)
-    weights = sorted(((sum(p.group_nodes), i) for i, p in enumerate(plans)), reverse=True)
+    if only:
+        classes = [c for c in classes if "".join(map(str, c)) in only]
+    budgets = budgets or {}
+
+    def cfg(c):
+        return budgets.get("".join(map(str, c)), (budget, waves))
+
+    plans = [plan_class(*c, budget=cfg(c)[0]) for c in classes]
+    jobs = []  # (plan, mode)
+    for p in plans:
+        jobs.append((p, 'eri'))
+        if (p.la, p.lb) == (p.lc, p.ld):
+            jobs.append((p, 'schwarz'))
+    weights = sorted(((sum(p.group_nodes) * (0.6 if m == 'schwarz' else 1.0), i) for i, (p, m) in enumerate(jobs)),
+                     reverse=True)
     files = [[] for _ in range(nfiles)]
     load = [0] * nfiles
     for w, i in weights:
         j = load.index(min(load))
         files[j].append(i)
         load[j] += w
-    table = []
+    table, stable = [], []
     written = []
     for fi, idxs in enumerate(files):
-        if not idxs:
-            continue
         fn = os.path.join(outdir, f"eri_gen_{fi}.hip")
         parts = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
                  '#include "../eri_device.h"', '']
         for i in sorted(idxs):
-            p = plans[i]
-            kname, code, in_regs = emit_class(p)
+            p, mode = jobs[i]
+            wv = cfg((p.la, p.lb, p.lc, p.ld))[1]
+            kname, code, in_regs = emit_class(p, mode=mode, waves=wv)
             parts.append(code)
             parts.append(f"void {kname}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s) {{")
             parts.append(f"  hipLaunchKernelGGL({kname}, dim3(grid), dim3(block), 0, s, A);")
             parts.append("}")
             parts.append("")
-            table.append((p, kname, in_regs))
+            (table if mode == 'eri' else stable).append((p, kname, in_regs))
         new = '\n'.join(parts)
         if not (os.path.exists(fn) and open(fn).read() == new):
             with open(fn, 'w') as f:
                 f.write(new)
         written.append(fn)
-    # dispatch table
     lines = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
              '#include "../eri_device.h"', '']
-    for p, k, _ in table:
+    for p, k, _ in table + stable:
         lines.append(f"void {k}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s);")
     lines.append('')
     lines.append(f"extern const int l2g_gen_lmax = {lmax};")
-    lines.append("extern const L2gClassInfo l2g_gen_classes[] = {")
-    for p, k, in_regs in table:
-        ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld)
-        lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
-                     f"{sum(p.group_nodes)}, &{k}_launch}},")
-    lines.append("};")
+    for name, tab in (("l2g_gen_classes", table), ("l2g_gen_schwarz", stable)):
+        lines.append(f"extern const L2gClassInfo {name}[] = {{")
+        for p, k, in_regs in tab:
+            ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld)
+            lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
+                         f"{sum(p.group_nodes)}, &{k}_launch}},")
+        if not tab:
+            lines.append("  {0, 0, 0, 0, 0, 0, 0, nullptr},")
+        lines.append("};")
     lines.append(f"extern const int l2g_gen_nclasses = {len(table)};")
+    lines.append(f"extern const int l2g_gen_nschwarz = {len(stable)};")
     fn = os.path.join(outdir, "eri_table.cpp")
     new = '\n'.join(lines) + '\n'
     if not (os.path.exists(fn) and open(fn).read() == new):
@@ -712,9 +747,22 @@ def generate(outdir, lmax=2, budget=96, nfiles=8):
 
 
 if __name__ == '__main__':
-    out = sys.argv[1]
-    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
-    budget = int(sys.argv[3]) if len(sys.argv) > 3 else 96
-    files, plans = generate(out, lmax, budget)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("outdir")
+    ap.add_argument("--lmax", type=int, default=2)
+    ap.add_argument("--budget", type=int, default=96)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--only", default="", help="comma-separated class names, e.g. 2222")
+    ap.add_argument("--class-cfg", default="", help="per class budget:waves, e.g. 2222=64:2,2122=80:2")
+    ap.add_argument("--nfiles", type=int, default=8)
+    a = ap.parse_args()
+    per = {}
+    for item in filter(None, a.class_cfg.split(",")):
+        k, v = item.split("=")
+        b, w = v.split(":")
+        per[k] = (int(b), int(w))
+    files, plans = generate(a.outdir, a.lmax, a.budget, a.nfiles, a.waves,
+                            set(filter(None, a.only.split(","))) or None, per)
     for p in plans:
         print(p.name, 'groups', len(p.groups), 'nodes', sum(p.group_nodes), 'peak', max(p.group_peak))

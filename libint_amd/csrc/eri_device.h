@@ -39,6 +39,7 @@ struct L2gEriArgs {
   int method;
   int shg_gaussian;
   int uniform_cart;
+  int schwarz_norm;  // Schwarz kernels: 0 = max-abs norm, 1 = Frobenius norm
   double* __restrict__ out;
   double* __restrict__ ybuf;
   long long ystride;

@@ -20,6 +20,8 @@
 
 extern const L2gClassInfo l2g_gen_classes[];
 extern const int l2g_gen_nclasses;
+extern const L2gClassInfo l2g_gen_schwarz[];
+extern const int l2g_gen_nschwarz;
 extern const int l2g_gen_lmax;
 
 namespace {
@@ -184,6 +186,7 @@ struct l2g_context_s {
   int shg_gaussian = 0;
   int ncu = 256;
   int class_of[8][8][8][8];
+  int schwarz_of[8][8];
 };
 
 struct l2g_basis_s {
@@ -395,6 +398,12 @@ int l2g_init(int device, l2g_context* out) {
     for (int i = 0; i < l2g_gen_nclasses; ++i) {
       const auto& c = l2g_gen_classes[i];
       ctx->class_of[c.la][c.lb][c.lc][c.ld] = i;
+    }
+    for (auto& a : ctx->schwarz_of)
+      for (auto& b : a) b = -1;
+    for (int i = 0; i < l2g_gen_nschwarz; ++i) {
+      const auto& c = l2g_gen_schwarz[i];
+      if (c.launch) ctx->schwarz_of[c.la][c.lb] = i;
     }
     *out = ctx.release();
   });
@@ -701,6 +710,95 @@ int l2g_eri_plan_destroy(l2g_eri_plan p) {
   }
   delete p;
   return L2G_OK;
+}
+
+int l2g_schwarz(l2g_basis B, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K) {
+  return guarded([&] {
+    if (!B) throw L2gError(L2G_E_STATE, "null basis");
+    if (npairs < 0 || (npairs > 0 && (!pairs || !K)) || (norm != 0 && norm != 1))
+      throw std::invalid_argument("l2g_schwarz: bad arguments");
+    if (npairs == 0) return;
+    l2g_context_s* ctx = B->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    const double lnp = precision > 0 ? std::log(precision) : std::numeric_limits<double>::lowest();
+    PairBuilder pb(*B, lnp, L2G_SCREEN_ORIGINAL);
+    std::vector<std::vector<int>> pid(l2g_gen_nschwarz), meta(l2g_gen_nschwarz);
+    std::vector<std::vector<long long>> idx(l2g_gen_nschwarz);
+    for (int64_t q = 0; q < npairs; ++q) {
+      int a = pairs[2 * q], b = pairs[2 * q + 1];
+      if (a < 0 || b < 0 || a >= B->nshell || b >= B->nshell) throw std::invalid_argument("shell index out of range");
+      if (B->l[a] < B->l[b]) std::swap(a, b);
+      const int c = ctx->schwarz_of[B->l[a]][B->l[b]];
+      if (c < 0) throw L2gError(L2G_E_LMAX, "no Schwarz kernel for this shell pair");
+      const int pa = (B->pure[a] && B->l[a] > 0) ? 1 : 0, pbit = (B->pure[b] && B->l[b] > 0) ? 1 : 0;
+      pid[c].push_back(pb.get(a, b));
+      meta[c].push_back((pa << 3) | (pbit << 4) | (pa << 5) | (pbit << 6));
+      idx[c].push_back(q);
+    }
+    std::vector<int> allp, allm;
+    std::vector<long long> alli;
+    std::vector<std::pair<int, int64_t>> ranges;
+    int64_t ymax = 0, maxn = 0;
+    for (int c = 0; c < l2g_gen_nschwarz; ++c) {
+      if (pid[c].empty()) continue;
+      ranges.push_back({c, (int64_t)allp.size()});
+      allp.insert(allp.end(), pid[c].begin(), pid[c].end());
+      allm.insert(allm.end(), meta[c].begin(), meta[c].end());
+      alli.insert(alli.end(), idx[c].begin(), idx[c].end());
+      ymax = std::max<int64_t>(ymax, l2g_gen_schwarz[c].ybuf_doubles);
+      maxn = std::max<int64_t>(maxn, (int64_t)pid[c].size());
+    }
+    hipStream_t st = ctx->stream;
+    DevBuf dp, dm, di, pf, pn, pab, pP, pPA, pAux, pNs, dK, yb;
+    upload(dp, allp, st);
+    upload(dm, allm, st);
+    upload(di, alli, st);
+    upload(pf, pb.first, st);
+    upload(pn, pb.np, st);
+    upload(pab, pb.AB, st);
+    upload(pP, pb.P, st);
+    upload(pPA, pb.PA, st);
+    upload(pAux, pb.aux, st);
+    upload(pNs, pb.ns, st);
+    dK.alloc(npairs * sizeof(double));
+    const int block = 128;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
+    const int64_t ystride = (int64_t)grid * block;
+    yb.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
+    L2gEriArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.pair_first = pf.as<int>();
+    A.pair_np = pn.as<int>();
+    A.pair_AB = pab.as<double>();
+    A.pp_P = pP.as<double4>();
+    A.pp_PA = pPA.as<double4>();
+    A.pp_aux = pAux.as<double2>();
+    A.pp_nonsph = pNs.as<double>();
+    A.boys = ctx->boys.as<double>();
+    A.precision = precision > 0 ? precision : 0.0;
+    A.ln_precision = lnp;
+    A.scale = 1.0;
+    A.method = L2G_SCREEN_ORIGINAL;
+    A.shg_gaussian = 0;
+    A.schwarz_norm = norm;
+    A.out = dK.as<double>();
+    A.ybuf = yb.as<double>();
+    A.ystride = ystride;
+    for (auto& r : ranges) {
+      const int c = r.first;
+      const int64_t n = (int64_t)pid[c].size();
+      A.bra_pair = dp.as<int>() + r.second;
+      A.ket_pair = A.bra_pair;
+      A.meta = dm.as<int>() + r.second;
+      A.out_off = di.as<long long>() + r.second;
+      A.n = (int)n;
+      l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
+      HIPCHK(hipGetLastError());
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Schwarz kernel failed: ") + hipGetErrorString(e));
+    HIPCHK(hipMemcpy(K, dK.p, npairs * sizeof(double), hipMemcpyDeviceToHost));
+  });
 }
 
 int l2g_eri_compute(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double* host_out,

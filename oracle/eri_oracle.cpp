@@ -129,7 +129,9 @@ static void boys_reference(long double T, int mmax, long double* F) {
 struct Cheb7 {
   static constexpr int ORDER = 7, NCOEF = 8, MMAX = 40;
   static constexpr double TCRIT = 117.0, DELTA = 1.0 / 7.0;
-  static constexpr int NINT = 819;  // 117 * 7
+  // 117 * 7 intervals plus one, so that T = 117 exactly (which is not above
+  // T_crit) still finds its interval
+  static constexpr int NINT = 820;
   std::vector<double> c;            // [NINT][MMAX+1][8]
   Cheb7() {
     c.resize((size_t)NINT * (MMAX + 1) * NCOEF);

@@ -140,3 +140,23 @@ def test_engine_facade_single_quartet(ctx):
     o, _, _ = oracle.eri_quartets(la.BasisArrays.from_shells(sh), [[0, 1, 2, 3]])
     nfail, _, _ = compare(v, o, ABS, REL)
     assert nfail == 0
+
+
+def test_schwarz_factors(ctx):
+    """GPU Schwarz kernels vs max|.| / ||.||_2 of the oracle's (ab|ab) at
+    precision 0 (tests/hartree-fock/hartree-fock++.cc:1231-1298)."""
+    mol = molecule.make_molecule(1)
+    B = mol.basis
+    i, j = np.tril_indices(B.nshell)
+    pairs = np.stack([i, j], 1).astype(np.int32)
+    say(f"case: Schwarz factors workload-1, {len(pairs)} pairs")
+    b = la.Basis(ctx, B)
+    Kinf = la.schwarz(b, pairs, 0.0, "inf")
+    K2 = la.schwarz(b, pairs, 0.0, "2")
+    q = np.stack([pairs[:, 0], pairs[:, 1], pairs[:, 0], pairs[:, 1]], 1)
+    o, off, _ = oracle.eri_quartets(B, q, precision=0.0)
+    ends = np.append(off[1:], o.size)
+    Oinf = np.array([np.sqrt(np.abs(o[a:e]).max()) for a, e in zip(off, ends)])
+    O2 = np.array([np.sqrt(np.linalg.norm(o[a:e])) for a, e in zip(off, ends)])
+    assert compare(Kinf, Oinf, ABS, REL)[0] == 0
+    assert compare(K2, O2, ABS, REL)[0] == 0
