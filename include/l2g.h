@@ -47,6 +47,7 @@ extern "C" {
 typedef struct l2g_context_s* l2g_context;
 typedef struct l2g_basis_s* l2g_basis;
 typedef struct l2g_eri_plan_s* l2g_eri_plan;
+typedef struct l2g_fock_s* l2g_fock;
 
 /* ---- library / device ------------------------------------------------- */
 
@@ -153,6 +154,38 @@ int l2g_eri_plan_destroy(l2g_eri_plan plan);
  * compute_schwarz_ints (tests/hartree-fock/hartree-fock++.cc:1231-1298),
  * which uses precision 0 (no primitive screening); pass 0 for the same. */
 int l2g_schwarz(l2g_basis basis, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K);
+
+/* ---- Fock build ---------------------------------------------------------- */
+
+/* Set-up of a screened two-electron Fock build for a basis, restating
+ * compute_shellpairs and compute_schwarz_ints of
+ * tests/hartree-fock/hartree-fock++.cc:1231-1436: the significant shell-pair
+ * list (same centre, or overlap-block Frobenius norm >= pair_threshold,
+ * 1e-12 there), the Schwarz matrix sqrt(max|(ab|ab)|) and shell-pair data at
+ * max_engine_precision (epsilon/1e10 there) with the given primitive
+ * screening (SchwarzInf there, :67).  Schwarz factors run on the GPU. */
+int l2g_fock_create(l2g_basis basis, double pair_threshold, double max_engine_precision, int screening,
+                    l2g_fock* fock);
+
+/* Number of basis functions of the Fock build's basis. */
+int l2g_fock_nbf(l2g_fock fock, int* nbf);
+
+/* Copies the nshell x nshell Schwarz matrix (zero for insignificant pairs). */
+int l2g_fock_schwarz(l2g_fock fock, double* K);
+
+/* One screened Fock build for density D (nbf x nbf, row-major, symmetric),
+ * replacing compute_2body_fock (hartree-fock++.cc:1574-1772): quartets with
+ * Dnorm1234 K12 K34 < precision are skipped, each quartet is evaluated at
+ * precision / Dnorm1234, integrals are digested on the GPU in FP64 with the
+ * permutational degeneracy.  Outputs (any may be NULL, nbf x nbf row-major):
+ * J_ab = sum (ab|cd) D_cd, K_ab = sum (ac|bd) D_cd, G = 2J - K. */
+int l2g_fock_build(l2g_fock fock, const double* D, double precision, double* J, double* K, double* G);
+
+/* Counters of the last build: quartets evaluated, primitive quartets (after
+ * pair screening) and the GPU time of the digest kernels in ms. */
+int l2g_fock_stats(l2g_fock fock, int64_t* nquartets, int64_t* nprim_quartets, float* kernel_ms);
+
+int l2g_fock_destroy(l2g_fock fock);
 
 /* One-shot convenience: plan, execute, download, destroy.  offsets and
  * nonzero may be NULL. */

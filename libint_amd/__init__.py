@@ -62,6 +62,13 @@ ABI = {
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
     "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
     "l2g_schwarz": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _f64p]),
+    "l2g_fock_create": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "l2g_fock_nbf": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "l2g_fock_schwarz": (ctypes.c_int, [_vp, _f64p]),
+    "l2g_fock_build": (ctypes.c_int, [_vp, _f64p, ctypes.c_double, _vp, _vp, _vp]),
+    "l2g_fock_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_float)]),
+    "l2g_fock_destroy": (ctypes.c_int, [_vp]),
 }
 
 _lib = None
@@ -313,6 +320,50 @@ def schwarz(basis: Basis, pairs, precision: float = 0.0, norm: str = "inf") -> n
     K = np.zeros(max(p.shape[0], 1))
     _check(lib().l2g_schwarz(basis.handle, p.shape[0], p.reshape(-1), precision, 0 if norm == "inf" else 1, K))
     return K[: p.shape[0]]
+
+
+class FockBuilder:
+    """Screened two-electron Fock builds on the GPU for one basis
+    (compute_shellpairs + compute_schwarz_ints + compute_2body_fock of
+    tests/hartree-fock/hartree-fock++.cc)."""
+
+    def __init__(self, basis: Basis, pair_threshold: float = 1e-12,
+                 max_engine_precision: float = np.finfo(float).eps / 1e10, screening: int = SCREEN_SCHWARZINF):
+        self.basis = basis
+        h = _vp()
+        _check(lib().l2g_fock_create(basis.handle, pair_threshold, max_engine_precision, screening, ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int()
+        _check(lib().l2g_fock_nbf(h, ctypes.byref(n)))
+        self.nbf = n.value
+
+    def schwarz(self) -> np.ndarray:
+        n = self.basis.arrays.nshell
+        K = np.zeros(n * n)
+        _check(lib().l2g_fock_schwarz(self._h, K))
+        return K.reshape(n, n)
+
+    def build(self, D, precision: float = 1e-12):
+        """Returns (G, J, K) with G = 2J - K."""
+        D = np.ascontiguousarray(D, np.float64)
+        N = self.nbf
+        assert D.shape == (N, N)
+        G, J, K = np.zeros((N, N)), np.zeros((N, N)), np.zeros((N, N))
+        _check(lib().l2g_fock_build(self._h, D.reshape(-1), precision, J.ctypes.data, K.ctypes.data, G.ctypes.data))
+        return G, J, K
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float()
+        _check(lib().l2g_fock_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(nquartets=a.value, nprim=b.value, kernel_ms=c.value)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().l2g_fock_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
 
 
 def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,

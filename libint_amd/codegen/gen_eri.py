@@ -510,6 +510,12 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     E(f"// group graph nodes: {plan.group_nodes}; estimated peak live doubles: {plan.group_peak}")
     kname = f"l2g_{mode}_{plan.name}"
     attr = f" __attribute__((amdgpu_waves_per_eu({waves}, {waves})))" if waves else ""
+    if mode == 'fock':
+        E("// Fock variant: digests each integral of the unique quartet into the")
+        E("// unsymmetrised Coulomb (jg) and exchange (kg) accumulators with the")
+        E("// degeneracy factor, as compute_2body_fock does")
+        E("// (tests/hartree-fock/hartree-fock++.cc:1709-1743); J = (jg + jg^T)/4,")
+        E("// K = (kg + kg^T)/8, G = 2J - K after symmetrisation.")
     if mode == 'schwarz':
         E("// Schwarz-bound variant: work item = shell pair p, evaluates (p|p) and")
         E("// stores sqrt(max|.|) (norm 0) or sqrt(||.||_2) (norm 1) at out[out_off]")
@@ -529,6 +535,12 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     E("const double CDx = A.pair_AB[3 * kp], CDy = A.pair_AB[3 * kp + 1], CDz = A.pair_AB[3 * kp + 2];")
     E(f"const double* __restrict__ boysL = A.boys + (size_t){plan.L} * L2G_BOYS_NINT * 8;")
     E("const double npbk = (double)npb * (double)npk;")
+    if mode == 'fock':
+        E("const double lnprec = A.q_lnprec[qi];")
+        E("const double qprec = A.method == L2G_SCREEN_ORIGINAL || A.method == L2G_SCREEN_CONSERVATIVE ? exp(lnprec) : 0.0;")
+    else:
+        E("const double lnprec = A.ln_precision;")
+        E("const double qprec = A.precision;")
     E("(void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz; (void)npbk;")
     E("// lane-private scratch column; the empty asm keeps the compiler from")
     E("// hoisting one address per row/column out of the quartet loop")
@@ -555,7 +567,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
         E("const double2 xb = A.pp_aux[pb0 + ib];")
         E.block("for (int ik = 0; ik < npk; ++ik)")
         E("const double2 xk = A.pp_aux[pk0 + ik];")
-        E("if (!(xb.x + xk.x > A.ln_precision)) continue;")
+        E("if (!(xb.x + xk.x > lnprec)) continue;")
         E("const double4 Qg = A.pp_P[pk0 + ik];")
         E("const double4 QCk = A.pp_PA[pk0 + ik];")
         E("const double gp = Pg.w, gq = Qg.w;")
@@ -563,11 +575,11 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
         E("const double oos = rs * rs;")
         E("const double pfac = PAk.w * QCk.w * rs * A.scale;")
         E.block("if (A.method == L2G_SCREEN_ORIGINAL)")
-        E("if (fabs(pfac) < A.precision) continue;")
+        E("if (fabs(pfac) < qprec) continue;")
         E.end()
         E.block("else if (A.method == L2G_SCREEN_CONSERVATIVE)")
         E("const double nsk = A.pp_nonsph[pk0 + ik], nsb = A.pp_nonsph[pb0 + ib];")
-        E("if (fabs(pfac) * fmax(1.0, nsb * nsk) * npbk < A.precision) continue;")
+        E("if (fabs(pfac) * fmax(1.0, nsb * nsk) * npbk < qprec) continue;")
         E.end()
         if gi == 0:
             E("++nsurv;")
@@ -613,8 +625,14 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     # bra HRR + bra sph + permuted store, column by column over the ket
     if mode == 'eri':
         E("if (A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
-    else:
+    elif mode == 'schwarz':
         E("double smax = 0.0, ssq = 0.0;")
+    else:
+        E("if (nsurv == 0) continue;  // every primitive screened: libint returns nullptr, nothing to digest")
+        E("const int4 bf = A.q_bf[qi];")
+        E("const double deg = A.q_scale[qi];")
+        E("const long long N = A.nbf;")
+        E("const double* __restrict__ Dm = A.dens;")
     E("// ---- bra HRR, bra spherical transform and permuted store")
     E(f"const L2gStore st = make_store(meta, A, qi, {la}, {lb}, {lc}, {ld});")
     E("const int ncol = st.nk1 * st.nk2;")
@@ -623,23 +641,73 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
         E("const int ic = (" + colexpr + ") / st.nk2, id = (" + colexpr + ") - ic * st.nk2;")
         E(f"const size_t colofs = st.base + (size_t)st.pos3(ic) * st.s3 + (size_t)st.pos4(id) * st.s4;")
         E("const double colscale = st.uniform ? st.scale3(ic) * st.scale4(id) : 1.0;")
+        if mode == 'fock':
+            E("const long long bk3 = bf.z + st.pos3(ic), bk4 = bf.w + st.pos4(id);")
+            E("const double d34 = Dm[bk3 * N + bk4];")
         inp = {e: ysrc(ie) for ie, e in enumerate(plan.ecomps)}
         bvals = emit_hrr(E, la, lb, inp, "g", "AB")
         E.block("switch ((meta >> 3) & 3)")
         for (pa, pb) in pure_variants(la, lb):
             E.block(f"case {pa | (pb << 1)}:")
             d1, d2, vals = emit_sph_2(E, la, lb, pa, pb, bvals, "b")
-            for i1 in range(d1):
+            if mode == 'fock':
+                for i1 in range(d1):
+                    E(f"const long long ba1_{i1} = bf.x + st.pos(st.l1, st.p1, {i1});")
                 for i2 in range(d2):
-                    v = vals[i1 * d2 + i2]
-                    if mode == 'eri':
-                        E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {v});")
-                    else:
-                        E(f"smax = fmax(smax, fabs({v})); ssq += {v} * {v};")
+                    E(f"const long long ba2_{i2} = bf.y + st.pos(st.l2, st.p2, {i2});")
+                E("double jcol = 0.0;")
+                for i1 in range(d1):
+                    E(f"double k13_{i1} = 0.0, k14_{i1} = 0.0;")
+                for i2 in range(d2):
+                    E(f"double k24_{i2} = 0.0, k23_{i2} = 0.0;")
+                for i2 in range(d2):
+                    E(f"const double d24_{i2} = Dm[ba2_{i2} * N + bk4], d23_{i2} = Dm[ba2_{i2} * N + bk3];")
+                for i1 in range(d1):
+                    E(f"const double d13_{i1} = Dm[ba1_{i1} * N + bk3], d14_{i1} = Dm[ba1_{i1} * N + bk4];")
+                for i1 in range(d1):
+                    for i2 in range(d2):
+                        v = vals[i1 * d2 + i2]
+                        E(f"jcol += d12_{i1}_{i2} * {v}; j12_{i1}_{i2} += d34 * {v};")
+                        E(f"k13_{i1} += d24_{i2} * {v}; k14_{i1} += d23_{i2} * {v};")
+                        E(f"k24_{i2} += d13_{i1} * {v}; k23_{i2} += d14_{i1} * {v};")
+                E("unsafeAtomicAdd(&A.jg[bk3 * N + bk4], deg * jcol);")
+                for i1 in range(d1):
+                    E(f"unsafeAtomicAdd(&A.kg[ba1_{i1} * N + bk3], deg * k13_{i1});")
+                    E(f"unsafeAtomicAdd(&A.kg[ba1_{i1} * N + bk4], deg * k14_{i1});")
+                for i2 in range(d2):
+                    E(f"unsafeAtomicAdd(&A.kg[ba2_{i2} * N + bk4], deg * k24_{i2});")
+                    E(f"unsafeAtomicAdd(&A.kg[ba2_{i2} * N + bk3], deg * k23_{i2});")
+            else:
+                for i1 in range(d1):
+                    for i2 in range(d2):
+                        v = vals[i1 * d2 + i2]
+                        if mode == 'eri':
+                            E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {v});")
+                        else:
+                            E(f"smax = fmax(smax, fabs({v})); ssq += {v} * {v};")
             E("break;")
             E.end()
         E("default: break;")
         E.end()
+
+    if mode == 'fock':
+        # D(ab) block and the J(ab) accumulator of the bra, sized for the
+        # Cartesian case (the pure case uses the leading part)
+        n1m, n2m = ncart(la), ncart(lb)
+        for i1 in range(n1m):
+            for i2 in range(n2m):
+                E(f"double j12_{i1}_{i2} = 0.0;")
+        E.block("switch ((meta >> 3) & 3)")
+        for (pa, pb) in pure_variants(la, lb):
+            d1 = 2 * la + 1 if pa else ncart(la)
+            d2 = 2 * lb + 1 if pb else ncart(lb)
+            E(f"case {pa | (pb << 1)}: break;")
+        E("default: break;")
+        E.end()
+        for i1 in range(n1m):
+            for i2 in range(n2m):
+                E(f"const double d12_{i1}_{i2} = ({i1} < st.n1 && {i2} < st.n2) ? "
+                  f"Dm[(bf.x + st.pos(st.l1, st.p1, {i1})) * N + bf.y + st.pos(st.l2, st.p2, {i2})] : 0.0;")
 
     if in_regs:
         for col in range(ncol_max):
@@ -652,6 +720,11 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
         E.end()
     if mode == 'schwarz':
         E("A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(ssq)) : sqrt(smax);")
+    if mode == 'fock':
+        for i1 in range(ncart(la)):
+            for i2 in range(ncart(lb)):
+                E(f"if ({i1} < st.n1 && {i2} < st.n2) unsafeAtomicAdd(&A.jg[(bf.x + st.pos(st.l1, st.p1, {i1})) * N + "
+                  f"bf.y + st.pos(st.l2, st.p2, {i2})], deg * j12_{i1}_{i2});")
     E.end()  # quartet loop
     E.ind -= 1
     E("}")
@@ -689,9 +762,10 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
     jobs = []  # (plan, mode)
     for p in plans:
         jobs.append((p, 'eri'))
+        jobs.append((p, 'fock'))
         if (p.la, p.lb) == (p.lc, p.ld):
             jobs.append((p, 'schwarz'))
-    weights = sorted(((sum(p.group_nodes) * (0.6 if m == 'schwarz' else 1.0), i) for i, (p, m) in enumerate(jobs)),
+    weights = sorted(((sum(p.group_nodes) * (0.6 if m == 'schwarz' else 1.0) + 200, i) for i, (p, m) in enumerate(jobs)),
                      reverse=True)
     files = [[] for _ in range(nfiles)]
     load = [0] * nfiles
@@ -699,7 +773,7 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
         j = load.index(min(load))
         files[j].append(i)
         load[j] += w
-    table, stable = [], []
+    table, stable, ftable = [], [], []
     written = []
     for fi, idxs in enumerate(files):
         fn = os.path.join(outdir, f"eri_gen_{fi}.hip")
@@ -714,7 +788,7 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
             parts.append(f"  hipLaunchKernelGGL({kname}, dim3(grid), dim3(block), 0, s, A);")
             parts.append("}")
             parts.append("")
-            (table if mode == 'eri' else stable).append((p, kname, in_regs))
+            {'eri': table, 'schwarz': stable, 'fock': ftable}[mode].append((p, kname, in_regs))
         new = '\n'.join(parts)
         if not (os.path.exists(fn) and open(fn).read() == new):
             with open(fn, 'w') as f:
@@ -722,11 +796,11 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
         written.append(fn)
     lines = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
              '#include "../eri_device.h"', '']
-    for p, k, _ in table + stable:
+    for p, k, _ in table + stable + ftable:
         lines.append(f"void {k}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s);")
     lines.append('')
     lines.append(f"extern const int l2g_gen_lmax = {lmax};")
-    for name, tab in (("l2g_gen_classes", table), ("l2g_gen_schwarz", stable)):
+    for name, tab in (("l2g_gen_classes", table), ("l2g_gen_schwarz", stable), ("l2g_gen_fock", ftable)):
         lines.append(f"extern const L2gClassInfo {name}[] = {{")
         for p, k, in_regs in tab:
             ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld)
@@ -737,6 +811,7 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
         lines.append("};")
     lines.append(f"extern const int l2g_gen_nclasses = {len(table)};")
     lines.append(f"extern const int l2g_gen_nschwarz = {len(stable)};")
+    lines.append(f"extern const int l2g_gen_nfock = {len(ftable)};")
     fn = os.path.join(outdir, "eri_table.cpp")
     new = '\n'.join(lines) + '\n'
     if not (os.path.exists(fn) and open(fn).read() == new):

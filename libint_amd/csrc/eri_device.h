@@ -45,6 +45,14 @@ struct L2gEriArgs {
   long long ystride;
   signed char* __restrict__ nz;          // optional: 0 if every primitive was screened out
   const long long* __restrict__ qidx;    // caller index of each work item (for nz)
+  // Fock digestion (l2g_fock_* kernels)
+  const double* __restrict__ q_lnprec;   // per work item ln(precision)
+  const double* __restrict__ q_scale;    // per work item degeneracy factor
+  const int4* __restrict__ q_bf;         // first basis function of canonical b1, b2, k1, k2
+  const double* __restrict__ dens;       // density, nbf x nbf row-major
+  double* __restrict__ jg;               // unsymmetrised Coulomb accumulator
+  double* __restrict__ kg;               // unsymmetrised exchange accumulator
+  long long nbf;
 };
 
 typedef void (*L2gLaunchFn)(const L2gEriArgs&, int grid, int block, hipStream_t s);
@@ -126,7 +134,7 @@ __device__ __forceinline__ double l2g_uniform_cart_scale(int l, int c) {
 // the Gaussian position when that ordering is selected (shgshell_ordering.h).
 struct L2gStore {
   size_t base;
-  int nk1, nk2;
+  int n1, n2, nk1, nk2;
   int s1, s2, s3, s4;
   int l1, l2, l3, l4;
   int p1, p2, p3, p4;
@@ -164,6 +172,8 @@ __device__ __forceinline__ L2gStore make_store(int meta, const L2gEriArgs& A, in
   const int n2 = st.p2 ? 2 * l2 + 1 : l2g_ncart(l2);
   const int n3 = st.p3 ? 2 * l3 + 1 : l2g_ncart(l3);
   const int n4 = st.p4 ? 2 * l4 + 1 : l2g_ncart(l4);
+  st.n1 = n1;
+  st.n2 = n2;
   st.nk1 = n3;
   st.nk2 = n4;
   const int swap_tbra = meta & 1, swap_tket = (meta >> 1) & 1, swap_braket = (meta >> 2) & 1;

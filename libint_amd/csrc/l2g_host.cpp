@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -10,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -22,6 +25,8 @@ extern const L2gClassInfo l2g_gen_classes[];
 extern const int l2g_gen_nclasses;
 extern const L2gClassInfo l2g_gen_schwarz[];
 extern const int l2g_gen_nschwarz;
+extern const L2gClassInfo l2g_gen_fock[];
+extern const int l2g_gen_nfock;
 extern const int l2g_gen_lmax;
 
 namespace {
@@ -185,7 +190,8 @@ struct l2g_context_s {
   DevBuf boys;
   int shg_gaussian = 0;
   int ncu = 256;
-  int class_of[8][8][8][8];
+  int class_of[8][8][8][8];  // index into l2g_gen_classes
+  int fock_of[8][8][8][8];   // index into l2g_gen_fock (its own order)
   int schwarz_of[8][8];
 };
 
@@ -242,12 +248,27 @@ struct PairBuilder {
   std::vector<double> ns;
   std::unordered_map<uint64_t, int> id;
 
+  // Schwarz / SchwarzInf primitive screening (shell.h:1258-1328): factors of
+  // the ordered pair (i, j), p1-major, supplied per pair by the caller
+  std::unordered_map<uint64_t, const std::vector<double>*> pfac;
+
   PairBuilder(const l2g_basis_s& b, double lp, int m) : B(b), ln_prec(lp), method(m) {}
 
   int get(int i, int j) {
     const uint64_t key = (uint64_t)(uint32_t)i << 32 | (uint32_t)j;
     auto it = id.find(key);
     if (it != id.end()) return it->second;
+    const std::vector<double>* fac = nullptr;
+    bool fac_t = false;
+    if (method == L2G_SCREEN_SCHWARZ || method == L2G_SCREEN_SCHWARZINF) {
+      auto f = pfac.find(key);
+      if (f == pfac.end()) {
+        f = pfac.find((uint64_t)(uint32_t)j << 32 | (uint32_t)i);
+        fac_t = true;
+      }
+      if (f == pfac.end()) throw L2gError(L2G_E_STATE, "no primitive Schwarz factors for a shell pair");
+      fac = f->second;
+    }
     const int pid = (int)first.size();
     id.emplace(key, pid);
     first.push_back((int)P.size());
@@ -269,6 +290,11 @@ struct PairBuilder {
         const double rho = a1 * a2 * oog;
         const double mr = -rho * AB2;
         double ln_scr = mr + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
+        if (fac) {
+          const double f = fac_t ? (*fac)[(size_t)p2 * n1 + p1] : (*fac)[(size_t)p1 * n2 + p2];
+          ln_scr = std::log((double)(n1 * n2) * f) + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
+          if (ln_scr < ln_prec) continue;
+        }
         if (method == L2G_SCREEN_ORIGINAL && ln_scr < ln_prec) continue;
         double Pp[3];
         for (int k = 0; k < 3; ++k) Pp[k] = (AB2 == 0.0) ? A[k] : (a1 * A[k] + a2 * Bo[k]) * oog;
@@ -398,6 +424,14 @@ int l2g_init(int device, l2g_context* out) {
     for (int i = 0; i < l2g_gen_nclasses; ++i) {
       const auto& c = l2g_gen_classes[i];
       ctx->class_of[c.la][c.lb][c.lc][c.ld] = i;
+    }
+    for (auto& a : ctx->fock_of)
+      for (auto& b : a)
+        for (auto& c : b)
+          for (auto& d : c) d = -1;
+    for (int i = 0; i < l2g_gen_nfock; ++i) {
+      const auto& c = l2g_gen_fock[i];
+      if (c.launch) ctx->fock_of[c.la][c.lb][c.lc][c.ld] = i;
     }
     for (auto& a : ctx->schwarz_of)
       for (auto& b : a) b = -1;
@@ -799,6 +833,424 @@ int l2g_schwarz(l2g_basis B, int64_t npairs, const int32_t* pairs, double precis
     if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Schwarz kernel failed: ") + hipGetErrorString(e));
     HIPCHK(hipMemcpy(K, dK.p, npairs * sizeof(double), hipMemcpyDeviceToHost));
   });
+}
+
+
+}  // extern "C"
+
+namespace {
+
+// Overlap of two contracted shells by Obara-Saika in each direction,
+// S_{i+1,j} = X_PA S_ij + (i S_{i-1,j} + j S_{i,j-1}) / (2p); Frobenius norm
+// of the (solid-harmonic transformed) block.  Used for the significant
+// shell-pair list of compute_shellpairs (hartree-fock++.cc:1305-1436).
+double overlap_norm(const l2g_basis_s& B, int a, int b) {
+  const int la = B.l[a], lb = B.l[b];
+  const int na = ncart(la), nb = ncart(lb);
+  std::vector<double> S((size_t)na * nb, 0.0);
+  double s1d[3][8][8];
+  for (int pa = 0; pa < B.nprim[a]; ++pa)
+    for (int pb = 0; pb < B.nprim[b]; ++pb) {
+      const double al = B.alpha[B.off[a] + pa], be = B.alpha[B.off[b] + pb], p = al + be;
+      const double mu = al * be / p, oo2p = 0.5 / p;
+      for (int d = 0; d < 3; ++d) {
+        const double A = B.origin[3 * a + d], Bc = B.origin[3 * b + d];
+        const double X = A - Bc, P = (al * A + be * Bc) / p;
+        const double PA = P - A, PB = P - Bc;
+        s1d[d][0][0] = std::sqrt(M_PI / p) * std::exp(-mu * X * X);
+        for (int i = 0; i <= la; ++i)
+          for (int j = 0; j <= lb; ++j) {
+            if (i == 0 && j == 0) continue;
+            double v;
+            if (i > 0) {
+              v = PA * s1d[d][i - 1][j];
+              if (i > 1) v += (i - 1) * oo2p * s1d[d][i - 2][j];
+              if (j > 0) v += j * oo2p * s1d[d][i - 1][j - 1];
+            } else {
+              v = PB * s1d[d][i][j - 1];
+              if (j > 1) v += (j - 1) * oo2p * s1d[d][i][j - 2];
+            }
+            s1d[d][i][j] = v;
+          }
+      }
+      const double c = B.coeff[B.off[a] + pa] * B.coeff[B.off[b] + pb];
+      int ia = 0;
+      for (int x1 = la; x1 >= 0; --x1)
+        for (int y1 = la - x1; y1 >= 0; --y1, ++ia) {
+          const int z1 = la - x1 - y1;
+          int ib = 0;
+          for (int x2 = lb; x2 >= 0; --x2)
+            for (int y2 = lb - x2; y2 >= 0; --y2, ++ib) {
+              const int z2 = lb - x2 - y2;
+              S[(size_t)ia * nb + ib] += c * s1d[0][x1][x2] * s1d[1][y1][y2] * s1d[2][z1][z2];
+            }
+        }
+    }
+  auto sph = [&](int l, std::vector<double>& M) {
+    M.assign((size_t)(2 * l + 1) * ncart(l), 0.0);
+    for (int m = -l; m <= l; ++m) {
+      int ic = 0;
+      for (int x = l; x >= 0; --x)
+        for (int y = l - x; y >= 0; --y, ++ic) M[(size_t)(m + l) * ncart(l) + ic] = l2g_solid_harmonics_coeff(l, m, x, y, l - x - y);
+    }
+  };
+  int n1 = na, n2 = nb;
+  std::vector<double> cur = S, M;
+  if (B.pure[a] && la > 0) {
+    sph(la, M);
+    std::vector<double> nx((size_t)(2 * la + 1) * n2, 0.0);
+    for (int r = 0; r < 2 * la + 1; ++r)
+      for (int c = 0; c < n1; ++c)
+        for (int j = 0; j < n2; ++j) nx[(size_t)r * n2 + j] += M[(size_t)r * n1 + c] * cur[(size_t)c * n2 + j];
+    cur.swap(nx);
+    n1 = 2 * la + 1;
+  }
+  if (B.pure[b] && lb > 0) {
+    sph(lb, M);
+    std::vector<double> nx((size_t)n1 * (2 * lb + 1), 0.0);
+    for (int i = 0; i < n1; ++i)
+      for (int r = 0; r < 2 * lb + 1; ++r)
+        for (int c = 0; c < n2; ++c) nx[(size_t)i * (2 * lb + 1) + r] += M[(size_t)r * n2 + c] * cur[(size_t)i * n2 + c];
+    cur.swap(nx);
+  }
+  double nrm = 0;
+  for (double v : cur) nrm += v * v;
+  return std::sqrt(nrm);
+}
+
+template <class F>
+void parallel_for(int n, F&& f) {
+  const int nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  std::atomic<int> next{0};
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (int i = next++; i < n; i = next++) f(i);
+    });
+  for (auto& t : th) t.join();
+}
+
+// launches one generated kernel family over per-class work lists
+struct WorkLists {
+  std::vector<int> bra, ket, meta;
+  std::vector<long long> off;
+  std::vector<double> lnp, scale;
+  std::vector<int4> bf;
+};
+
+}  // namespace
+
+struct l2g_fock_s {
+  l2g_basis_s* basis;
+  int method;
+  double max_engine_precision;
+  int nbf = 0;
+  std::vector<int> bf1st;
+  std::vector<std::vector<int>> splist;   // splist[s1] ascending s2 <= s1
+  std::vector<double> K;                  // nsh x nsh Schwarz (inf norm), significant pairs only
+  std::vector<std::vector<double>> pfac;  // primitive Schwarz factors per significant pair (loop order)
+  std::vector<uint64_t> pfac_key;
+  int64_t last_nq = 0, last_nprim = 0;
+  float last_ms = 0;
+  DevBuf dens, jg, kg;
+};
+
+extern "C" {
+
+int l2g_fock_create(l2g_basis B, double pair_threshold, double max_engine_precision, int screening, l2g_fock* out) {
+  return guarded([&] {
+    if (!B || !out) throw std::invalid_argument("l2g_fock_create: null argument");
+    if (screening != L2G_SCREEN_ORIGINAL && screening != L2G_SCREEN_CONSERVATIVE && screening != L2G_SCREEN_SCHWARZ &&
+        screening != L2G_SCREEN_SCHWARZINF)
+      throw std::invalid_argument("unknown screening method");
+    auto f = std::make_unique<l2g_fock_s>();
+    f->basis = B;
+    f->method = screening;
+    f->max_engine_precision = max_engine_precision;
+    const int n = B->nshell;
+    f->bf1st.resize(n + 1);
+    f->bf1st[0] = 0;
+    for (int s = 0; s < n; ++s) f->bf1st[s + 1] = f->bf1st[s] + nfun(B->l[s], B->pure[s]);
+    f->nbf = f->bf1st[n];
+    // significant shell pairs: same centre, or overlap-block norm >= threshold
+    f->splist.assign(n, {});
+    parallel_for(n, [&](int s1) {
+      for (int s2 = 0; s2 <= s1; ++s2) {
+        const bool same = B->origin[3 * s1] == B->origin[3 * s2] && B->origin[3 * s1 + 1] == B->origin[3 * s2 + 1] &&
+                          B->origin[3 * s1 + 2] == B->origin[3 * s2 + 2];
+        if (same || overlap_norm(*B, s1, s2) >= pair_threshold) f->splist[s1].push_back(s2);
+      }
+    });
+    std::vector<int32_t> pairs;
+    for (int s1 = 0; s1 < n; ++s1)
+      for (int s2 : f->splist[s1]) {
+        pairs.push_back(s1);
+        pairs.push_back(s2);
+      }
+    const int64_t np = (int64_t)pairs.size() / 2;
+    // Schwarz factors of the significant pairs, no primitive screening
+    std::vector<double> Kp(np);
+    int rc = l2g_schwarz(B, np, pairs.data(), 0.0, 0, Kp.data());
+    if (rc) throw L2gError(rc, g_err);
+    f->K.assign((size_t)n * n, 0.0);
+    for (int64_t i = 0; i < np; ++i) {
+      const int a = pairs[2 * i], b = pairs[2 * i + 1];
+      f->K[(size_t)a * n + b] = f->K[(size_t)b * n + a] = Kp[i];
+    }
+    // primitive-pair Schwarz factors from normalization-free primitives
+    // (Shell::extract_primitive(p, false), shell.h:938-947), on the GPU
+    if (screening == L2G_SCREEN_SCHWARZ || screening == L2G_SCREEN_SCHWARZINF) {
+      std::vector<int> pl, pp, pn;
+      std::vector<double> pa, pc, po;
+      std::vector<int> first(n);
+      for (int s = 0; s < n; ++s) {
+        first[s] = (int)pl.size();
+        for (int k = 0; k < B->nprim[s]; ++k) {
+          pl.push_back(B->l[s]);
+          pp.push_back(B->pure[s]);
+          pn.push_back(1);
+          pa.push_back(B->alpha[B->off[s] + k]);
+          pc.push_back(1.0);
+          for (int d = 0; d < 3; ++d) po.push_back(B->origin[3 * s + d]);
+        }
+      }
+      l2g_basis PB = nullptr;
+      rc = l2g_basis_create(B->ctx, (int)pl.size(), pl.data(), pp.data(), pn.data(), pa.data(), pc.data(), po.data(), &PB);
+      if (rc) throw L2gError(rc, g_err);
+      std::unique_ptr<l2g_basis_s> hold(PB);
+      std::vector<int32_t> ppairs;
+      for (int64_t i = 0; i < np; ++i) {
+        const int a = pairs[2 * i], b = pairs[2 * i + 1];
+        for (int p1 = 0; p1 < B->nprim[a]; ++p1)
+          for (int p2 = 0; p2 < B->nprim[b]; ++p2) {
+            ppairs.push_back(first[a] + p1);
+            ppairs.push_back(first[b] + p2);
+          }
+      }
+      std::vector<double> kf(ppairs.size() / 2);
+      rc = l2g_schwarz(PB, (int64_t)kf.size(), ppairs.data(), 0.0, screening == L2G_SCREEN_SCHWARZINF ? 0 : 1, kf.data());
+      if (rc) throw L2gError(rc, g_err);
+      f->pfac.resize(np);
+      f->pfac_key.resize(np);
+      size_t k = 0;
+      for (int64_t i = 0; i < np; ++i) {
+        const int a = pairs[2 * i], b = pairs[2 * i + 1];
+        f->pfac[i].assign(kf.begin() + k, kf.begin() + k + (size_t)B->nprim[a] * B->nprim[b]);
+        k += (size_t)B->nprim[a] * B->nprim[b];
+        f->pfac_key[i] = (uint64_t)(uint32_t)a << 32 | (uint32_t)b;
+      }
+    }
+    *out = f.release();
+  });
+}
+
+int l2g_fock_nbf(l2g_fock f, int* nbf) {
+  return guarded([&] {
+    if (!f || !nbf) throw std::invalid_argument("null argument");
+    *nbf = f->nbf;
+  });
+}
+
+int l2g_fock_schwarz(l2g_fock f, double* K) {
+  return guarded([&] {
+    if (!f || !K) throw std::invalid_argument("null argument");
+    std::memcpy(K, f->K.data(), f->K.size() * sizeof(double));
+  });
+}
+
+int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, double* K, double* G) {
+  return guarded([&] {
+    if (!f || !D) throw std::invalid_argument("l2g_fock_build: null argument");
+    l2g_basis_s* B = f->basis;
+    l2g_context_s* ctx = B->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = B->nshell, N = f->nbf;
+    // shell-block infinity norms of D (compute_shellblock_norm)
+    std::vector<double> Dn((size_t)n * n, 0.0);
+    double dmax = 0;
+    for (int s1 = 0; s1 < n; ++s1)
+      for (int s2 = 0; s2 < n; ++s2) {
+        double m = 0;
+        for (int i = f->bf1st[s1]; i < f->bf1st[s1 + 1]; ++i)
+          for (int j = f->bf1st[s2]; j < f->bf1st[s2 + 1]; ++j) m = std::max(m, std::fabs(D[(size_t)i * N + j]));
+        Dn[(size_t)s1 * n + s2] = m;
+        dmax = std::max(dmax, m);
+      }
+    const double needed = dmax > 0 ? precision / dmax : precision;
+    if (needed < f->max_engine_precision)
+      throw std::invalid_argument("requested Fock precision needs more engine precision than the shell-pair data hold");
+    PairBuilder pb(*B, std::log(f->max_engine_precision), f->method);
+    for (size_t i = 0; i < f->pfac.size(); ++i) pb.pfac[f->pfac_key[i]] = &f->pfac[i];
+    // permutationally-unique screened quartets (hartree-fock++.cc:1628-1696)
+    std::vector<WorkLists> W(l2g_gen_nfock);
+    int64_t nq = 0, nprim = 0;
+    for (int s1 = 0; s1 < n; ++s1)
+      for (int s2 : f->splist[s1]) {
+        const double Dn12 = Dn[(size_t)s1 * n + s2];
+        for (int s3 = 0; s3 <= s1; ++s3) {
+          const double Dn123 = std::max(std::max(Dn[(size_t)s1 * n + s3], Dn[(size_t)s2 * n + s3]), Dn12);
+          const int s4max = (s1 == s3) ? s2 : s3;
+          for (int s4 : f->splist[s3]) {
+            if (s4 > s4max) break;
+            const double Dn1234 = std::max(std::max(Dn[(size_t)s1 * n + s4], Dn[(size_t)s2 * n + s4]),
+                                           std::max(Dn[(size_t)s3 * n + s4], Dn123));
+            if (Dn1234 * f->K[(size_t)s1 * n + s2] * f->K[(size_t)s3 * n + s4] < precision) continue;
+            const double qprec = Dn1234 != 0. ? precision / Dn1234 : needed;
+            const int deg = (s1 == s2 ? 1 : 2) * (s3 == s4 ? 1 : 2) * (s1 == s3 ? (s2 == s4 ? 1 : 2) : 2);
+            // canonical order (engine.impl.h:1181-1208)
+            const int l1 = B->l[s1], l2 = B->l[s2], l3 = B->l[s3], l4 = B->l[s4];
+            const bool stb = l1 < l2, stk = l3 < l4, sbk = (l1 + l2) > (l3 + l4);
+            const int b1 = sbk ? (stk ? s4 : s3) : (stb ? s2 : s1);
+            const int b2 = sbk ? (stk ? s3 : s4) : (stb ? s1 : s2);
+            const int k1 = sbk ? (stb ? s2 : s1) : (stk ? s4 : s3);
+            const int k2 = sbk ? (stb ? s1 : s2) : (stk ? s3 : s4);
+            const int cls = ctx->fock_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+            if (cls < 0) throw L2gError(L2G_E_LMAX, "no Fock kernel for this angular momentum class");
+            auto purebit = [&](int sh) { return (B->pure[sh] && B->l[sh] > 0) ? 1 : 0; };
+            WorkLists& w = W[cls];
+            const int bp = pb.get(b1, b2), kp = pb.get(k1, k2);
+            w.bra.push_back(bp);
+            w.ket.push_back(kp);
+            w.meta.push_back((purebit(b1) << 3) | (purebit(b2) << 4) | (purebit(k1) << 5) | (purebit(k2) << 6));
+            w.off.push_back(0);
+            w.lnp.push_back(qprec > 0 ? std::log(qprec) : std::numeric_limits<double>::lowest());
+            w.scale.push_back((double)deg);
+            w.bf.push_back(int4{f->bf1st[b1], f->bf1st[b2], f->bf1st[k1], f->bf1st[k2]});
+            ++nq;
+            nprim += (int64_t)pb.np[bp] * pb.np[kp];
+          }
+        }
+      }
+    f->last_nq = nq;
+    f->last_nprim = nprim;
+    hipStream_t st = ctx->stream;
+    // device data
+    DevBuf pf, pn, pab, pP, pPA, pAux, pNs, yb;
+    upload(pf, pb.first, st);
+    upload(pn, pb.np, st);
+    upload(pab, pb.AB, st);
+    upload(pP, pb.P, st);
+    upload(pPA, pb.PA, st);
+    upload(pAux, pb.aux, st);
+    upload(pNs, pb.ns, st);
+    const size_t NN = (size_t)N * N;
+    f->dens.alloc(NN * sizeof(double));
+    f->jg.alloc(NN * sizeof(double));
+    f->kg.alloc(NN * sizeof(double));
+    HIPCHK(hipMemcpyAsync(f->dens.p, D, NN * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(f->jg.p, 0, NN * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(f->kg.p, 0, NN * sizeof(double), st));
+    const int block = 128;
+    int64_t maxn = 0, ymax = 0;
+    for (int c = 0; c < l2g_gen_nfock; ++c) {
+      maxn = std::max<int64_t>(maxn, (int64_t)W[c].bra.size());
+      if (!W[c].bra.empty()) ymax = std::max<int64_t>(ymax, l2g_gen_fock[c].ybuf_doubles);
+    }
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
+    const int64_t ystride = (int64_t)grid * block;
+    yb.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
+    std::vector<std::unique_ptr<DevBuf[]>> keep;
+    std::deque<WorkLists> hostkeep;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int c = 0; c < l2g_gen_nfock; ++c) {
+      WorkLists& w = W[c];
+      if (w.bra.empty()) continue;
+      // sort by contraction depth for SIMT uniformity
+      const int64_t m = (int64_t)w.bra.size();
+      std::vector<int64_t> ord(m);
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+        return (int64_t)pb.np[w.bra[a]] * pb.np[w.ket[a]] > (int64_t)pb.np[w.bra[b]] * pb.np[w.ket[b]];
+      });
+      // host arrays must outlive the asynchronous uploads: keep them until
+      // the stream is synchronised below
+      hostkeep.emplace_back();
+      WorkLists& s = hostkeep.back();
+      for (int64_t i : ord) {
+        s.bra.push_back(w.bra[i]);
+        s.ket.push_back(w.ket[i]);
+        s.meta.push_back(w.meta[i]);
+        s.off.push_back(0);
+        s.lnp.push_back(w.lnp[i]);
+        s.scale.push_back(w.scale[i]);
+        s.bf.push_back(w.bf[i]);
+      }
+      keep.emplace_back(new DevBuf[7]);
+      DevBuf* d = keep.back().get();
+      upload(d[0], s.bra, st);
+      upload(d[1], s.ket, st);
+      upload(d[2], s.meta, st);
+      upload(d[3], s.off, st);
+      upload(d[4], s.lnp, st);
+      upload(d[5], s.scale, st);
+      upload(d[6], s.bf, st);
+      L2gEriArgs A;
+      std::memset(&A, 0, sizeof(A));
+      A.bra_pair = d[0].as<int>();
+      A.ket_pair = d[1].as<int>();
+      A.meta = d[2].as<int>();
+      A.out_off = d[3].as<long long>();
+      A.n = (int)m;
+      A.pair_first = pf.as<int>();
+      A.pair_np = pn.as<int>();
+      A.pair_AB = pab.as<double>();
+      A.pp_P = pP.as<double4>();
+      A.pp_PA = pPA.as<double4>();
+      A.pp_aux = pAux.as<double2>();
+      A.pp_nonsph = pNs.as<double>();
+      A.boys = ctx->boys.as<double>();
+      A.scale = 1.0;
+      A.method = f->method;
+      A.shg_gaussian = ctx->shg_gaussian;
+      A.ybuf = yb.as<double>();
+      A.ystride = ystride;
+      A.q_lnprec = d[4].as<double>();
+      A.q_scale = d[5].as<double>();
+      A.q_bf = d[6].as<int4>();
+      A.dens = f->dens.as<double>();
+      A.jg = f->jg.as<double>();
+      A.kg = f->kg.as<double>();
+      A.nbf = N;
+      l2g_gen_fock[c].launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(e1, st));
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Fock kernels failed: ") + hipGetErrorString(e));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    f->last_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<double> jg(NN), kg(NN);
+    HIPCHK(hipMemcpy(jg.data(), f->jg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(kg.data(), f->kg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        const double Jv = 0.25 * (jg[(size_t)i * N + j] + jg[(size_t)j * N + i]);
+        const double Kv = 0.125 * (kg[(size_t)i * N + j] + kg[(size_t)j * N + i]);
+        if (J) J[(size_t)i * N + j] = Jv;
+        if (K) K[(size_t)i * N + j] = Kv;
+        if (G) G[(size_t)i * N + j] = 2 * Jv - Kv;
+      }
+  });
+}
+
+int l2g_fock_stats(l2g_fock f, int64_t* nquartets, int64_t* nprim_quartets, float* kernel_ms) {
+  return guarded([&] {
+    if (!f) throw std::invalid_argument("null argument");
+    if (nquartets) *nquartets = f->last_nq;
+    if (nprim_quartets) *nprim_quartets = f->last_nprim;
+    if (kernel_ms) *kernel_ms = f->last_ms;
+  });
+}
+
+int l2g_fock_destroy(l2g_fock f) {
+  delete f;
+  return L2G_OK;
 }
 
 int l2g_eri_compute(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double* host_out,

@@ -68,11 +68,11 @@ def lib():
         L.oracle_eri_quartets.restype = ctypes.c_int64
         L.oracle_eri_brute.argtypes = basis_args + [ctypes.c_int64, _i32p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_eri_brute.restype = ctypes.c_int64
-        if hasattr(L, "oracle_fock_2e"):
-            L.oracle_fock_2e.argtypes = basis_args + [
-                _f64p, ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int,
-                ctypes.POINTER(ctypes.c_int64)]
-            L.oracle_fock_2e.restype = ctypes.c_int
+        L.oracle_fock_2e.argtypes = basis_args + [
+            _f64p, ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.oracle_fock_2e.restype = ctypes.c_int
+        L.oracle_fock_pairs.argtypes = basis_args + [ctypes.c_double, _i8p, _f64p]
+        L.oracle_fock_pairs.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -162,3 +162,27 @@ def eri_brute(basis, quartets, nthreads=0):
     out = np.zeros(max(total, 1))
     L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), out.ctypes.data, nthreads)
     return out[:total]
+
+
+def fock_2e(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0):
+    """Restated compute_2body_fock: returns (G, J, K, n_quartets) with
+    G = 2J - K (tests/hartree-fock/hartree-fock++.cc:1574-1772)."""
+    ob = OBasis.of(basis)
+    D = np.ascontiguousarray(D, np.float64)
+    N = D.shape[0]
+    G = np.zeros((N, N))
+    JK = np.zeros((2, N, N))
+    nq = ctypes.c_int64()
+    lib().oracle_fock_2e(*ob.args(), D.reshape(-1), precision, G.reshape(-1), screening, JK.reshape(-1),
+                         nthreads, ctypes.byref(nq))
+    return G, JK[0], JK[1], nq.value
+
+
+def fock_pairs(basis, threshold=1e-12):
+    """(significant-pair mask (s1 >= s2), Schwarz matrix)."""
+    ob = OBasis.of(basis)
+    n = len(ob.l)
+    mask = np.zeros((n, n), np.int8)
+    K = np.zeros((n, n))
+    lib().oracle_fock_pairs(*ob.args(), threshold, mask.reshape(-1), K.reshape(-1))
+    return mask, K

@@ -299,6 +299,41 @@ struct PrimPair {
 struct ShellPair {
   std::vector<PrimPair> pp;
   double AB[3];
+  // Schwarz / SchwarzInf variant (shell.h:1258-1328): factor[p1*n2+p2] is the
+  // primitive-pair Schwarz factor supplied by the caller
+  void init_schwarz(const Shell& s1, const Shell& s2, double ln_prec, const double* factor) {
+    pp.clear();
+    const auto& A = s1.O;
+    const auto& B = s2.O;
+    double AB2 = 0;
+    for (int i = 0; i < 3; ++i) {
+      AB[i] = A[i] - B[i];
+      AB2 += AB[i] * AB[i];
+    }
+    const int n1 = s1.nprim(), n2 = s2.nprim();
+    for (int p1 = 0; p1 < n1; ++p1)
+      for (int p2 = 0; p2 < n2; ++p2) {
+        const double ln_scr =
+            std::log((double)(n1 * n2) * factor[p1 * n2 + p2]) + s1.max_ln_coeff[p1] + s2.max_ln_coeff[p2];
+        if (ln_scr < ln_prec) continue;
+        const double a1 = s1.alpha[p1], a2 = s2.alpha[p2];
+        const double gamma = a1 + a2, oogamma = 1 / gamma;
+        const double rho = a1 * a2 * oogamma;
+        PrimPair q;
+        if (AB2 == 0.) {
+          for (int i = 0; i < 3; ++i) q.P[i] = A[i];
+        } else {
+          for (int i = 0; i < 3; ++i) q.P[i] = (a1 * A[i] + a2 * B[i]) * oogamma;
+        }
+        q.ln_scr = ln_scr;
+        q.p1 = p1;
+        q.p2 = p2;
+        q.K = 5.9149671727956128778 * std::exp(-rho * AB2) * oogamma;
+        q.nonsph = 0;
+        q.one_over_gamma = oogamma;
+        pp.push_back(q);
+      }
+  }
   void init(const Shell& s1, const Shell& s2, double ln_prec, int method) {
     pp.clear();
     const auto& A = s1.O;
@@ -926,6 +961,223 @@ static long double md_primitive(const int* a, const int* b, const int* c, const 
   return 2 * powl(PI, 2.5L) / (p * q * sqrtl(p + q)) * sum;
 }
 
+
+// ----------------------------------------------------------------------------
+// overlap of two contracted shells (for the significant shell-pair list of
+// compute_shellpairs, tests/hartree-fock/hartree-fock++.cc:1305-1436), by
+// Hermite expansion: S_ij = E^{ij}_0 sqrt(pi/p) per Cartesian direction.
+// Returns the Frobenius norm of the (sph-transformed) block.
+// ----------------------------------------------------------------------------
+static double overlap_block_norm(const Shell& a, const Shell& b) {
+  auto ca = cart_comps(a.l), cb = cart_comps(b.l);
+  std::vector<double> S(ca.size() * cb.size(), 0.0);
+  std::vector<long double> E[3];
+  for (int pa = 0; pa < a.nprim(); ++pa)
+    for (int pb = 0; pb < b.nprim(); ++pb) {
+      const long double al = a.alpha[pa], be = b.alpha[pb], p = al + be;
+      for (int d = 0; d < 3; ++d) md_E(a.l, b.l, al, be, (long double)a.O[d] - b.O[d], E[d]);
+      const long double pre = powl(3.141592653589793238462643383279502884L / p, 1.5L) * a.coeff[pa] * b.coeff[pb];
+      const int T = a.l + b.l + 1;
+      for (size_t i = 0; i < ca.size(); ++i)
+        for (size_t j = 0; j < cb.size(); ++j) {
+          long double v = pre;
+          for (int d = 0; d < 3; ++d) v *= E[d][((size_t)ca[i][d] * (b.l + 1) + cb[j][d]) * (T + 1)];
+          S[i * cb.size() + j] += (double)v;
+        }
+    }
+  // sph transforms
+  std::vector<double> cur = S;
+  int n1 = (int)ca.size(), n2 = (int)cb.size();
+  if (a.pure && a.l > 0) {
+    auto M = sh_matrix(a.l);
+    std::vector<double> nx((2 * a.l + 1) * n2, 0.0);
+    for (int s = 0; s < 2 * a.l + 1; ++s)
+      for (int c = 0; c < n1; ++c)
+        for (int j = 0; j < n2; ++j) nx[s * n2 + j] += M[s * n1 + c] * cur[c * n2 + j];
+    cur.swap(nx);
+    n1 = 2 * a.l + 1;
+  }
+  if (b.pure && b.l > 0) {
+    auto M = sh_matrix(b.l);
+    std::vector<double> nx(n1 * (2 * b.l + 1), 0.0);
+    for (int i = 0; i < n1; ++i)
+      for (int s = 0; s < 2 * b.l + 1; ++s)
+        for (int c = 0; c < n2; ++c) nx[i * (2 * b.l + 1) + s] += M[s * n2 + c] * cur[i * n2 + c];
+    cur.swap(nx);
+  }
+  double nrm = 0;
+  for (double v : cur) nrm += v * v;
+  return std::sqrt(nrm);
+}
+
+struct FockSetup {
+  std::vector<std::vector<int>> splist;                    // splist[s1] = sorted s2 <= s1
+  std::vector<std::vector<std::unique_ptr<ShellPair>>> sp;  // parallel to splist
+  std::vector<double> K;                                   // nsh x nsh Schwarz (inf norm)
+  std::vector<int> bf1st;
+  int nbf = 0;
+};
+
+// compute_shellpairs + compute_schwarz_ints (hartree-fock++.cc:1231-1436)
+static void fock_setup(const std::vector<Shell>& sh, double threshold, double max_engine_precision, int method,
+                       FockSetup& F) {
+  const int n = (int)sh.size();
+  F.bf1st.resize(n + 1);
+  F.bf1st[0] = 0;
+  for (int s = 0; s < n; ++s) F.bf1st[s + 1] = F.bf1st[s] + sh[s].size();
+  F.nbf = F.bf1st[n];
+  F.splist.assign(n, {});
+  F.sp.clear();
+  F.sp.resize(n);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int s1 = 0; s1 < n; ++s1)
+    for (int s2 = 0; s2 <= s1; ++s2) {
+      const bool same = sh[s1].O == sh[s2].O;
+      if (same || overlap_block_norm(sh[s1], sh[s2]) >= threshold) F.splist[s1].push_back(s2);
+    }
+  // Schwarz matrix, infinity norm, no primitive screening
+  F.K.assign((size_t)n * n, 0.0);
+#pragma omp parallel
+  {
+    Engine e;
+    e.set_precision(0.0);
+#pragma omp for schedule(dynamic, 1)
+    for (int s1 = 0; s1 < n; ++s1)
+      for (int s2 = 0; s2 <= s1; ++s2) {
+        e.compute2(sh[s1], sh[s2], sh[s1], sh[s2]);
+        double mx = 0;
+        for (double v : e.result) mx = std::max(mx, std::abs(v));
+        F.K[(size_t)s1 * n + s2] = F.K[(size_t)s2 * n + s1] = std::sqrt(mx);
+      }
+  }
+  // shell-pair data at max_engine_precision
+  const double lnp = std::log(max_engine_precision);
+#pragma omp parallel
+  {
+    Engine e;
+    e.set_precision(0.0);
+#pragma omp for schedule(dynamic, 1)
+    for (int s1 = 0; s1 < n; ++s1) {
+      for (int s2 : F.splist[s1]) {
+        auto p = std::make_unique<ShellPair>();
+        if (method == Original || method == Conservative) {
+          p->init(sh[s1], sh[s2], lnp, method);
+        } else {
+          // primitive Schwarz factors from normalization-free primitives
+          // (Shell::extract_primitive(p, false), shell.h:938-947)
+          const int n1 = sh[s1].nprim(), n2 = sh[s2].nprim();
+          std::vector<double> fac(n1 * n2);
+          for (int p1 = 0; p1 < n1; ++p1)
+            for (int p2 = 0; p2 < n2; ++p2) {
+              Shell a = sh[s1], b = sh[s2];
+              a.alpha = {sh[s1].alpha[p1]};
+              a.coeff = {1.0};
+              a.max_ln_coeff = {0.0};
+              b.alpha = {sh[s2].alpha[p2]};
+              b.coeff = {1.0};
+              b.max_ln_coeff = {0.0};
+              double mx = 0, ss = 0;
+              if (e.compute2(a, b, a, b))
+                for (double v : e.result) {
+                  mx = std::max(mx, std::abs(v));
+                  ss += v * v;
+                }
+              fac[p1 * n2 + p2] = std::sqrt(method == SchwarzInf ? mx : std::sqrt(ss));
+            }
+          p->init_schwarz(sh[s1], sh[s2], lnp, fac.data());
+        }
+        F.sp[s1].push_back(std::move(p));
+      }
+    }
+  }
+}
+
+// compute_2body_fock (hartree-fock++.cc:1574-1772): returns G = 2J - K and
+// optionally J and K (J_ab = sum (ab|cd) D_cd, K_ab = sum (ac|bd) D_cd).
+static void fock_build(const std::vector<Shell>& sh, const FockSetup& F, const double* D, double precision,
+                       int method, double* G, double* J, double* K, int64_t* nq_out) {
+  const int n = (int)sh.size();
+  const int N = F.nbf;
+  std::vector<double> Dn((size_t)n * n, 0.0);
+  for (int s1 = 0; s1 < n; ++s1)
+    for (int s2 = 0; s2 < n; ++s2) {
+      double m = 0;
+      for (int i = F.bf1st[s1]; i < F.bf1st[s1 + 1]; ++i)
+        for (int j = F.bf1st[s2]; j < F.bf1st[s2 + 1]; ++j) m = std::max(m, std::abs(D[(size_t)i * N + j]));
+      Dn[(size_t)s1 * n + s2] = m;
+    }
+  double dmax = 0;
+  for (double v : Dn) dmax = std::max(dmax, v);
+  const double needed = precision / dmax;
+  std::vector<double> jg((size_t)N * N, 0.0), kg((size_t)N * N, 0.0);
+  int64_t nq = 0;
+#pragma omp parallel reduction(+ : nq)
+  {
+    Engine e;
+    e.screening = method;
+    std::vector<double> jl((size_t)N * N, 0.0), kl((size_t)N * N, 0.0);
+#pragma omp for schedule(dynamic, 1)
+    for (int s1 = 0; s1 < n; ++s1) {
+      const int b1 = F.bf1st[s1], n1 = sh[s1].size();
+      for (size_t i2 = 0; i2 < F.splist[s1].size(); ++i2) {
+        const int s2 = F.splist[s1][i2];
+        const int b2 = F.bf1st[s2], n2 = sh[s2].size();
+        const ShellPair* sp12 = F.sp[s1][i2].get();
+        const double Dn12 = Dn[(size_t)s1 * n + s2];
+        for (int s3 = 0; s3 <= s1; ++s3) {
+          const int b3 = F.bf1st[s3], n3 = sh[s3].size();
+          const double Dn123 = std::max(std::max(Dn[(size_t)s1 * n + s3], Dn[(size_t)s2 * n + s3]), Dn12);
+          const int s4max = (s1 == s3) ? s2 : s3;
+          for (size_t i4 = 0; i4 < F.splist[s3].size(); ++i4) {
+            const int s4 = F.splist[s3][i4];
+            if (s4 > s4max) break;
+            const ShellPair* sp34 = F.sp[s3][i4].get();
+            const double Dn1234 =
+                std::max(std::max(Dn[(size_t)s1 * n + s4], Dn[(size_t)s2 * n + s4]), std::max(Dn[(size_t)s3 * n + s4], Dn123));
+            if (Dn1234 * F.K[(size_t)s1 * n + s2] * F.K[(size_t)s3 * n + s4] < precision) continue;
+            const int b4 = F.bf1st[s4], n4 = sh[s4].size();
+            const double deg = (s1 == s2 ? 1 : 2) * (s3 == s4 ? 1 : 2) * (s1 == s3 ? (s2 == s4 ? 1 : 2) : 2);
+            e.set_precision(Dn1234 != 0. ? precision / Dn1234 : needed);
+            if (!e.compute2(sh[s1], sh[s2], sh[s3], sh[s4], sp12, sp34)) continue;
+            ++nq;
+            const double* buf = e.result.data();
+            for (int f1 = 0, f = 0; f1 < n1; ++f1)
+              for (int f2 = 0; f2 < n2; ++f2)
+                for (int f3 = 0; f3 < n3; ++f3)
+                  for (int f4 = 0; f4 < n4; ++f4, ++f) {
+                    const size_t i1 = b1 + f1, i2b = b2 + f2, i3 = b3 + f3, i4 = b4 + f4;
+                    const double v = buf[f] * deg;
+                    jl[i1 * N + i2b] += D[i3 * N + i4] * v;
+                    jl[i3 * N + i4] += D[i1 * N + i2b] * v;
+                    kl[i1 * N + i3] += D[i2b * N + i4] * v;
+                    kl[i2b * N + i4] += D[i1 * N + i3] * v;
+                    kl[i1 * N + i4] += D[i2b * N + i3] * v;
+                    kl[i2b * N + i3] += D[i1 * N + i4] * v;
+                  }
+          }
+        }
+      }
+    }
+#pragma omp critical
+    {
+      for (size_t i = 0; i < jg.size(); ++i) {
+        jg[i] += jl[i];
+        kg[i] += kl[i];
+      }
+    }
+  }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      const double Jv = 0.25 * (jg[(size_t)i * N + j] + jg[(size_t)j * N + i]);
+      const double Kv = 0.125 * (kg[(size_t)i * N + j] + kg[(size_t)j * N + i]);
+      if (J) J[(size_t)i * N + j] = Jv;
+      if (K) K[(size_t)i * N + j] = Kv;
+      // g = jg - kg / 4, G = (g + g^T) / 2  (hartree-fock++.cc:1730-1766)
+      G[(size_t)i * N + j] = 2 * Jv - Kv;
+    }
+  if (nq_out) *nq_out = nq;
+}
+
 }  // namespace orc
 
 // ============================================================================
@@ -1106,6 +1358,37 @@ int64_t oracle_eri_brute(int nshell, const int* l, const int* pure, const int* n
     for (size_t i = 0; i < cur.size(); ++i) out[off[q] + i] = (double)cur[i];
   }
   return off[nq];
+}
+
+// Screened Fock build restating compute_shellpairs + compute_schwarz_ints +
+// compute_2body_fock.  D, G, J, K are nbf x nbf row-major; J/K may be NULL.
+// splist_out (optional, nsh*nsh bytes) receives the significant-pair mask.
+int oracle_fock_2e(int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                   const double* coeff, const double* origin, const double* D, double precision, double* G,
+                   int method, double* JK /* 2*nbf*nbf or NULL */, int nthreads, int64_t* nq) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  Basis B = make_basis(nshell, l, pure, nprim, alpha, coeff, origin);
+  FockSetup F;
+  fock_setup(B.shells, 1e-12, std::numeric_limits<double>::epsilon() / 1e10, method, F);
+  const size_t NN = (size_t)F.nbf * F.nbf;
+  fock_build(B.shells, F, D, precision, method, G, JK ? JK : nullptr, JK ? JK + NN : nullptr, nq);
+  return F.nbf;
+}
+
+// Significant shell pairs (overlap-norm >= threshold or same centre) and the
+// Schwarz matrix, as the Fock set-up computes them.
+int oracle_fock_pairs(int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                      const double* coeff, const double* origin, double threshold, int8_t* mask, double* K) {
+  Basis B = make_basis(nshell, l, pure, nprim, alpha, coeff, origin);
+  FockSetup F;
+  fock_setup(B.shells, threshold, std::numeric_limits<double>::epsilon() / 1e10, Original, F);
+  std::memset(mask, 0, (size_t)nshell * nshell);
+  for (int s1 = 0; s1 < nshell; ++s1)
+    for (int s2 : F.splist[s1]) mask[(size_t)s1 * nshell + s2] = 1;
+  std::memcpy(K, F.K.data(), sizeof(double) * nshell * nshell);
+  return 0;
 }
 
 }  // extern "C"
