@@ -1,0 +1,59 @@
+"""The C++ façade (include/libint2_amd/engine.h): compiles with g++ against
+the C ABI, its CPU-side error behaviour holds, and (GPU) its results equal
+the oracle's for the same shells."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import libint_amd as la
+import oracle
+from tests.helpers import compare
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "facade_test")
+
+
+def build_exe():
+    src = os.path.join(ROOT, "tests", "cpp", "facade_test.cpp")
+    libdir = os.path.dirname(la.LIBPATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), src, "-o", EXE,
+                    "-L", libdir, "-ll2g", f"-Wl,-rpath,{libdir}"], check=True)
+
+
+def test_facade_compiles_and_cpu_checks():
+    build_exe()
+    r = subprocess.run([EXE, "cpu"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_facade_results_match_oracle():
+    if not os.path.exists(EXE):
+        build_exe()
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    shells = [
+        (0, [0.9, 3.1], [0.4, 0.7], (0.1, 0.2, 0.3), False),
+        (1, [1.3, 0.4], [0.5, 0.6], (1.0, 0.0, -0.2), False),
+        (2, [2.2], [1.0], (-0.3, 0.8, 0.1), True),
+        (2, [0.7, 1.9], [0.3, 0.9], (0.4, -0.5, 0.6), False),
+    ]
+    sh = [la.Shell(l, list(zip(a, c)), o, pure=p) for (l, a, c, o, p) in shells]
+    arrays = la.BasisArrays.from_shells(sh)
+    lines = r.stdout.split("\n")
+    i = 0
+    checked = 0
+    while i < len(lines):
+        if lines[i].startswith("Q "):
+            _, a, b, c, d, n = lines[i].split()
+            n = int(n)
+            vals = np.array([float(x) for x in lines[i + 1:i + 1 + n]])
+            o, _, _ = oracle.eri_quartets(arrays, [[int(a), int(b), int(c), int(d)]])
+            assert compare(vals, o, 1e-12, 1e-10)[0] == 0
+            checked += 1
+            i += n + 1
+        else:
+            i += 1
+    assert checked == 5
