@@ -66,7 +66,9 @@ int l2g_max_l(void);
  * (include/libint2/initialize.h:76-90) for this path. */
 int l2g_init(int device, l2g_context* ctx);
 
-/* Releases the context (libint2::finalize, initialize.h:105-113). */
+/* Releases the context (libint2::finalize, initialize.h:105-113).  Returns
+ * L2G_E_STATE, and keeps the context, while any basis, plan or Fock builder
+ * created on it has not been destroyed. */
 int l2g_finalize(l2g_context ctx);
 
 /* Solid-harmonic ordering of pure shells for results produced after the call
@@ -115,6 +117,16 @@ int l2g_basis_destroy(l2g_basis basis);
 int l2g_eri_plan_create(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision,
                         int screening, double scale, int uniform_cart, l2g_eri_plan* plan);
 
+/* Mixed-precision variant of l2g_eri_plan_create: quartets whose Schwarz
+ * bound K12 K34 (K = sqrt(max|(ab|ab)|), computed on the GPU for the pairs of
+ * the batch) is below fp32_threshold (1e-6 in the mixed-precision workload)
+ * run the recursion, HRR and transforms in FP32; the Boys function and the
+ * prefactors stay FP64, outputs are FP64.  fp32_threshold = 0 is the plain
+ * FP64 plan.  No libint counterpart: libint evaluates everything in FP64. */
+int l2g_eri_plan_create_mixed(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision,
+                              int screening, double scale, int uniform_cart, double fp32_threshold,
+                              l2g_eri_plan* plan);
+
 /* Number of doubles the batch produces. */
 int l2g_eri_plan_size(l2g_eri_plan plan, int64_t* total);
 
@@ -155,6 +167,16 @@ int l2g_eri_plan_destroy(l2g_eri_plan plan);
  * which uses precision 0 (no primitive screening); pass 0 for the same. */
 int l2g_schwarz(l2g_basis basis, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K);
 
+/* The two per-pair factors that select FP32 in mixed batches
+ * (l2g_eri_plan_create_mixed, l2g_fock_set_fp32_threshold): K as
+ * l2g_schwarz with norm 0, and S = sqrt(max_e |[e0|e0]|) (1 + max_x|AB_x|)^lb
+ * from the contracted pre-HRR integrals of the pair in its canonical
+ * orientation.  A quartet runs in FP32 when K_ab K_cd < threshold and
+ * S_ab S_cd < 16 threshold.  No counterpart in the reference, which has no
+ * mixed precision; it is here so callers can see which quartets a
+ * threshold sends to FP32. */
+int l2g_precision_bounds(l2g_basis basis, int64_t npairs, const int32_t* pairs, double* K, double* S);
+
 /* ---- Fock build ---------------------------------------------------------- */
 
 /* Set-up of a screened two-electron Fock build for a basis, restating
@@ -184,6 +206,14 @@ int l2g_fock_build(l2g_fock fock, const double* D, double precision, double* J, 
 /* Counters of the last build: quartets evaluated, primitive quartets (after
  * pair screening) and the GPU time of the digest kernels in ms. */
 int l2g_fock_stats(l2g_fock fock, int64_t* nquartets, int64_t* nprim_quartets, float* kernel_ms);
+
+/* Quartets of later builds whose Schwarz bound K12 K34 is below threshold
+ * are evaluated in FP32 (digest and accumulation stay FP64); 0 = all FP64.
+ * No libint counterpart. */
+int l2g_fock_set_fp32_threshold(l2g_fock fock, double threshold);
+
+/* Number of quartets the last build evaluated in FP32. */
+int l2g_fock_stats_mixed(l2g_fock fock, int64_t* nquartets_fp32);
 
 int l2g_fock_destroy(l2g_fock fock);
 

@@ -21,6 +21,7 @@ from __future__ import annotations
 import ctypes
 import os
 import subprocess
+import weakref
 from typing import Iterable, Sequence
 
 import numpy as np
@@ -30,6 +31,8 @@ LIBPATH = os.environ.get("LIBINT_AMD_LIB") or os.path.join(_HERE, "lib", "libl2g
 
 SCREEN_ORIGINAL, SCREEN_CONSERVATIVE, SCREEN_SCHWARZ, SCREEN_SCHWARZINF = 0x0001, 0x0010, 0x0100, 0x1000
 SHG_STANDARD, SHG_GAUSSIAN = 1, 2
+# error codes of include/l2g.h
+L2G_OK, L2G_E_INVALID, L2G_E_LMAX, L2G_E_NODEVICE, L2G_E_OOM, L2G_E_LAUNCH, L2G_E_STATE = 0, -1, -2, -3, -4, -5, -6
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -69,6 +72,11 @@ ABI = {
     "l2g_fock_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_float)]),
     "l2g_fock_destroy": (ctypes.c_int, [_vp]),
+    "l2g_eri_plan_create_mixed": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "l2g_fock_set_fp32_threshold": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "l2g_fock_stats_mixed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "l2g_precision_bounds": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, _f64p, _f64p]),
 }
 
 _lib = None
@@ -208,6 +216,8 @@ class Context:
         h = _vp()
         _check(lib().l2g_init(device, ctypes.byref(h)))
         self._h = h
+        # live bases, plans and Fock builders: released before the context
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -220,7 +230,16 @@ class Context:
         _check(lib().l2g_synchronize(self._h))
 
     def close(self) -> None:
+        """Releases every object still alive on this context (plans and Fock
+        builders first, then bases), then the context itself."""
         if self._h:
+            kids = list(self._children)
+            for k in kids:
+                if not isinstance(k, Basis):
+                    k._release()
+            for k in kids:
+                if isinstance(k, Basis):
+                    k._release()
             _check(lib().l2g_finalize(self._h))
             self._h = None
 
@@ -240,16 +259,22 @@ class Basis:
         _check(lib().l2g_basis_create(ctx.handle, a.nshell, a.l, a.pure, a.nprim, a.alpha, a.coeff, a.origin,
                                       ctypes.byref(h)))
         self._h = h
+        ctx._children.add(self)
 
     @property
     def handle(self):
+        if not self._h:
+            raise L2gError(L2G_E_STATE, "basis released (its context was closed)")
         return self._h
+
+    def _release(self):
+        if self._h:
+            lib().l2g_basis_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                lib().l2g_basis_destroy(self._h)
-                self._h = None
+            self._release()
         except Exception:
             pass
 
@@ -258,14 +283,20 @@ class EriPlan:
     """A planned batch of shell quartets (caller order)."""
 
     def __init__(self, basis: Basis, quartets, precision: float = np.finfo(float).eps,
-                 screening: int = SCREEN_ORIGINAL, scale: float = 1.0, uniform_cart: bool = False):
+                 screening: int = SCREEN_ORIGINAL, scale: float = 1.0, uniform_cart: bool = False,
+                 fp32_threshold: float = 0.0):
         self.basis = basis
         q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
         self.nq = q.shape[0]
         h = _vp()
-        _check(lib().l2g_eri_plan_create(basis.handle, self.nq, q.reshape(-1), precision, screening, scale,
-                                         int(uniform_cart), ctypes.byref(h)))
+        if fp32_threshold > 0:
+            _check(lib().l2g_eri_plan_create_mixed(basis.handle, self.nq, q.reshape(-1), precision, screening,
+                                                   scale, int(uniform_cart), fp32_threshold, ctypes.byref(h)))
+        else:
+            _check(lib().l2g_eri_plan_create(basis.handle, self.nq, q.reshape(-1), precision, screening, scale,
+                                             int(uniform_cart), ctypes.byref(h)))
         self._h = h
+        basis.ctx._children.add(self)
         n = ctypes.c_int64()
         _check(lib().l2g_eri_plan_size(h, ctypes.byref(n)))
         self.size = n.value
@@ -300,14 +331,17 @@ class EriPlan:
         _check(lib().l2g_eri_plan_stats(self._h, cap, ctypes.byref(n), code.ctypes.data, nq.ctypes.data,
                                         npq.ctypes.data, ms.ctypes.data))
         k = n.value
-        return [dict(cls=f"{c:04d}", nquartets=int(a), nprim=int(b), ms=float(t))
+        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c >= 10000), nquartets=int(a), nprim=int(b), ms=float(t))
                 for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
+
+    def _release(self):
+        if self._h:
+            lib().l2g_eri_plan_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                lib().l2g_eri_plan_destroy(self._h)
-                self._h = None
+            self._release()
         except Exception:
             pass
 
@@ -322,6 +356,20 @@ def schwarz(basis: Basis, pairs, precision: float = 0.0, norm: str = "inf") -> n
     return K[: p.shape[0]]
 
 
+def precision_bounds(basis: Basis, pairs):
+    """(K, S) per shell pair (n, 2): the Schwarz factor and the pre-HRR
+    factor that together select FP32 in mixed batches; a quartet runs in FP32
+    when K_ab K_cd < threshold and S_ab S_cd < 16 threshold."""
+    p = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+    K = np.zeros(max(p.shape[0], 1))
+    S = np.zeros(max(p.shape[0], 1))
+    _check(lib().l2g_precision_bounds(basis.handle, p.shape[0], p.reshape(-1), K, S))
+    return K[: p.shape[0]], S[: p.shape[0]]
+
+
+FP32_HRR_FACTOR = 16.0
+
+
 class FockBuilder:
     """Screened two-electron Fock builds on the GPU for one basis
     (compute_shellpairs + compute_schwarz_ints + compute_2body_fock of
@@ -333,6 +381,7 @@ class FockBuilder:
         h = _vp()
         _check(lib().l2g_fock_create(basis.handle, pair_threshold, max_engine_precision, screening, ctypes.byref(h)))
         self._h = h
+        basis.ctx._children.add(self)
         n = ctypes.c_int()
         _check(lib().l2g_fock_nbf(h, ctypes.byref(n)))
         self.nbf = n.value
@@ -352,24 +401,32 @@ class FockBuilder:
         _check(lib().l2g_fock_build(self._h, D.reshape(-1), precision, J.ctypes.data, K.ctypes.data, G.ctypes.data))
         return G, J, K
 
+    def set_fp32_threshold(self, threshold: float) -> None:
+        """Quartets with Schwarz bound below `threshold` run in FP32."""
+        _check(lib().l2g_fock_set_fp32_threshold(self._h, threshold))
+
     def stats(self):
-        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float()
+        a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float(), ctypes.c_int64()
         _check(lib().l2g_fock_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return dict(nquartets=a.value, nprim=b.value, kernel_ms=c.value)
+        _check(lib().l2g_fock_stats_mixed(self._h, ctypes.byref(d)))
+        return dict(nquartets=a.value, nprim=b.value, kernel_ms=c.value, nquartets_fp32=d.value)
+
+    def _release(self):
+        if self._h:
+            lib().l2g_fock_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                lib().l2g_fock_destroy(self._h)
-                self._h = None
+            self._release()
         except Exception:
             pass
 
 
 def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,
-                     scale=1.0, uniform_cart=False):
+                     scale=1.0, uniform_cart=False, fp32_threshold=0.0):
     """Evaluate a batch; returns (values, offsets, nonzero)."""
-    p = EriPlan(basis, quartets, precision, screening, scale, uniform_cart)
+    p = EriPlan(basis, quartets, precision, screening, scale, uniform_cart, fp32_threshold)
     offs = p.offsets()
     p.execute()
     vals, nz = p.download()

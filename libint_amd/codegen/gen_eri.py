@@ -260,9 +260,20 @@ class ClassPlan:
     ld: int
     ecomps: list
     fcomps: list
-    groups: list            # list of lists of e indices
+    groups: list            # list of (e indices, f indices) target blocks
     group_nodes: list       # node count per group
     group_peak: list
+
+    def full_rows(self, g):
+        return len(self.groups[g][1]) == len(self.fcomps)
+
+    def row_last_group(self, ie):
+        """index of the last group that contributes to bra row ie"""
+        return max(g for g, (es, fs) in enumerate(self.groups) if ie in es)
+
+    @property
+    def split_rows(self):
+        return any(not self.full_rows(g) for g in range(len(self.groups)))
 
     @property
     def L(self):
@@ -278,34 +289,48 @@ class ClassPlan:
 
 
 def plan_class(la, lb, lc, ld, budget=96):
+    """Partition the contracted targets [e0|f0] into blocks whose recursion
+    sub-graph (live intermediates + accumulators) fits `budget` doubles:
+    whole bra rows are merged while they fit; a bra row that alone exceeds
+    the budget is cut into chunks of ket targets."""
     ecomps = [c for l in range(la, la + lb + 1) for c in cart(l)]
     fcomps = [c for l in range(lc, lc + ld + 1) for c in cart(l)]
     nf = len(fcomps)
+    allf = list(range(nf))
     g = VrrGraph()
-    # greedy grouping of bra components; components sorted so that groups
-    # share sub-components
     order_e = sorted(range(len(ecomps)), key=lambda i: ecomps[i], reverse=True)
-    groups = []
-    cur = []
+    order_f = sorted(range(nf), key=lambda j: fcomps[j], reverse=True)
 
-    def cost(es):
-        tg = [g.get(ecomps[i], f, 0) for i in es for f in fcomps]
+    def cost(es, fs):
+        tg = [g.get(ecomps[i], fcomps[j], 0) for i in es for j in fs]
         o = g.order(tg)
         return len(o), peak_live(o, g, tg) + len(tg)
 
+    groups = []
+    cur = []
     for i in order_e:
-        trial = cur + [i]
-        n, pk = cost(trial)
-        if cur and pk > budget:
-            groups.append(cur)
+        if cost(cur + [i], allf)[1] <= budget:
+            cur.append(i)
+            continue
+        if cur:
+            groups.append((cur, allf))
+            cur = []
+        if cost([i], allf)[1] <= budget:
             cur = [i]
-        else:
-            cur = trial
+            continue
+        fcur = []
+        for j in order_f:
+            if not fcur or cost([i], fcur + [j])[1] <= budget:
+                fcur.append(j)
+            else:
+                groups.append(([i], sorted(fcur)))
+                fcur = [j]
+        groups.append(([i], sorted(fcur)))
     if cur:
-        groups.append(cur)
+        groups.append((cur, allf))
     gn, gp = [], []
-    for grp in groups:
-        n, pk = cost(grp)
+    for es, fs in groups:
+        n, pk = cost(es, fs)
         gn.append(n)
         gp.append(pk)
     return ClassPlan(la, lb, lc, ld, ecomps, fcomps, groups, gn, gp)
@@ -318,11 +343,16 @@ def plan_class(la, lb, lc, ld, budget=96):
 AX = 'xyz'
 
 
+# arithmetic type of the recursion, HRR and transforms of the kernel being
+# emitted ('double', or 'float' for the reduced-precision variants)
+_R = 'double'
+
+
 def fmt(v):
     r = repr(float(v))
     if 'e' not in r and '.' not in r and 'inf' not in r and 'nan' not in r:
         r += '.0'
-    return r
+    return r + ('f' if _R == 'float' else '')
 
 
 class Emitter:
@@ -346,8 +376,8 @@ class Emitter:
 
 
 def vrr_group_order(plan, graph, grp_idx):
-    es = plan.groups[grp_idx]
-    targets = [graph.get(plan.ecomps[i], f, 0) for i in es for f in plan.fcomps]
+    es, fs = plan.groups[grp_idx]
+    targets = [graph.get(plan.ecomps[i], plan.fcomps[j], 0) for i in es for j in fs]
     return targets, graph.order(targets)
 
 
@@ -366,24 +396,29 @@ def emit_vrr_coefs(E, need):
     """Per-primitive multiples k/(2 zeta), k rho/(2 zeta^2), ... so that every
     recursion term is a single fused multiply-add."""
     for k in sorted(need['e']):
-        E(f"const double ez{k} = {fmt(k)} * oo2z, ezr{k} = ez{k} * roz;")
+        E(f"const {_R} ez{k} = {fmt(k)} * oo2z, ezr{k} = ez{k} * roz;")
     for k in sorted(need['f']):
-        E(f"const double fe{k} = {fmt(k)} * oo2e, fer{k} = fe{k} * roe;")
+        E(f"const {_R} fe{k} = {fmt(k)} * oo2e, fer{k} = fe{k} * roe;")
     for k in sorted(need['x']):
-        E(f"const double xz{k} = {fmt(k)} * oo2ze;")
+        E(f"const {_R} xz{k} = {fmt(k)} * oo2ze;")
 
 
-def emit_vrr_group(E, plan, graph, grp_idx, acc_names):
-    """Emit the VRR sub-graph for group `grp_idx`; accumulate targets."""
+def emit_vrr_group(E, plan, graph, grp_idx, acc_names, mid=None):
+    """Emit the VRR sub-graph for group `grp_idx`; accumulate targets.
+    `mid`, if given, is called halfway through the node list (the pipelined
+    loop issues the next primitive's Boys-table load there)."""
     targets, order = vrr_group_order(plan, graph, grp_idx)
     name = {}
     tpos = {t: k for k, t in enumerate(targets)}
+    half = len(order) // 2
     for k, key in enumerate(order):
+        if mid is not None and k == half:
+            mid()
         n = graph.nodes[key]
         v = f"v{k}"
         name[key] = v
         if n.kind == 'base':
-            expr = f"F[{key[2]}]"
+            expr = f"Fv[{key[2]}]"
         elif n.kind == 'bra':
             i = AX[n.dir]
             expr = f"PA{i} * {name[n.deps[0][1]]} + WP{i} * {name[n.deps[1][1]]}"
@@ -403,9 +438,11 @@ def emit_vrr_group(E, plan, graph, grp_idx, acc_names):
                 else:
                     expr += f" + xz{c[1]} * {name[kk]}"
                     j += 1
-        E(f"const double {v} = {expr};")
+        E(f"const {_R} {v} = {expr};")
         if key in tpos:
             E(f"{acc_names[tpos[key]]} += {v};")
+    if mid is not None and half >= len(order):
+        mid()
     return len(order)
 
 
@@ -445,7 +482,7 @@ def emit_hrr(E, l1, l2, inp, prefix, dvec):
         name[(a, b)] = inp[a]
     for n, (k, s0, s1, i) in enumerate(ops):
         v = f"{prefix}{n}"
-        E(f"const double {v} = {name[s0]} + {dvec}{AX[i]} * {name[s1]};")
+        E(f"const {_R} {v} = {name[s0]} + {dvec}{AX[i]} * {name[s1]};")
         name[k] = v
     return [name[k] for k in outs]
 
@@ -461,7 +498,7 @@ def emit_sph_2(E, l1, l2, pure1, pure2, vals, prefix):
     def newv(expr):
         v = f"{prefix}{cnt[0]}"
         cnt[0] += 1
-        E(f"const double {v} = {expr};")
+        E(f"const {_R} {v} = {expr};")
         return v
 
     if pure1 and l1 > 0:
@@ -498,7 +535,98 @@ def pure_variants(l1, l2):
 # ---------------------------------------------------------------------------
 
 
-def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
+def _emit_stage2(E, sfx, ib, ik, count):
+    """Screening, pfac, T and the Boys-table fetch of one primitive quartet
+    (pair data in Pg{sfx}.. Qg{sfx}..).  Screened primitives get pfac = 0:
+    the recursion then produces exact zeros, which keeps the loop free of
+    divergent branches."""
+    E(f"const double gp{sfx} = Pg{sfx}.w, gq{sfx} = Qg{sfx}.w;")
+    E(f"const double rs{sfx} = rsqrt(gp{sfx} + gq{sfx});")
+    E(f"const double oos{sfx} = rs{sfx} * rs{sfx};")
+    E(f"const double pf{sfx} = PAk{sfx}.w * QCk{sfx}.w * rs{sfx} * A.scale;")
+    E(f"bool ok{sfx} = xb{sfx}.x + xk{sfx}.x > lnprec;")
+    E.block("if (A.method == L2G_SCREEN_ORIGINAL)")
+    E(f"ok{sfx} = ok{sfx} && !(fabs(pf{sfx}) < qprec);")
+    E.end()
+    E.block("else if (A.method == L2G_SCREEN_CONSERVATIVE)")
+    E(f"const double nsb = A.pp_nonsph[pb0 + {ib}], nsk = A.pp_nonsph[pk0 + {ik}];")
+    E(f"ok{sfx} = ok{sfx} && !(fabs(pf{sfx}) * fmax(1.0, nsb * nsk) * npbk < qprec);")
+    E.end()
+    if count is not None:
+        E(f"if ({count}ok{sfx}) ++nsurv;")
+    E(f"pfac{sfx} = ok{sfx} ? pf{sfx} : 0.0;")
+    E(f"const double PQx{sfx} = Pg{sfx}.x - Qg{sfx}.x, PQy{sfx} = Pg{sfx}.y - Qg{sfx}.y, "
+      f"PQz{sfx} = Pg{sfx}.z - Qg{sfx}.z;")
+    E(f"boys_fetch(gp{sfx} * gq{sfx} * oos{sfx} * (PQx{sfx} * PQx{sfx} + PQy{sfx} * PQy{sfx} + "
+      f"PQz{sfx} * PQz{sfx}), boysL, bq{sfx});")
+
+
+def emit_prim_loop(E, plan, graph, gi, accs):
+    """Software-pipelined loop over the primitive quartets of one quartet:
+    pair data of the next primitive are loaded at the top of an iteration and
+    its Boys-table row halfway through the recursion, so both memory
+    latencies overlap the FP64 work of the current primitive."""
+    E("const int npq = npb * npk;")
+    E.block("if (npq > 0)")
+    E("int ib = 0, ik = 0;")
+    E("double4 Pg = A.pp_P[pb0], PAk = A.pp_PA[pb0];")
+    E("double2 xb = A.pp_aux[pb0];")
+    E("double4 Qg = A.pp_P[pk0], QCk = A.pp_PA[pk0];")
+    E("double2 xk = A.pp_aux[pk0];")
+    E("double pfac;")
+    E("L2gBoysArg bq;")
+    E.block("")
+    _emit_stage2(E, "", "0", "0", "" if gi == 0 else None)
+    E.end()
+    E.block("for (int p = 0; p < npq; ++p)")
+    E("int ikn = ik + 1, ibn = ib;")
+    E("if (ikn == npk) { ikn = 0; ++ibn; }")
+    E("if (ibn == npb) ibn = 0;  // past the end: any valid pair, result unused")
+    E("const double4 Pgn = A.pp_P[pb0 + ibn], PAkn = A.pp_PA[pb0 + ibn];")
+    E("const double2 xbn = A.pp_aux[pb0 + ibn];")
+    E("const double4 Qgn = A.pp_P[pk0 + ikn], QCkn = A.pp_PA[pk0 + ikn];")
+    E("const double2 xkn = A.pp_aux[pk0 + ikn];")
+    E("double pfacn;")
+    E("L2gBoysArg bqn;")
+    E(f"double F[{plan.L + 1}];")
+    E(f"boys_eval<{plan.L}>(bq, pfac, F);")
+    if _R == 'double':
+        E("const double* Fv = F;")
+    else:
+        E(f"{_R} Fv[{plan.L + 1}];")
+        for m in range(plan.L + 1):
+            E(f"Fv[{m}] = ({_R})F[{m}];")
+    if plan.L > 0:
+        E("const double gp = Pg.w, gq = Qg.w;")
+        E("const double rs = rsqrt(gp + gq);")
+        E("const double oos = rs * rs;")
+        E("const double PQx = Pg.x - Qg.x, PQy = Pg.y - Qg.y, PQz = Pg.z - Qg.z;")
+        R = _R
+        E(f"const {R} PAx = ({R})PAk.x, PAy = ({R})PAk.y, PAz = ({R})PAk.z;")
+        E(f"const {R} QCx = ({R})QCk.x, QCy = ({R})QCk.y, QCz = ({R})QCk.z;")
+        E("const double goq = gq * oos, gop = gp * oos;")
+        E(f"const {R} WPx = ({R})(-goq * PQx), WPy = ({R})(-goq * PQy), WPz = ({R})(-goq * PQz);")
+        E(f"const {R} WQx = ({R})(gop * PQx), WQy = ({R})(gop * PQy), WQz = ({R})(gop * PQz);")
+        E(f"const {R} oo2z = ({R})xb.y, oo2e = ({R})xk.y, oo2ze = ({R})(0.5 * oos);")
+        E(f"const {R} roz = ({R})goq, roe = ({R})gop;")
+        E("(void)PAx; (void)PAy; (void)PAz; (void)QCx; (void)QCy; (void)QCz;")
+        E("(void)WPx; (void)WPy; (void)WPz; (void)WQx; (void)WQy; (void)WQz;")
+        E("(void)oo2z; (void)oo2e; (void)oo2ze; (void)roz; (void)roe;")
+        emit_vrr_coefs(E, vrr_coef_names(plan, graph, gi))
+
+    def mid():
+        E.block("")
+        _emit_stage2(E, "n", "ibn", "ikn", "p + 1 < npq && " if gi == 0 else None)
+        E.end()
+
+    emit_vrr_group(E, plan, graph, gi, accs, mid=mid)
+    E("Pg = Pgn; PAk = PAkn; xb = xbn; Qg = Qgn; QCk = QCkn; xk = xkn;")
+    E("pfac = pfacn; bq = bqn; ib = ibn; ik = ikn;")
+    E.end()  # p loop
+    E.end()  # npq > 0
+
+
+def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     la, lb, lc, ld = plan.la, plan.lb, plan.lc, plan.ld
     graph = VrrGraph()
     ne, nf = len(plan.ecomps), len(plan.fcomps)
@@ -506,9 +634,11 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     E = Emitter()
     E(f"// class ({la}{lb}|{lc}{ld}): L={plan.L}, {ne}x{nf} contracted [e0|f0] targets, "
       f"{G} target group(s)")
-    E(f"// group sizes (bra components): {[len(g) for g in plan.groups]}")
+    E(f"// group sizes (bra components x ket targets): {[(len(g), len(f)) for g, f in plan.groups]}")
     E(f"// group graph nodes: {plan.group_nodes}; estimated peak live doubles: {plan.group_peak}")
-    kname = f"l2g_{mode}_{plan.name}"
+    global _R
+    _R = real
+    kname = f"l2g_{mode}{'32' if real == 'float' else ''}_{plan.name}"
     attr = f" __attribute__((amdgpu_waves_per_eu({waves}, {waves})))" if waves else ""
     if mode == 'fock':
         E("// Fock variant: digests each integral of the unique quartet into the")
@@ -551,62 +681,48 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     # storage of ket-transformed rows: registers when the whole target fits
     # one group and the row block is small, else a lane-private HBM column
     ncol_max = ncart(lc) * ncart(ld)
-    in_regs = (G == 1) and ne * ncol_max <= 48
+    in_regs = (G == 1) and ne * ncol_max <= 48 and not plan.split_rows
     if in_regs:
-        E(f"double Y[{ne}][{ncol_max}];")
+        E(f"{_R} Y[{ne}][{ncol_max}];")
+    if _R != 'double':
+        E(f"const {_R} fABx = ({_R})ABx, fABy = ({_R})ABy, fABz = ({_R})ABz;")
+        E(f"const {_R} fCDx = ({_R})CDx, fCDy = ({_R})CDy, fCDz = ({_R})CDz;")
     E("int nsurv = 0;  // primitive quartets that pass screening")
-    for gi, grp in enumerate(plan.groups):
-        accs = [f"acc{gi}_{t}" for t in range(len(grp) * nf)]
-        E(f"// ---- target group {gi}: bra components {[plan.ecomps[i] for i in grp]}")
+    if mode == 'schwarz':
+        E("double hmax = 0.0;  // max |[e0|e0]| over the pre-HRR diagonal")
+    xoff = ne * ncol_max  # raw-row region of the scratch column (split rows)
+    for gi, (grp, gfs) in enumerate(plan.groups):
+        ngf = len(gfs)
+        accs = [f"acc{gi}_{t}" for t in range(len(grp) * ngf)]
+        E(f"// ---- target group {gi}: bra components {[plan.ecomps[i] for i in grp]}, "
+          f"{ngf} of {nf} ket targets")
         E.block("")
         for a in accs:
-            E(f"double {a} = 0.0;")
-        E.block("for (int ib = 0; ib < npb; ++ib)")
-        E("const double4 Pg = A.pp_P[pb0 + ib];")
-        E("const double4 PAk = A.pp_PA[pb0 + ib];")
-        E("const double2 xb = A.pp_aux[pb0 + ib];")
-        E.block("for (int ik = 0; ik < npk; ++ik)")
-        E("const double2 xk = A.pp_aux[pk0 + ik];")
-        E("if (!(xb.x + xk.x > lnprec)) continue;")
-        E("const double4 Qg = A.pp_P[pk0 + ik];")
-        E("const double4 QCk = A.pp_PA[pk0 + ik];")
-        E("const double gp = Pg.w, gq = Qg.w;")
-        E("const double rs = rsqrt(gp + gq);")
-        E("const double oos = rs * rs;")
-        E("const double pfac = PAk.w * QCk.w * rs * A.scale;")
-        E.block("if (A.method == L2G_SCREEN_ORIGINAL)")
-        E("if (fabs(pfac) < qprec) continue;")
-        E.end()
-        E.block("else if (A.method == L2G_SCREEN_CONSERVATIVE)")
-        E("const double nsk = A.pp_nonsph[pk0 + ik], nsb = A.pp_nonsph[pb0 + ib];")
-        E("if (fabs(pfac) * fmax(1.0, nsb * nsk) * npbk < qprec) continue;")
-        E.end()
-        if gi == 0:
-            E("++nsurv;")
-        E("const double PQx = Pg.x - Qg.x, PQy = Pg.y - Qg.y, PQz = Pg.z - Qg.z;")
-        E("const double T = gp * gq * oos * (PQx * PQx + PQy * PQy + PQz * PQz);")
-        E(f"double F[{plan.L + 1}];")
-        E(f"boys_fm<{plan.L}>(T, boysL, pfac, F);")
-        if plan.L > 0:
-            E("const double PAx = PAk.x, PAy = PAk.y, PAz = PAk.z;")
-            E("const double QCx = QCk.x, QCy = QCk.y, QCz = QCk.z;")
-            E("const double goq = gq * oos, gop = gp * oos;")
-            E("const double WPx = -goq * PQx, WPy = -goq * PQy, WPz = -goq * PQz;")
-            E("const double WQx = gop * PQx, WQy = gop * PQy, WQz = gop * PQz;")
-            E("const double oo2z = xb.y, oo2e = xk.y, oo2ze = 0.5 * oos;")
-            E("const double roz = goq, roe = gop;")
-            E("(void)PAx; (void)PAy; (void)PAz; (void)QCx; (void)QCy; (void)QCz;")
-            E("(void)WPx; (void)WPy; (void)WPz; (void)WQx; (void)WQy; (void)WQz;")
-            E("(void)oo2z; (void)oo2e; (void)oo2ze; (void)roz; (void)roe;")
-            emit_vrr_coefs(E, vrr_coef_names(plan, graph, gi))
-        emit_vrr_group(E, plan, graph, gi, accs)
-        E.end()  # ik
-        E.end()  # ib
-        # ket HRR + ket spherical transform for each row of the group
+            E(f"{_R} {a} = 0;")
+        emit_prim_loop(E, plan, graph, gi, accs)
+        if mode == 'schwarz':
+            for r, ie in enumerate(grp):
+                for t, jf in enumerate(gfs):
+                    if plan.ecomps[ie] == plan.fcomps[jf]:
+                        E(f"hmax = fmax(hmax, fabs((double){accs[r * ngf + t]}));")
+        # ket HRR + ket spherical transform for each row of the group; a row
+        # cut across groups is parked raw and transformed after its last piece
+        full = plan.full_rows(gi)
+        if not full:
+            for r, ie in enumerate(grp):
+                for t, jf in enumerate(gfs):
+                    E(f"yb[(size_t)({xoff + ie * nf + jf}) * A.ystride] = {accs[r * ngf + t]};")
         for r, ie in enumerate(grp):
+            if not full and plan.row_last_group(ie) != gi:
+                continue
             E.block("")
-            inp = {f: accs[r * nf + jf] for jf, f in enumerate(plan.fcomps)}
-            cvals = emit_hrr(E, lc, ld, inp, "h", "CD")
+            if full:
+                inp = {plan.fcomps[jf]: accs[r * ngf + t] for t, jf in enumerate(gfs)}
+            else:
+                for jf in range(nf):
+                    E(f"const {_R} xr{jf} = ({_R})yb[(size_t)({xoff + ie * nf + jf}) * A.ystride];")
+                inp = {plan.fcomps[jf]: f"xr{jf}" for jf in range(nf)}
+            cvals = emit_hrr(E, lc, ld, inp, "h", "CD" if _R == 'double' else "fCD")
             E.block("switch ((meta >> 5) & 3)")
             for (pc, pd) in pure_variants(lc, ld):
                 E.block(f"case {pc | (pd << 1)}:")
@@ -645,7 +761,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
             E("const long long bk3 = bf.z + st.pos3(ic), bk4 = bf.w + st.pos4(id);")
             E("const double d34 = Dm[bk3 * N + bk4];")
         inp = {e: ysrc(ie) for ie, e in enumerate(plan.ecomps)}
-        bvals = emit_hrr(E, la, lb, inp, "g", "AB")
+        bvals = emit_hrr(E, la, lb, inp, "g", "AB" if _R == 'double' else "fAB")
         E.block("switch ((meta >> 3) & 3)")
         for (pa, pb) in pure_variants(la, lb):
             E.block(f"case {pa | (pb << 1)}:")
@@ -716,10 +832,11 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
             E.end()
     else:
         E.block("for (int col = 0; col < ncol; ++col)")
-        bra_column("col", lambda ie: f"yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
+        bra_column("col", lambda ie: f"({_R})yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
         E.end()
     if mode == 'schwarz':
         E("A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(ssq)) : sqrt(smax);")
+        E("if (A.out2) A.out2[A.out_off[qi]] = sqrt(hmax);")
     if mode == 'fock':
         for i1 in range(ncart(la)):
             for i2 in range(ncart(lb)):
@@ -728,6 +845,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0):
     E.end()  # quartet loop
     E.ind -= 1
     E("}")
+    _R = 'double'
     return kname, E.text(), in_regs
 
 
@@ -746,7 +864,8 @@ def class_waves(plan, budget, waves):
     return waves
 
 
-def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=None):
+def generate(outdir, lmax=2, budget=150, nfiles=8, waves=0, only=None, budgets=None,
+             modes=('eri', 'fock', 'schwarz', 'fp32')):
     """Write the kernels and the dispatch tables.  `budgets` maps a class name
     ('2222') to its own (budget, waves)."""
     os.makedirs(outdir, exist_ok=True)
@@ -759,51 +878,65 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
         return budgets.get("".join(map(str, c)), (budget, waves))
 
     plans = [plan_class(*c, budget=cfg(c)[0]) for c in classes]
-    jobs = []  # (plan, mode)
+    # reduced-precision variants hold twice as many values per register
+    plans32 = [plan_class(*c, budget=2 * cfg(c)[0]) for c in classes] if 'fp32' in modes else []
+    jobs = []  # (plan, mode, real)
     for p in plans:
-        jobs.append((p, 'eri'))
-        jobs.append((p, 'fock'))
-        if (p.la, p.lb) == (p.lc, p.ld):
-            jobs.append((p, 'schwarz'))
-    weights = sorted(((sum(p.group_nodes) * (0.6 if m == 'schwarz' else 1.0) + 200, i) for i, (p, m) in enumerate(jobs)),
-                     reverse=True)
+        if 'eri' in modes:
+            jobs.append((p, 'eri', 'double'))
+        if 'fock' in modes:
+            jobs.append((p, 'fock', 'double'))
+        if 'schwarz' in modes and (p.la, p.lb) == (p.lc, p.ld):
+            jobs.append((p, 'schwarz', 'double'))
+    for p in plans32:
+        if 'eri' in modes:
+            jobs.append((p, 'eri', 'float'))
+        if 'fock' in modes:
+            jobs.append((p, 'fock', 'float'))
+    weights = sorted(((sum(p.group_nodes) * (0.6 if m == 'schwarz' else 1.0) + 200, i)
+                      for i, (p, m, r) in enumerate(jobs)), reverse=True)
     files = [[] for _ in range(nfiles)]
     load = [0] * nfiles
     for w, i in weights:
         j = load.index(min(load))
         files[j].append(i)
         load[j] += w
-    table, stable, ftable = [], [], []
+    tables = {('eri', 'double'): [], ('schwarz', 'double'): [], ('fock', 'double'): [],
+              ('eri', 'float'): [], ('fock', 'float'): []}
     written = []
     for fi, idxs in enumerate(files):
         fn = os.path.join(outdir, f"eri_gen_{fi}.hip")
         parts = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
-                 '#include "../eri_device.h"', '']
+                 '#include "eri_device.h"', '']
         for i in sorted(idxs):
-            p, mode = jobs[i]
+            p, mode, real = jobs[i]
             wv = cfg((p.la, p.lb, p.lc, p.ld))[1]
-            kname, code, in_regs = emit_class(p, mode=mode, waves=wv)
+            kname, code, in_regs = emit_class(p, mode=mode, waves=wv, real=real)
             parts.append(code)
             parts.append(f"void {kname}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s) {{")
             parts.append(f"  hipLaunchKernelGGL({kname}, dim3(grid), dim3(block), 0, s, A);")
             parts.append("}")
             parts.append("")
-            {'eri': table, 'schwarz': stable, 'fock': ftable}[mode].append((p, kname, in_regs))
+            tables[(mode, real)].append((p, kname, in_regs))
         new = '\n'.join(parts)
         if not (os.path.exists(fn) and open(fn).read() == new):
             with open(fn, 'w') as f:
                 f.write(new)
         written.append(fn)
+    table, stable, ftable = tables[('eri', 'double')], tables[('schwarz', 'double')], tables[('fock', 'double')]
+    table32, ftable32 = tables[('eri', 'float')], tables[('fock', 'float')]
     lines = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
-             '#include "../eri_device.h"', '']
-    for p, k, _ in table + stable + ftable:
+             '#include "eri_device.h"', '']
+    for p, k, _ in table + stable + ftable + table32 + ftable32:
         lines.append(f"void {k}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s);")
     lines.append('')
     lines.append(f"extern const int l2g_gen_lmax = {lmax};")
-    for name, tab in (("l2g_gen_classes", table), ("l2g_gen_schwarz", stable), ("l2g_gen_fock", ftable)):
+    for name, tab in (("l2g_gen_classes", table), ("l2g_gen_schwarz", stable), ("l2g_gen_fock", ftable),
+                      ("l2g_gen_classes32", table32), ("l2g_gen_fock32", ftable32)):
         lines.append(f"extern const L2gClassInfo {name}[] = {{")
         for p, k, in_regs in tab:
-            ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld)
+            ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld) + \
+                (len(p.ecomps) * len(p.fcomps) if p.split_rows else 0)
             lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
                          f"{sum(p.group_nodes)}, &{k}_launch}},")
         if not tab:
@@ -812,6 +945,8 @@ def generate(outdir, lmax=2, budget=96, nfiles=8, waves=0, only=None, budgets=No
     lines.append(f"extern const int l2g_gen_nclasses = {len(table)};")
     lines.append(f"extern const int l2g_gen_nschwarz = {len(stable)};")
     lines.append(f"extern const int l2g_gen_nfock = {len(ftable)};")
+    lines.append(f"extern const int l2g_gen_nclasses32 = {len(table32)};")
+    lines.append(f"extern const int l2g_gen_nfock32 = {len(ftable32)};")
     fn = os.path.join(outdir, "eri_table.cpp")
     new = '\n'.join(lines) + '\n'
     if not (os.path.exists(fn) and open(fn).read() == new):
@@ -826,11 +961,12 @@ if __name__ == '__main__':
     ap = argparse.ArgumentParser()
     ap.add_argument("outdir")
     ap.add_argument("--lmax", type=int, default=2)
-    ap.add_argument("--budget", type=int, default=96)
+    ap.add_argument("--budget", type=int, default=150)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--only", default="", help="comma-separated class names, e.g. 2222")
     ap.add_argument("--class-cfg", default="", help="per class budget:waves, e.g. 2222=64:2,2122=80:2")
     ap.add_argument("--nfiles", type=int, default=8)
+    ap.add_argument("--modes", default="eri,fock,schwarz,fp32")
     a = ap.parse_args()
     per = {}
     for item in filter(None, a.class_cfg.split(",")):
@@ -838,6 +974,6 @@ if __name__ == '__main__':
         b, w = v.split(":")
         per[k] = (int(b), int(w))
     files, plans = generate(a.outdir, a.lmax, a.budget, a.nfiles, a.waves,
-                            set(filter(None, a.only.split(","))) or None, per)
+                            set(filter(None, a.only.split(","))) or None, per, tuple(a.modes.split(",")))
     for p in plans:
         print(p.name, 'groups', len(p.groups), 'nodes', sum(p.group_nodes), 'peak', max(p.group_peak))

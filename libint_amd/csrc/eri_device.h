@@ -53,6 +53,8 @@ struct L2gEriArgs {
   double* __restrict__ jg;               // unsymmetrised Coulomb accumulator
   double* __restrict__ kg;               // unsymmetrised exchange accumulator
   long long nbf;
+  // Schwarz kernels: optional sqrt(max |[e0|e0]|) of the pre-HRR diagonal
+  double* __restrict__ out2;
 };
 
 typedef void (*L2gLaunchFn)(const L2gEriArgs&, int grid, int block, hipStream_t s);
@@ -100,6 +102,60 @@ __device__ __forceinline__ void boys_fm(double T, const double* __restrict__ tab
     if (L > 0) {
       const double e = exp(-T);
       const double t2 = T + T;
+#pragma unroll
+      for (int m = L; m > 0; --m) fm[m - 1] = (t2 * fm[m] + e) * (1.0 / (double)(2 * m - 1));
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) F[m] = fm[m] * pfac;
+  }
+}
+
+// Boys function split in two steps so that the table row of the next
+// primitive quartet can be fetched while the current one is being processed:
+// boys_fetch locates the interval and loads its 8 coefficients (for T above
+// T_crit the load is of a valid row but unused), boys_eval evaluates.
+struct L2gBoysArg {
+  double T, xd;
+  double4 lo, hi;
+};
+
+__device__ __forceinline__ void boys_fetch(double T, const double* __restrict__ tabL, L2gBoysArg& b) {
+  const double x7 = T * 7.0;
+  const double ivd = fmin(floor(x7), (double)(L2G_BOYS_NINT - 1));
+  const int iv = (int)ivd;
+  b.T = T;
+  b.xd = x7 - ivd - 0.5;
+  const double4* c4 = reinterpret_cast<const double4*>(tabL + (size_t)iv * 8);
+  b.lo = c4[0];
+  b.hi = c4[1];
+}
+
+template <int L>
+__device__ __forceinline__ void boys_eval(const L2gBoysArg& b, double pfac, double* F) {
+  if (b.T > L2G_BOYS_TCRIT) {
+    const double oox = 1.0 / b.T;
+    double f = 0.88622692545275801365 * sqrt(oox);
+    F[0] = f * pfac;
+#pragma unroll
+    for (int m = 1; m <= L; ++m) {
+      f = f * ((double)m - 0.5) * oox;
+      F[m] = f * pfac;
+    }
+  } else {
+    const double xd = b.xd;
+    double f = b.hi.w;
+    f = fma(f, xd, b.hi.z);
+    f = fma(f, xd, b.hi.y);
+    f = fma(f, xd, b.hi.x);
+    f = fma(f, xd, b.lo.w);
+    f = fma(f, xd, b.lo.z);
+    f = fma(f, xd, b.lo.y);
+    f = fma(f, xd, b.lo.x);
+    double fm[L + 1];
+    fm[L] = f;
+    if (L > 0) {
+      const double e = exp(-b.T);
+      const double t2 = b.T + b.T;
 #pragma unroll
       for (int m = L; m > 0; --m) fm[m - 1] = (t2 * fm[m] + e) * (1.0 / (double)(2 * m - 1));
     }

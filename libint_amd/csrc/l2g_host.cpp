@@ -27,6 +27,10 @@ extern const L2gClassInfo l2g_gen_schwarz[];
 extern const int l2g_gen_nschwarz;
 extern const L2gClassInfo l2g_gen_fock[];
 extern const int l2g_gen_nfock;
+extern const L2gClassInfo l2g_gen_classes32[];
+extern const int l2g_gen_nclasses32;
+extern const L2gClassInfo l2g_gen_fock32[];
+extern const int l2g_gen_nfock32;
 extern const int l2g_gen_lmax;
 
 namespace {
@@ -192,7 +196,12 @@ struct l2g_context_s {
   int ncu = 256;
   int class_of[8][8][8][8];  // index into l2g_gen_classes
   int fock_of[8][8][8][8];   // index into l2g_gen_fock (its own order)
+  int class32_of[8][8][8][8];  // index into l2g_gen_classes32
+  int fock32_of[8][8][8][8];   // index into l2g_gen_fock32
   int schwarz_of[8][8];
+  // bases, plans and Fock builders created on this context and not yet
+  // destroyed; l2g_finalize refuses while any is alive
+  std::atomic<int> live{0};
 };
 
 struct l2g_basis_s {
@@ -200,10 +209,47 @@ struct l2g_basis_s {
   int nshell;
   std::vector<int> l, pure, nprim, off;
   std::vector<double> alpha, coeff, lnc, origin;
+  std::unordered_map<uint64_t, bool> orient;  // second_is_centre() memo
+
+  ~l2g_basis_s() {
+    if (ctx) --ctx->live;
+  }
+
+  // Which shell of a pair with equal l is the HRR build centre.  The
+  // reference keeps the caller's order for equal l (engine.impl.h:1181-1208);
+  // the integral does not depend on the choice, its rounding does: HRR
+  // (a,b) = sum_k C(b,k) AB^k (a+k,0) cancels badly when the build centre is
+  // the diffuse shell of a distant pair, because the pair's charge sits near
+  // the tight shell.  True when shell j is tighter, i.e. when the
+  // K-weighted mean of |P - B_j| = alpha_i/gamma |AB| is below that of
+  // |P - A_i| = alpha_j/gamma |AB|.
+  bool second_is_centre(int i, int j) {
+    if (i == j || l[i] != l[j]) return false;
+    const uint64_t key = (uint64_t)(uint32_t)i << 32 | (uint32_t)j;
+    auto it = orient.find(key);
+    if (it != orient.end()) return it->second;
+    double AB2 = 0;
+    for (int k = 0; k < 3; ++k) AB2 += (origin[3 * i + k] - origin[3 * j + k]) * (origin[3 * i + k] - origin[3 * j + k]);
+    double s = 0;
+    if (AB2 > 0)
+      for (int p1 = 0; p1 < nprim[i]; ++p1)
+        for (int p2 = 0; p2 < nprim[j]; ++p2) {
+          const double a1 = alpha[off[i] + p1], a2 = alpha[off[j] + p2], g = a1 + a2;
+          const double w = std::fabs(coeff[off[i] + p1] * coeff[off[j] + p2]) * std::exp(-a1 * a2 / g * AB2) / g;
+          s += w * (a2 - a1) / g;
+        }
+    const bool r = s > 0;
+    orient.emplace(key, r);
+    orient.emplace((uint64_t)(uint32_t)j << 32 | (uint32_t)i, s < 0);
+    return r;
+  }
+  // canonical bra orientation of (i, j): l_i >= l_j, ties by second_is_centre
+  bool swap_pair(int i, int j) { return l[i] < l[j] || (l[i] == l[j] && second_is_centre(i, j)); }
 };
 
 struct ClassWork {
-  int cls;  // index into l2g_gen_classes
+  int cls;  // index into l2g_gen_classes (l2g_gen_classes32 when fp32)
+  bool fp32 = false;
   int64_t first, count;
   int64_t nprimq;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -218,6 +264,7 @@ struct l2g_eri_plan_s {
   std::vector<ClassWork> classes;
   double precision, ln_precision, scale;
   int method, uniform;
+  double fp32_threshold = 0;  // quartets with K12 K34 below it run in FP32
   // device arrays
   DevBuf bra, ket, off, meta, qidx;
   DevBuf pair_first, pair_np, pair_AB, ppP, ppPA, ppAux, ppNs;
@@ -231,6 +278,22 @@ struct l2g_eri_plan_s {
     }
   }
 };
+
+// FP32 selection of a mixed batch: a quartet runs in FP32 when its Schwarz
+// bound K_ab K_cd is below the caller's threshold and its pre-HRR bound
+// S_ab S_cd (schwarz_impl) is below this factor times the threshold, which
+// keeps eps_fp32 S_ab S_cd, the scale of the FP32 rounding that reaches an
+// integral, near eps_fp32 times the threshold.
+#ifndef L2G_FP32_HRR_FACTOR
+#define L2G_FP32_HRR_FACTOR 16.0
+#endif
+
+namespace {
+
+void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K,
+                  double* S);
+
+}  // namespace
 
 namespace {
 
@@ -425,14 +488,17 @@ int l2g_init(int device, l2g_context* out) {
       const auto& c = l2g_gen_classes[i];
       ctx->class_of[c.la][c.lb][c.lc][c.ld] = i;
     }
-    for (auto& a : ctx->fock_of)
-      for (auto& b : a)
-        for (auto& c : b)
-          for (auto& d : c) d = -1;
-    for (int i = 0; i < l2g_gen_nfock; ++i) {
-      const auto& c = l2g_gen_fock[i];
-      if (c.launch) ctx->fock_of[c.la][c.lb][c.lc][c.ld] = i;
-    }
+    auto fill = [](int (&m)[8][8][8][8], const L2gClassInfo* tab, int n) {
+      for (auto& a : m)
+        for (auto& b : a)
+          for (auto& c : b)
+            for (auto& d : c) d = -1;
+      for (int i = 0; i < n; ++i)
+        if (tab[i].launch) m[tab[i].la][tab[i].lb][tab[i].lc][tab[i].ld] = i;
+    };
+    fill(ctx->fock_of, l2g_gen_fock, l2g_gen_nfock);
+    fill(ctx->class32_of, l2g_gen_classes32, l2g_gen_nclasses32);
+    fill(ctx->fock32_of, l2g_gen_fock32, l2g_gen_nfock32);
     for (auto& a : ctx->schwarz_of)
       for (auto& b : a) b = -1;
     for (int i = 0; i < l2g_gen_nschwarz; ++i) {
@@ -446,6 +512,9 @@ int l2g_init(int device, l2g_context* out) {
 int l2g_finalize(l2g_context ctx) {
   return guarded([&] {
     if (!ctx) return;
+    if (ctx->live > 0)
+      throw L2gError(L2G_E_STATE, std::to_string(ctx->live.load()) +
+                                      " basis/plan/Fock object(s) of this context are still alive; destroy them first");
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -470,6 +539,7 @@ int l2g_basis_create(l2g_context ctx, int nshell, const int* l, const int* pure,
       throw std::invalid_argument("l2g_basis_create: bad arguments");
     auto b = std::make_unique<l2g_basis_s>();
     b->ctx = ctx;
+    ++ctx->live;
     b->nshell = nshell;
     b->l.assign(l, l + nshell);
     b->pure.assign(pure, pure + nshell);
@@ -499,8 +569,8 @@ int l2g_basis_destroy(l2g_basis b) {
   return L2G_OK;
 }
 
-int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double scale,
-                        int uniform_cart, l2g_eri_plan* out) {
+static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening,
+                            double scale, int uniform_cart, double fp32_threshold, l2g_eri_plan* out) {
   return guarded([&] {
     if (!B) throw L2gError(L2G_E_STATE, "null basis");
     if (!out || nq < 0 || (nq > 0 && !quartets)) throw std::invalid_argument("l2g_eri_plan_create: bad arguments");
@@ -522,14 +592,44 @@ int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double
       plan->precision = precision;
       plan->ln_precision = std::log(precision);
     }
+    plan->fp32_threshold = fp32_threshold;
     PairBuilder pb(*B, plan->ln_precision, screening);
+    // Schwarz and FP32-selection factors of the canonical pairs, when the
+    // batch is mixed
+    std::unordered_map<uint64_t, std::pair<double, double>> Kp;
+    auto pkey = [&](int a, int b) {
+      if (B->swap_pair(a, b)) std::swap(a, b);
+      return (uint64_t)(uint32_t)a << 32 | (uint32_t)b;
+    };
+    if (fp32_threshold > 0) {
+      std::vector<int32_t> pr;
+      for (int64_t q = 0; q < nq; ++q)
+        for (int h = 0; h < 2; ++h) {
+          const int a = quartets[4 * q + 2 * h], b = quartets[4 * q + 2 * h + 1];
+          if (a < 0 || b < 0 || a >= B->nshell || b >= B->nshell) throw std::invalid_argument("shell index out of range");
+          if (Kp.emplace(pkey(a, b), std::make_pair(0.0, 0.0)).second) {
+            pr.push_back(a);
+            pr.push_back(b);
+          }
+        }
+      std::vector<double> K(pr.size() / 2), S(pr.size() / 2);
+      schwarz_impl(B, (int64_t)K.size(), pr.data(), 0.0, 0, K.data(), S.data());
+      for (size_t i = 0; i < K.size(); ++i) Kp[pkey(pr[2 * i], pr[2 * i + 1])] = {K[i], S[i]};
+    }
+    // FP32 when both the Schwarz bound and the pre-HRR bound are small
+    auto use_fp32 = [&](const int32_t* s) {
+      const auto& x = Kp[pkey(s[0], s[1])];
+      const auto& y = Kp[pkey(s[2], s[3])];
+      return x.first * y.first < fp32_threshold && x.second * y.second < L2G_FP32_HRR_FACTOR * fp32_threshold;
+    };
+    const int nc64 = l2g_gen_nclasses;
     // canonicalise every quartet and bucket by class
     struct Item {
       int bp, kp, meta;
       int64_t off, q;
       int64_t w;
     };
-    std::vector<std::vector<Item>> buckets(l2g_gen_nclasses);
+    std::vector<std::vector<Item>> buckets(l2g_gen_nclasses + l2g_gen_nclasses32);
     plan->offsets.resize(nq);
     int64_t total = 0;
     for (int64_t q = 0; q < nq; ++q) {
@@ -537,14 +637,18 @@ int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double
       for (int k = 0; k < 4; ++k)
         if (s[k] < 0 || s[k] >= B->nshell) throw std::invalid_argument("shell index out of range");
       const int l1 = B->l[s[0]], l2 = B->l[s[1]], l3 = B->l[s[2]], l4 = B->l[s[3]];
-      // engine.impl.h:1181-1208
-      const bool stb = l1 < l2, stk = l3 < l4, sbk = (l1 + l2) > (l3 + l4);
+      // engine.impl.h:1181-1208, equal-l pairs oriented by second_is_centre
+      const bool stb = B->swap_pair(s[0], s[1]), stk = B->swap_pair(s[2], s[3]), sbk = (l1 + l2) > (l3 + l4);
       const int b1 = sbk ? (stk ? s[3] : s[2]) : (stb ? s[1] : s[0]);
       const int b2 = sbk ? (stk ? s[2] : s[3]) : (stb ? s[0] : s[1]);
       const int k1 = sbk ? (stb ? s[1] : s[0]) : (stk ? s[3] : s[2]);
       const int k2 = sbk ? (stb ? s[0] : s[1]) : (stk ? s[2] : s[3]);
-      const int cls = ctx->class_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+      int cls = ctx->class_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
       if (cls < 0) throw L2gError(L2G_E_LMAX, "no kernel for this angular momentum class");
+      if (fp32_threshold > 0 && use_fp32(s)) {
+        const int c32 = ctx->class32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+        if (c32 >= 0) cls = nc64 + c32;
+      }
       auto purebit = [&](int sh) { return (B->pure[sh] && B->l[sh] > 0) ? 1 : 0; };
       const int meta = (stb ? 1 : 0) | (stk ? 2 : 0) | (sbk ? 4 : 0) | (purebit(b1) << 3) | (purebit(b2) << 4) |
                        (purebit(k1) << 5) | (purebit(k2) << 6);
@@ -570,9 +674,11 @@ int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double
     off.reserve(nq);
     qidx.reserve(nq);
     int64_t ymax = 0;
-    for (int c = 0; c < l2g_gen_nclasses; ++c) {
-      auto& v = buckets[c];
+    for (int cb = 0; cb < (int)buckets.size(); ++cb) {
+      auto& v = buckets[cb];
       if (v.empty()) continue;
+      const bool is32 = cb >= nc64;
+      const int c = is32 ? cb - nc64 : cb;
       std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
         if (a.w != b.w) return a.w > b.w;
         if (a.bp != b.bp) return a.bp < b.bp;
@@ -580,6 +686,7 @@ int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double
       });
       ClassWork cw;
       cw.cls = c;
+      cw.fp32 = is32;
       cw.first = (int64_t)bra.size();
       cw.count = (int64_t)v.size();
       cw.nprimq = 0;
@@ -594,7 +701,7 @@ int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double
       HIPCHK(hipEventCreate(&cw.ev0));
       HIPCHK(hipEventCreate(&cw.ev1));
       plan->classes.push_back(cw);
-      ymax = std::max<int64_t>(ymax, l2g_gen_classes[c].ybuf_doubles);
+      ymax = std::max<int64_t>(ymax, (is32 ? l2g_gen_classes32 : l2g_gen_classes)[c].ybuf_doubles);
     }
     hipStream_t st = ctx->stream;
     upload(plan->bra, bra, st);
@@ -618,8 +725,23 @@ int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double
     plan->ystride = (int64_t)plan->grid * plan->block;
     plan->ybuf.alloc(std::max<int64_t>(ymax * plan->ystride, 1) * sizeof(double));
     HIPCHK(hipStreamSynchronize(st));
+    ++ctx->live;
     *out = plan.release();
   });
+}
+
+int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double scale,
+                        int uniform_cart, l2g_eri_plan* out) {
+  return plan_create_impl(B, nq, quartets, precision, screening, scale, uniform_cart, 0.0, out);
+}
+
+int l2g_eri_plan_create_mixed(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening,
+                              double scale, int uniform_cart, double fp32_threshold, l2g_eri_plan* out) {
+  if (!(fp32_threshold >= 0)) {
+    g_err = "fp32_threshold must be >= 0";
+    return L2G_E_INVALID;
+  }
+  return plan_create_impl(B, nq, quartets, precision, screening, scale, uniform_cart, fp32_threshold, out);
 }
 
 int l2g_eri_plan_size(l2g_eri_plan p, int64_t* total) {
@@ -668,7 +790,7 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     p->nz.alloc(std::max<int64_t>(p->nq, 1));
     A.nz = p->nz.as<signed char>();
     for (auto& c : p->classes) {
-      const L2gClassInfo& ci = l2g_gen_classes[c.cls];
+      const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
       A.bra_pair = p->bra.as<int>() + c.first;
       A.ket_pair = p->ket.as<int>() + c.first;
       A.meta = p->meta.as<int>() + c.first;
@@ -724,8 +846,8 @@ int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* n
     *ncls = (int)p->classes.size();
     for (int i = 0; i < (int)p->classes.size() && i < cap; ++i) {
       auto& c = p->classes[i];
-      const auto& ci = l2g_gen_classes[c.cls];
-      if (code) code[i] = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld;
+      const auto& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
+      if (code) code[i] = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld + (c.fp32 ? 10000 : 0);
       if (nquart) nquart[i] = c.count;
       if (nprimq) nprimq[i] = c.nprimq;
       if (ms) {
@@ -739,12 +861,125 @@ int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* n
 
 int l2g_eri_plan_destroy(l2g_eri_plan p) {
   if (p) {
-    (void)hipSetDevice(p->basis->ctx->device);
-    (void)hipStreamSynchronize(p->basis->ctx->stream);
+    l2g_context_s* ctx = p->basis->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete p;
+    --ctx->live;
   }
-  delete p;
   return L2G_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Schwarz factors K of shell pairs and, when S is given, the FP32-selection
+// factor of each pair in its canonical orientation,
+//   S_ab = sqrt(max_e |[e0|e0]|) (1 + max_x |AB_x|)^lb,
+// where [e0|e0] are the contracted pre-HRR integrals (e = la..la+lb on the
+// build centre).  By the Cauchy-Schwarz inequality |[e0|f0]| <= S'_ab S'_cd,
+// and the bra and ket HRR sums have absolute coefficient sums at most
+// (1 + max|AB_x|)^lb and (1 + max|CD_x|)^ld, so S_ab S_cd bounds what HRR
+// adds up: FP32 rounding in the recursion enters a final integral at about
+// eps_fp32 S_ab S_cd.
+void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K,
+                  double* S) {
+  l2g_context_s* ctx = B->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  const double lnp = precision > 0 ? std::log(precision) : std::numeric_limits<double>::lowest();
+  PairBuilder pb(*B, lnp, L2G_SCREEN_ORIGINAL);
+  std::vector<std::vector<int>> pid(l2g_gen_nschwarz), meta(l2g_gen_nschwarz);
+  std::vector<std::vector<long long>> idx(l2g_gen_nschwarz);
+  for (int64_t q = 0; q < npairs; ++q) {
+    int a = pairs[2 * q], b = pairs[2 * q + 1];
+    if (a < 0 || b < 0 || a >= B->nshell || b >= B->nshell) throw std::invalid_argument("shell index out of range");
+    if (B->swap_pair(a, b)) std::swap(a, b);
+    const int c = ctx->schwarz_of[B->l[a]][B->l[b]];
+    if (c < 0) throw L2gError(L2G_E_LMAX, "no Schwarz kernel for this shell pair");
+    const int pa = (B->pure[a] && B->l[a] > 0) ? 1 : 0, pbit = (B->pure[b] && B->l[b] > 0) ? 1 : 0;
+    pid[c].push_back(pb.get(a, b));
+    meta[c].push_back((pa << 3) | (pbit << 4) | (pa << 5) | (pbit << 6));
+    idx[c].push_back(q);
+  }
+  std::vector<int> allp, allm;
+  std::vector<long long> alli;
+  std::vector<std::pair<int, int64_t>> ranges;
+  int64_t ymax = 0, maxn = 0;
+  for (int c = 0; c < l2g_gen_nschwarz; ++c) {
+    if (pid[c].empty()) continue;
+    ranges.push_back({c, (int64_t)allp.size()});
+    allp.insert(allp.end(), pid[c].begin(), pid[c].end());
+    allm.insert(allm.end(), meta[c].begin(), meta[c].end());
+    alli.insert(alli.end(), idx[c].begin(), idx[c].end());
+    ymax = std::max<int64_t>(ymax, l2g_gen_schwarz[c].ybuf_doubles);
+    maxn = std::max<int64_t>(maxn, (int64_t)pid[c].size());
+  }
+  hipStream_t st = ctx->stream;
+  DevBuf dp, dm, di, pf, pn, pab, pP, pPA, pAux, pNs, dK, yb;
+  upload(dp, allp, st);
+  upload(dm, allm, st);
+  upload(di, alli, st);
+  upload(pf, pb.first, st);
+  upload(pn, pb.np, st);
+  upload(pab, pb.AB, st);
+  upload(pP, pb.P, st);
+  upload(pPA, pb.PA, st);
+  upload(pAux, pb.aux, st);
+  upload(pNs, pb.ns, st);
+  dK.alloc(npairs * sizeof(double) * (S ? 2 : 1));
+  const int block = 128;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
+  const int64_t ystride = (int64_t)grid * block;
+  yb.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
+  L2gEriArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.pair_first = pf.as<int>();
+  A.pair_np = pn.as<int>();
+  A.pair_AB = pab.as<double>();
+  A.pp_P = pP.as<double4>();
+  A.pp_PA = pPA.as<double4>();
+  A.pp_aux = pAux.as<double2>();
+  A.pp_nonsph = pNs.as<double>();
+  A.boys = ctx->boys.as<double>();
+  A.precision = precision > 0 ? precision : 0.0;
+  A.ln_precision = lnp;
+  A.scale = 1.0;
+  A.method = L2G_SCREEN_ORIGINAL;
+  A.shg_gaussian = 0;
+  A.schwarz_norm = norm;
+  A.out = dK.as<double>();
+  A.out2 = S ? dK.as<double>() + npairs : nullptr;
+  A.ybuf = yb.as<double>();
+  A.ystride = ystride;
+  for (auto& r : ranges) {
+    const int c = r.first;
+    const int64_t n = (int64_t)pid[c].size();
+    A.bra_pair = dp.as<int>() + r.second;
+    A.ket_pair = A.bra_pair;
+    A.meta = dm.as<int>() + r.second;
+    A.out_off = di.as<long long>() + r.second;
+    A.n = (int)n;
+    l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
+    HIPCHK(hipGetLastError());
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Schwarz kernel failed: ") + hipGetErrorString(e));
+  HIPCHK(hipMemcpy(K, dK.p, npairs * sizeof(double), hipMemcpyDeviceToHost));
+  if (S) {
+    HIPCHK(hipMemcpy(S, dK.as<double>() + npairs, npairs * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < npairs; ++q) {
+      const int a = pairs[2 * q], b = pairs[2 * q + 1];
+      double m = 0;
+      for (int k = 0; k < 3; ++k) m = std::max(m, std::fabs(B->origin[3 * a + k] - B->origin[3 * b + k]));
+      S[q] *= std::pow(1.0 + m, std::min(B->l[a], B->l[b]));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
 
 int l2g_schwarz(l2g_basis B, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K) {
   return guarded([&] {
@@ -752,89 +987,18 @@ int l2g_schwarz(l2g_basis B, int64_t npairs, const int32_t* pairs, double precis
     if (npairs < 0 || (npairs > 0 && (!pairs || !K)) || (norm != 0 && norm != 1))
       throw std::invalid_argument("l2g_schwarz: bad arguments");
     if (npairs == 0) return;
-    l2g_context_s* ctx = B->ctx;
-    HIPCHK(hipSetDevice(ctx->device));
-    const double lnp = precision > 0 ? std::log(precision) : std::numeric_limits<double>::lowest();
-    PairBuilder pb(*B, lnp, L2G_SCREEN_ORIGINAL);
-    std::vector<std::vector<int>> pid(l2g_gen_nschwarz), meta(l2g_gen_nschwarz);
-    std::vector<std::vector<long long>> idx(l2g_gen_nschwarz);
-    for (int64_t q = 0; q < npairs; ++q) {
-      int a = pairs[2 * q], b = pairs[2 * q + 1];
-      if (a < 0 || b < 0 || a >= B->nshell || b >= B->nshell) throw std::invalid_argument("shell index out of range");
-      if (B->l[a] < B->l[b]) std::swap(a, b);
-      const int c = ctx->schwarz_of[B->l[a]][B->l[b]];
-      if (c < 0) throw L2gError(L2G_E_LMAX, "no Schwarz kernel for this shell pair");
-      const int pa = (B->pure[a] && B->l[a] > 0) ? 1 : 0, pbit = (B->pure[b] && B->l[b] > 0) ? 1 : 0;
-      pid[c].push_back(pb.get(a, b));
-      meta[c].push_back((pa << 3) | (pbit << 4) | (pa << 5) | (pbit << 6));
-      idx[c].push_back(q);
-    }
-    std::vector<int> allp, allm;
-    std::vector<long long> alli;
-    std::vector<std::pair<int, int64_t>> ranges;
-    int64_t ymax = 0, maxn = 0;
-    for (int c = 0; c < l2g_gen_nschwarz; ++c) {
-      if (pid[c].empty()) continue;
-      ranges.push_back({c, (int64_t)allp.size()});
-      allp.insert(allp.end(), pid[c].begin(), pid[c].end());
-      allm.insert(allm.end(), meta[c].begin(), meta[c].end());
-      alli.insert(alli.end(), idx[c].begin(), idx[c].end());
-      ymax = std::max<int64_t>(ymax, l2g_gen_schwarz[c].ybuf_doubles);
-      maxn = std::max<int64_t>(maxn, (int64_t)pid[c].size());
-    }
-    hipStream_t st = ctx->stream;
-    DevBuf dp, dm, di, pf, pn, pab, pP, pPA, pAux, pNs, dK, yb;
-    upload(dp, allp, st);
-    upload(dm, allm, st);
-    upload(di, alli, st);
-    upload(pf, pb.first, st);
-    upload(pn, pb.np, st);
-    upload(pab, pb.AB, st);
-    upload(pP, pb.P, st);
-    upload(pPA, pb.PA, st);
-    upload(pAux, pb.aux, st);
-    upload(pNs, pb.ns, st);
-    dK.alloc(npairs * sizeof(double));
-    const int block = 128;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
-    const int64_t ystride = (int64_t)grid * block;
-    yb.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
-    L2gEriArgs A;
-    std::memset(&A, 0, sizeof(A));
-    A.pair_first = pf.as<int>();
-    A.pair_np = pn.as<int>();
-    A.pair_AB = pab.as<double>();
-    A.pp_P = pP.as<double4>();
-    A.pp_PA = pPA.as<double4>();
-    A.pp_aux = pAux.as<double2>();
-    A.pp_nonsph = pNs.as<double>();
-    A.boys = ctx->boys.as<double>();
-    A.precision = precision > 0 ? precision : 0.0;
-    A.ln_precision = lnp;
-    A.scale = 1.0;
-    A.method = L2G_SCREEN_ORIGINAL;
-    A.shg_gaussian = 0;
-    A.schwarz_norm = norm;
-    A.out = dK.as<double>();
-    A.ybuf = yb.as<double>();
-    A.ystride = ystride;
-    for (auto& r : ranges) {
-      const int c = r.first;
-      const int64_t n = (int64_t)pid[c].size();
-      A.bra_pair = dp.as<int>() + r.second;
-      A.ket_pair = A.bra_pair;
-      A.meta = dm.as<int>() + r.second;
-      A.out_off = di.as<long long>() + r.second;
-      A.n = (int)n;
-      l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
-      HIPCHK(hipGetLastError());
-    }
-    hipError_t e = hipStreamSynchronize(st);
-    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Schwarz kernel failed: ") + hipGetErrorString(e));
-    HIPCHK(hipMemcpy(K, dK.p, npairs * sizeof(double), hipMemcpyDeviceToHost));
+    schwarz_impl(B, npairs, pairs, precision, norm, K, nullptr);
   });
 }
 
+int l2g_precision_bounds(l2g_basis B, int64_t npairs, const int32_t* pairs, double* K, double* S) {
+  return guarded([&] {
+    if (!B) throw L2gError(L2G_E_STATE, "null basis");
+    if (npairs < 0 || (npairs > 0 && (!pairs || !K || !S))) throw std::invalid_argument("l2g_precision_bounds: bad arguments");
+    if (npairs == 0) return;
+    schwarz_impl(B, npairs, pairs, 0.0, 0, K, S);
+  });
+}
 
 }  // extern "C"
 
@@ -948,10 +1112,12 @@ struct l2g_fock_s {
   std::vector<int> bf1st;
   std::vector<std::vector<int>> splist;   // splist[s1] ascending s2 <= s1
   std::vector<double> K;                  // nsh x nsh Schwarz (inf norm), significant pairs only
+  std::vector<double> S;                  // nsh x nsh FP32-selection factors (schwarz_impl)
   std::vector<std::vector<double>> pfac;  // primitive Schwarz factors per significant pair (loop order)
   std::vector<uint64_t> pfac_key;
-  int64_t last_nq = 0, last_nprim = 0;
+  int64_t last_nq = 0, last_nprim = 0, last_nq32 = 0;
   float last_ms = 0;
+  double fp32_threshold = 0;  // quartets with K12 K34 below it are evaluated in FP32
   DevBuf dens, jg, kg;
 };
 
@@ -989,14 +1155,16 @@ int l2g_fock_create(l2g_basis B, double pair_threshold, double max_engine_precis
       }
     const int64_t np = (int64_t)pairs.size() / 2;
     // Schwarz factors of the significant pairs, no primitive screening
-    std::vector<double> Kp(np);
-    int rc = l2g_schwarz(B, np, pairs.data(), 0.0, 0, Kp.data());
-    if (rc) throw L2gError(rc, g_err);
+    std::vector<double> Kp(np), Sp(np);
+    if (np > 0) schwarz_impl(B, np, pairs.data(), 0.0, 0, Kp.data(), Sp.data());
     f->K.assign((size_t)n * n, 0.0);
+    f->S.assign((size_t)n * n, 0.0);
     for (int64_t i = 0; i < np; ++i) {
       const int a = pairs[2 * i], b = pairs[2 * i + 1];
       f->K[(size_t)a * n + b] = f->K[(size_t)b * n + a] = Kp[i];
+      f->S[(size_t)a * n + b] = f->S[(size_t)b * n + a] = Sp[i];
     }
+    int rc = 0;
     // primitive-pair Schwarz factors from normalization-free primitives
     // (Shell::extract_primitive(p, false), shell.h:938-947), on the GPU
     if (screening == L2G_SCREEN_SCHWARZ || screening == L2G_SCREEN_SCHWARZINF) {
@@ -1040,6 +1208,7 @@ int l2g_fock_create(l2g_basis B, double pair_threshold, double max_engine_precis
         f->pfac_key[i] = (uint64_t)(uint32_t)a << 32 | (uint32_t)b;
       }
     }
+    ++B->ctx->live;
     *out = f.release();
   });
 }
@@ -1082,7 +1251,9 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     PairBuilder pb(*B, std::log(f->max_engine_precision), f->method);
     for (size_t i = 0; i < f->pfac.size(); ++i) pb.pfac[f->pfac_key[i]] = &f->pfac[i];
     // permutationally-unique screened quartets (hartree-fock++.cc:1628-1696)
-    std::vector<WorkLists> W(l2g_gen_nfock);
+    const int nf64 = l2g_gen_nfock;
+    std::vector<WorkLists> W(l2g_gen_nfock + l2g_gen_nfock32);
+    int64_t nq32 = 0;
     int64_t nq = 0, nprim = 0;
     for (int s1 = 0; s1 < n; ++s1)
       for (int s2 : f->splist[s1]) {
@@ -1097,15 +1268,24 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
             if (Dn1234 * f->K[(size_t)s1 * n + s2] * f->K[(size_t)s3 * n + s4] < precision) continue;
             const double qprec = Dn1234 != 0. ? precision / Dn1234 : needed;
             const int deg = (s1 == s2 ? 1 : 2) * (s3 == s4 ? 1 : 2) * (s1 == s3 ? (s2 == s4 ? 1 : 2) : 2);
-            // canonical order (engine.impl.h:1181-1208)
+            // canonical order (engine.impl.h:1181-1208), equal-l pairs
+            // oriented by second_is_centre
             const int l1 = B->l[s1], l2 = B->l[s2], l3 = B->l[s3], l4 = B->l[s4];
-            const bool stb = l1 < l2, stk = l3 < l4, sbk = (l1 + l2) > (l3 + l4);
+            const bool stb = B->swap_pair(s1, s2), stk = B->swap_pair(s3, s4), sbk = (l1 + l2) > (l3 + l4);
             const int b1 = sbk ? (stk ? s4 : s3) : (stb ? s2 : s1);
             const int b2 = sbk ? (stk ? s3 : s4) : (stb ? s1 : s2);
             const int k1 = sbk ? (stb ? s2 : s1) : (stk ? s4 : s3);
             const int k2 = sbk ? (stb ? s1 : s2) : (stk ? s3 : s4);
-            const int cls = ctx->fock_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+            int cls = ctx->fock_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
             if (cls < 0) throw L2gError(L2G_E_LMAX, "no Fock kernel for this angular momentum class");
+            if (f->fp32_threshold > 0 && f->K[(size_t)s1 * n + s2] * f->K[(size_t)s3 * n + s4] < f->fp32_threshold &&
+                f->S[(size_t)s1 * n + s2] * f->S[(size_t)s3 * n + s4] < L2G_FP32_HRR_FACTOR * f->fp32_threshold) {
+              const int c32 = ctx->fock32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+              if (c32 >= 0) {
+                cls = nf64 + c32;
+                ++nq32;
+              }
+            }
             auto purebit = [&](int sh) { return (B->pure[sh] && B->l[sh] > 0) ? 1 : 0; };
             WorkLists& w = W[cls];
             const int bp = pb.get(b1, b2), kp = pb.get(k1, k2);
@@ -1123,6 +1303,8 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       }
     f->last_nq = nq;
     f->last_nprim = nprim;
+    f->last_nq32 = nq32;
+    auto finfo = [&](int c) -> const L2gClassInfo& { return c < nf64 ? l2g_gen_fock[c] : l2g_gen_fock32[c - nf64]; };
     hipStream_t st = ctx->stream;
     // device data
     DevBuf pf, pn, pab, pP, pPA, pAux, pNs, yb;
@@ -1142,9 +1324,9 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     HIPCHK(hipMemsetAsync(f->kg.p, 0, NN * sizeof(double), st));
     const int block = 128;
     int64_t maxn = 0, ymax = 0;
-    for (int c = 0; c < l2g_gen_nfock; ++c) {
+    for (int c = 0; c < (int)W.size(); ++c) {
       maxn = std::max<int64_t>(maxn, (int64_t)W[c].bra.size());
-      if (!W[c].bra.empty()) ymax = std::max<int64_t>(ymax, l2g_gen_fock[c].ybuf_doubles);
+      if (!W[c].bra.empty()) ymax = std::max<int64_t>(ymax, finfo(c).ybuf_doubles);
     }
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
     const int64_t ystride = (int64_t)grid * block;
@@ -1155,7 +1337,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, st));
-    for (int c = 0; c < l2g_gen_nfock; ++c) {
+    for (int c = 0; c < (int)W.size(); ++c) {
       WorkLists& w = W[c];
       if (w.bra.empty()) continue;
       // sort by contraction depth for SIMT uniformity
@@ -1214,7 +1396,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       A.jg = f->jg.as<double>();
       A.kg = f->kg.as<double>();
       A.nbf = N;
-      l2g_gen_fock[c].launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
+      finfo(c).launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(e1, st));
@@ -1239,6 +1421,21 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
   });
 }
 
+int l2g_fock_set_fp32_threshold(l2g_fock f, double threshold) {
+  return guarded([&] {
+    if (!f) throw std::invalid_argument("null argument");
+    if (!(threshold >= 0)) throw std::invalid_argument("threshold must be >= 0");
+    f->fp32_threshold = threshold;
+  });
+}
+
+int l2g_fock_stats_mixed(l2g_fock f, int64_t* nquartets_fp32) {
+  return guarded([&] {
+    if (!f) throw std::invalid_argument("null argument");
+    if (nquartets_fp32) *nquartets_fp32 = f->last_nq32;
+  });
+}
+
 int l2g_fock_stats(l2g_fock f, int64_t* nquartets, int64_t* nprim_quartets, float* kernel_ms) {
   return guarded([&] {
     if (!f) throw std::invalid_argument("null argument");
@@ -1249,7 +1446,13 @@ int l2g_fock_stats(l2g_fock f, int64_t* nquartets, int64_t* nprim_quartets, floa
 }
 
 int l2g_fock_destroy(l2g_fock f) {
-  delete f;
+  if (f) {
+    l2g_context_s* ctx = f->basis->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete f;
+    --ctx->live;
+  }
   return L2G_OK;
 }
 

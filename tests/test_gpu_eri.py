@@ -163,3 +163,22 @@ def test_schwarz_factors(ctx):
     # square root amplifies the ~1e-17 absolute noise of tiny norms
     assert compare(Kinf ** 2, Oinf ** 2, ABS, REL)[0] == 0
     assert compare(K2 ** 2, O2 ** 2, ABS, REL)[0] == 0
+
+
+def test_context_close_releases_children():
+    """Closing a context with a basis and a plan still referenced releases
+    them first; the C ABI itself refuses to finalize a context with live
+    objects (L2G_E_STATE) instead of leaving them dangling."""
+    say("case: context lifetime")
+    c = la.Context(0)
+    rng = np.random.default_rng(3)
+    b = la.Basis(c, la.BasisArrays.from_shells(random_shells(rng, [0, 1])))
+    plan = la.EriPlan(b, [[0, 1, 0, 1]])
+    plan.execute()
+    rc = la.lib().l2g_finalize(c.handle)
+    assert rc == la.L2G_E_STATE
+    c.close()
+    with pytest.raises(la.L2gError):
+        plan.execute()
+    with pytest.raises(la.L2gError):
+        b.handle

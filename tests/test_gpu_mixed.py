@@ -1,0 +1,103 @@
+"""FP32/FP64 mixed precision: quartets with Schwarz bound K12 K34 < 1e-6 and
+pre-HRR bound S12 S34 < 16e-6 run in FP32, the rest in FP64.  Errors are
+reported per class against the FP64 oracle; FP64 quartets keep the 1e-12 /
+1e-10 bar, FP32 quartets must stay within 1e-11 absolute (1e-5 of the
+threshold) and within 64 eps_fp32 S12 S34, and the mixed Fock matrix within
+1e-8."""
+import sys
+
+import numpy as np
+import pytest
+
+import libint_amd as la
+import oracle
+from libint_amd import molecule
+from tests.helpers import compare_arbitrated
+
+pytestmark = pytest.mark.gpu
+
+THR = 1e-6
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_mixed_eri_per_class_errors(ctx):
+    mol = molecule.make_molecule(2, natoms=6)
+    B = mol.basis
+    b = la.Basis(ctx, B)
+    allq = molecule.unique_quartets(B.nshell)
+    rng = np.random.default_rng(4)
+    q = allq[rng.choice(len(allq), 60000, replace=False)]
+    say(f"case: mixed ERI, {len(q)} quartets of workload-2 recipe (6 atoms)")
+    plan = la.EriPlan(b, q, fp32_threshold=THR)
+    offs = plan.offsets()
+    plan.execute()
+    g, _ = plan.download()
+    st = plan.stats()
+    n32 = sum(s["nquartets"] for s in st if s["fp32"])
+    o, ooff, _ = oracle.eri_quartets(B, q)
+    assert np.array_equal(offs, ooff)
+    # per-pair K and S: a quartet runs in FP32 iff K K < THR and S S < 16 THR
+    i, j = np.tril_indices(B.nshell)
+    Kv, Sv = la.precision_bounds(b, np.stack([i, j], 1))
+    K = np.zeros((B.nshell, B.nshell))
+    S = np.zeros_like(K)
+    K[i, j] = K[j, i] = Kv
+    S[i, j] = S[j, i] = Sv
+    KK = K[q[:, 0], q[:, 1]] * K[q[:, 2], q[:, 3]]
+    SS = S[q[:, 0], q[:, 1]] * S[q[:, 2], q[:, 3]]
+    sel = (KK < THR) & (SS < la.FP32_HRR_FACTOR * THR)
+    sizes = B.sizes()
+    n = sizes[q[:, 0]] * sizes[q[:, 1]] * sizes[q[:, 2]] * sizes[q[:, 3]]
+    is32 = np.repeat(sel, n)
+    say(f"  {n32} of {len(q)} quartets in FP32 ({int((KK < THR).sum())} below the Schwarz threshold)")
+    assert n32 > 0 and n32 == int(sel.sum())
+    # FP64 quartets at the integral bar, oracle misses settled by brute force
+    q64 = ~sel
+    g64 = np.concatenate([g[a:a + m] for a, m in zip(offs[q64], n[q64])])
+    o64 = np.concatenate([o[a:a + m] for a, m in zip(offs[q64], n[q64])])
+    off64 = np.concatenate([[0], np.cumsum(n[q64])[:-1]])
+    nf64, narb, oerr = compare_arbitrated(B, q[q64], off64, g64, o64, 1e-12, 1e-10)
+    say(f"  FP64 part: fails {nf64}; {narb} values arbitrated by brute force "
+        f"(oracle's own error there up to {oerr:.2e})")
+    # FP32 quartets: error per quartet against eps_fp32 S_ab S_cd
+    ends = offs + n
+    e32 = np.array([np.abs(g[a:e] - o[a:e]).max() for a, e in zip(offs[sel], ends[sel])])
+    ratio = e32 / np.maximum(EPS32 * SS[sel], 1e-300)
+    say(f"  FP32 part: max abs {e32.max():.3e}; max err/(eps32 S S) {ratio[e32 > 1e-16].max(initial=0):.2f}")
+    ls = B.l[q]
+    cls = np.array([f"{a}{bb}{c}{d}" for a, bb, c, d in ls])
+    for c in sorted(set(cls[sel])):
+        m = (cls == c) & sel
+        gm = np.concatenate([g[a:e] for a, e in zip(offs[m], ends[m])])
+        om = np.concatenate([o[a:e] for a, e in zip(offs[m], ends[m])])
+        rel = np.abs(gm - om) / np.maximum(np.abs(om), 1e-300)
+        say(f"  class ({c[0]}{c[1]}|{c[2]}{c[3]}) fp32 x{int(m.sum())}: max abs {np.abs(gm - om).max():.2e} "
+            f"max rel (|ref|>1e-14) {rel[np.abs(om) > 1e-14].max(initial=0):.2e}")
+    assert nf64 == 0
+    # FP32 bar: 1e-5 of the threshold absolute, and within 64 eps_fp32 S S
+    assert e32.max() <= 1e-5 * THR
+    assert (e32 <= 64 * EPS32 * SS[sel] + 1e-16).all()
+
+
+def test_mixed_fock_error(ctx):
+    mol = molecule.make_molecule(2, natoms=4)
+    rng = np.random.default_rng(12)
+    C = rng.normal(0.0, 0.1, (mol.nbf, mol.nbf // 5))
+    D = C @ C.T
+    b = la.Basis(ctx, mol.basis)
+    fb = la.FockBuilder(b)
+    G64, J64, K64 = fb.build(D, 1e-12)
+    fb.set_fp32_threshold(THR)
+    G32, J32, K32 = fb.build(D, 1e-12)
+    st = fb.stats()
+    Go, Jo, Ko, _ = oracle.fock_2e(mol.basis, D, 1e-12)
+    say(f"case: mixed Fock nbf={mol.nbf}: {st['nquartets_fp32']} of {st['nquartets']} quartets FP32; "
+        f"|G32-Go| {np.abs(G32 - Go).max():.2e} |G64-Go| {np.abs(G64 - Go).max():.2e}")
+    assert st["nquartets_fp32"] > 0
+    assert np.abs(G32 - Go).max() <= 1e-8
+    assert np.abs(G64 - Go).max() <= 1e-8

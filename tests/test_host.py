@@ -128,11 +128,13 @@ def test_boys_table_vs_oracle_table():
     assert np.abs(t[:, :n, :] - np.transpose(to, (1, 0, 2))[:, :n, :]).max() < 4e-15
 
 
-def test_generator_groups_cover_targets_once():
+@pytest.mark.parametrize("budget", [40, 64, 96])
+def test_generator_groups_cover_targets_once(budget):
+    """The target blocks partition the contracted [e0|f0] set exactly."""
     for c in gen_eri.canonical_classes(2):
-        p = gen_eri.plan_class(*c)
-        flat = sorted(i for g in p.groups for i in g)
-        assert flat == list(range(len(p.ecomps)))
+        p = gen_eri.plan_class(*c, budget=budget)
+        flat = sorted((i, j) for es, fs in p.groups for i in es for j in fs)
+        assert flat == [(i, j) for i in range(len(p.ecomps)) for j in range(len(p.fcomps))]
 
 
 def test_flop_model_matches_canonical_counts():
