@@ -208,14 +208,79 @@ int l2g_fock_build(l2g_fock fock, const double* D, double precision, double* J, 
 int l2g_fock_stats(l2g_fock fock, int64_t* nquartets, int64_t* nprim_quartets, float* kernel_ms);
 
 /* Quartets of later builds whose Schwarz bound K12 K34 is below threshold
- * are evaluated in FP32 (digest and accumulation stay FP64); 0 = all FP64.
- * No libint counterpart. */
+ * (and whose pre-HRR bound S12 S34 is below 16 threshold, see
+ * l2g_precision_bounds) are evaluated in FP32; Boys function, prefactors,
+ * digest and accumulation stay FP64; 0 = all FP64.  No libint counterpart. */
 int l2g_fock_set_fp32_threshold(l2g_fock fock, double threshold);
 
 /* Number of quartets the last build evaluated in FP32. */
 int l2g_fock_stats_mixed(l2g_fock fock, int64_t* nquartets_fp32);
 
 int l2g_fock_destroy(l2g_fock fock);
+
+/* ---- Fock-build request server ------------------------------------------ */
+
+/* A server runs Fock builds (as l2g_fock_build) for a stream of requests,
+ * each with its own molecule and density.  It has no counterpart in the
+ * reference, whose callers loop over builds one at a time on the CPU; the
+ * reference-side equivalent of one request is compute_shellpairs +
+ * compute_schwarz_ints + compute_2body_fock
+ * (tests/hartree-fock/hartree-fock++.cc:1231-1772).
+ *
+ * A background admission thread takes submitted requests in order, builds
+ * their pair list, Schwarz factors and screened quartet list, and admits
+ * each one as soon as a slot and device memory are free; the request's
+ * shell-pair data, density and J/K accumulators then stay resident on the
+ * device until it completes.  A launch thread runs rounds of the Fock
+ * kernels: each round takes up to round_quartets quartets, split across all
+ * admitted requests, and launches each kernel class once over the quartets
+ * of every request in it (continuous batching).  A request that arrives
+ * while others run joins the next round after its admission. */
+typedef struct l2g_server_s* l2g_server;
+
+typedef struct {
+  int device;                  /* HIP device */
+  double hbm_bytes;            /* device memory for admitted requests' data; 0 = 16 GiB */
+  int max_active;              /* requests admitted at once; 0 = 16 */
+  int64_t round_quartets;      /* quartets per launch round across requests; 0 = 1 << 20 */
+  int screening;               /* primitive screening; 0 = L2G_SCREEN_SCHWARZINF */
+  double pair_threshold;       /* significant-pair threshold; 0 = 1e-12 */
+  double max_engine_precision; /* shell-pair precision; 0 = epsilon / 1e10 */
+} l2g_server_config;
+
+#define L2G_REQ_QUEUED 0   /* submitted, not yet admitted */
+#define L2G_REQ_ACTIVE 1   /* admitted: data resident, quartets in launch rounds */
+#define L2G_REQ_DONE 2     /* J, K, G ready for l2g_request_result */
+#define L2G_REQ_FAILED 3   /* l2g_request_wait returns the error */
+
+int l2g_server_create(const l2g_server_config* config, l2g_server* server);
+
+/* Submits one Fock build: a basis (as l2g_basis_create, coefficients
+ * renormalized), its density D (nbf x nbf row-major) and the Fock precision
+ * (as l2g_fock_build); fp32_threshold as l2g_fock_set_fp32_threshold.  All
+ * inputs are copied; returns at once with the request id. */
+int l2g_request_submit(l2g_server server, int nshell, const int* l, const int* pure, const int* nprim,
+                       const double* alpha, const double* coeff, const double* origin, const double* D,
+                       double precision, double fp32_threshold, int64_t* id);
+
+/* State of a request (L2G_REQ_*). */
+int l2g_request_state(l2g_server server, int64_t id, int* state);
+
+/* Blocks until the request is done (returns 0) or failed (returns its error). */
+int l2g_request_wait(l2g_server server, int64_t id);
+
+/* Copies the results of a done request (J, K, G: nbf x nbf row-major, any
+ * may be NULL) and its times in ms (ms[0] submit to admission, ms[1] submit
+ * to done; may be NULL), then releases the request. */
+int l2g_request_result(l2g_server server, int64_t id, double* J, double* K, double* G, double* ms);
+
+/* Counters: launch rounds run, the most requests that shared one round,
+ * quartets evaluated. */
+int l2g_server_stats(l2g_server server, int64_t* rounds, int64_t* max_requests_per_round, int64_t* nquartets);
+
+/* Waits for every submitted request to finish, then stops the threads and
+ * frees the server (results not yet collected are dropped). */
+int l2g_server_destroy(l2g_server server);
 
 /* One-shot convenience: plan, execute, download, destroy.  offsets and
  * nonzero may be NULL. */

@@ -33,6 +33,7 @@ SCREEN_ORIGINAL, SCREEN_CONSERVATIVE, SCREEN_SCHWARZ, SCREEN_SCHWARZINF = 0x0001
 SHG_STANDARD, SHG_GAUSSIAN = 1, 2
 # error codes of include/l2g.h
 L2G_OK, L2G_E_INVALID, L2G_E_LMAX, L2G_E_NODEVICE, L2G_E_OOM, L2G_E_LAUNCH, L2G_E_STATE = 0, -1, -2, -3, -4, -5, -6
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -77,7 +78,23 @@ ABI = {
     "l2g_fock_set_fp32_threshold": (ctypes.c_int, [_vp, ctypes.c_double]),
     "l2g_fock_stats_mixed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_precision_bounds": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, _f64p, _f64p]),
+    "l2g_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_vp)]),
+    "l2g_request_submit": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                          ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
+    "l2g_request_state": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int)]),
+    "l2g_request_wait": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "l2g_request_result": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp]),
+    "l2g_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64)]),
+    "l2g_server_destroy": (ctypes.c_int, [_vp]),
 }
+
+
+class ServerConfig(ctypes.Structure):
+    """l2g_server_config of include/l2g.h."""
+    _fields_ = [("device", ctypes.c_int), ("hbm_bytes", ctypes.c_double), ("max_active", ctypes.c_int),
+                ("round_quartets", ctypes.c_int64), ("screening", ctypes.c_int), ("pair_threshold", ctypes.c_double),
+                ("max_engine_precision", ctypes.c_double)]
 
 _lib = None
 
@@ -475,3 +492,64 @@ class Engine:
 
     def compute_batch(self, basis: Basis, quartets):
         return compute_quartets(basis, quartets, self.precision, self.screening, self.scale, self.uniform_cart)
+
+
+class Server:
+    """Fock-build request server (l2g_server_* of include/l2g.h): requests
+    are admitted as slots and device memory free up, and their quartets share
+    launch rounds."""
+
+    def __init__(self, device: int = 0, hbm_bytes: float = 0.0, max_active: int = 0, round_quartets: int = 0,
+                 screening: int = 0, pair_threshold: float = 0.0, max_engine_precision: float = 0.0):
+        cfg = ServerConfig(device, hbm_bytes, max_active, round_quartets, screening, pair_threshold,
+                           max_engine_precision)
+        h = _vp()
+        _check(lib().l2g_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._nbf = {}
+
+    def submit(self, arrays: BasisArrays, D, precision: float = 1e-12, fp32_threshold: float = 0.0) -> int:
+        a = arrays
+        D = np.ascontiguousarray(D, np.float64)
+        nbf = int(a.sizes().sum())
+        assert D.shape == (nbf, nbf)
+        rid = ctypes.c_int64()
+        _check(lib().l2g_request_submit(self._h, a.nshell, a.l, a.pure, a.nprim, a.alpha, a.coeff, a.origin,
+                                        D.reshape(-1), precision, fp32_threshold, ctypes.byref(rid)))
+        self._nbf[rid.value] = nbf
+        return rid.value
+
+    def state(self, rid: int) -> int:
+        st = ctypes.c_int()
+        _check(lib().l2g_request_state(self._h, rid, ctypes.byref(st)))
+        return st.value
+
+    def wait(self, rid: int) -> None:
+        _check(lib().l2g_request_wait(self._h, rid))
+
+    def result(self, rid: int):
+        """Waits for the request and returns (G, J, K, ms) with ms = (submit to
+        admission, submit to done) in milliseconds; releases the request."""
+        self.wait(rid)
+        n = self._nbf.pop(rid)
+        G, J, K = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        ms = np.zeros(2)
+        _check(lib().l2g_request_result(self._h, rid, J.ctypes.data, K.ctypes.data, G.ctypes.data, ms.ctypes.data))
+        return G, J, K, ms
+
+    def stats(self):
+        r, m, q = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().l2g_server_stats(self._h, ctypes.byref(r), ctypes.byref(m), ctypes.byref(q)))
+        return dict(rounds=r.value, max_requests_per_round=m.value, nquartets=q.value)
+
+    def close(self) -> None:
+        if self._h:
+            lib().l2g_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -747,8 +747,12 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         E("if (nsurv == 0) continue;  // every primitive screened: libint returns nullptr, nothing to digest")
         E("const int4 bf = A.q_bf[qi];")
         E("const double deg = A.q_scale[qi];")
-        E("const long long N = A.nbf;")
-        E("const double* __restrict__ Dm = A.dens;")
+        E("// matrices of this work item's request: D, jg, kg share one offset")
+        E("const L2gMatDesc md = A.mats[A.q_req ? A.q_req[qi] : 0];")
+        E("const long long N = md.n;")
+        E("const double* __restrict__ Dm = A.dens + md.off;")
+        E("double* __restrict__ jgm = A.jg + md.off;")
+        E("double* __restrict__ kgm = A.kg + md.off;")
     E("// ---- bra HRR, bra spherical transform and permuted store")
     E(f"const L2gStore st = make_store(meta, A, qi, {la}, {lb}, {lc}, {ld});")
     E("const int ncol = st.nk1 * st.nk2;")
@@ -786,13 +790,13 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
                         E(f"jcol += d12_{i1}_{i2} * {v}; j12_{i1}_{i2} += d34 * {v};")
                         E(f"k13_{i1} += d24_{i2} * {v}; k14_{i1} += d23_{i2} * {v};")
                         E(f"k24_{i2} += d13_{i1} * {v}; k23_{i2} += d14_{i1} * {v};")
-                E("unsafeAtomicAdd(&A.jg[bk3 * N + bk4], deg * jcol);")
+                E("unsafeAtomicAdd(&jgm[bk3 * N + bk4], deg * jcol);")
                 for i1 in range(d1):
-                    E(f"unsafeAtomicAdd(&A.kg[ba1_{i1} * N + bk3], deg * k13_{i1});")
-                    E(f"unsafeAtomicAdd(&A.kg[ba1_{i1} * N + bk4], deg * k14_{i1});")
+                    E(f"unsafeAtomicAdd(&kgm[ba1_{i1} * N + bk3], deg * k13_{i1});")
+                    E(f"unsafeAtomicAdd(&kgm[ba1_{i1} * N + bk4], deg * k14_{i1});")
                 for i2 in range(d2):
-                    E(f"unsafeAtomicAdd(&A.kg[ba2_{i2} * N + bk4], deg * k24_{i2});")
-                    E(f"unsafeAtomicAdd(&A.kg[ba2_{i2} * N + bk3], deg * k23_{i2});")
+                    E(f"unsafeAtomicAdd(&kgm[ba2_{i2} * N + bk4], deg * k24_{i2});")
+                    E(f"unsafeAtomicAdd(&kgm[ba2_{i2} * N + bk3], deg * k23_{i2});")
             else:
                 for i1 in range(d1):
                     for i2 in range(d2):
@@ -840,7 +844,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     if mode == 'fock':
         for i1 in range(ncart(la)):
             for i2 in range(ncart(lb)):
-                E(f"if ({i1} < st.n1 && {i2} < st.n2) unsafeAtomicAdd(&A.jg[(bf.x + st.pos(st.l1, st.p1, {i1})) * N + "
+                E(f"if ({i1} < st.n1 && {i2} < st.n2) unsafeAtomicAdd(&jgm[(bf.x + st.pos(st.l1, st.p1, {i1})) * N + "
                   f"bf.y + st.pos(st.l2, st.p2, {i2})], deg * j12_{i1}_{i2});")
     E.end()  # quartet loop
     E.ind -= 1

@@ -20,6 +20,13 @@
 //            K = sqrt(2) pi^(5/4) exp(-rho |AB|^2) / gamma (shell.h:1244)
 //   pp_aux = (ln_scr, 1 / (2 gamma))
 //   pp_nonsph = conservative-screening factor (shell.h:1208-1227)
+// Where one request's nbf x nbf matrices sit in the Fock arenas: D at
+// dens + off, the jg / kg accumulators at jg + off / kg + off.
+struct L2gMatDesc {
+  long long off;
+  long long n;
+};
+
 struct L2gEriArgs {
   const int* __restrict__ bra_pair;
   const int* __restrict__ ket_pair;
@@ -49,10 +56,11 @@ struct L2gEriArgs {
   const double* __restrict__ q_lnprec;   // per work item ln(precision)
   const double* __restrict__ q_scale;    // per work item degeneracy factor
   const int4* __restrict__ q_bf;         // first basis function of canonical b1, b2, k1, k2
-  const double* __restrict__ dens;       // density, nbf x nbf row-major
-  double* __restrict__ jg;               // unsymmetrised Coulomb accumulator
-  double* __restrict__ kg;               // unsymmetrised exchange accumulator
-  long long nbf;
+  const double* __restrict__ dens;       // density arena (row-major nbf x nbf blocks)
+  double* __restrict__ jg;               // unsymmetrised Coulomb accumulator arena
+  double* __restrict__ kg;               // unsymmetrised exchange accumulator arena
+  const L2gMatDesc* __restrict__ mats;   // matrix block of each request
+  const int* __restrict__ q_req;         // request of each work item (null: request 0)
   // Schwarz kernels: optional sqrt(max |[e0|e0]|) of the pre-HRR diagonal
   double* __restrict__ out2;
 };

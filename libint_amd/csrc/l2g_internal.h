@@ -1,0 +1,306 @@
+// Internals shared by the host translation units of the C ABI (l2g_host.cpp,
+// l2g_fock.cpp): errors, device buffers, context, basis and shell-pair data.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/l2g.h"
+#include "eri_device.h"
+
+extern const L2gClassInfo l2g_gen_classes[];
+extern const int l2g_gen_nclasses;
+extern const L2gClassInfo l2g_gen_schwarz[];
+extern const int l2g_gen_nschwarz;
+extern const L2gClassInfo l2g_gen_fock[];
+extern const int l2g_gen_nfock;
+extern const L2gClassInfo l2g_gen_classes32[];
+extern const int l2g_gen_nclasses32;
+extern const L2gClassInfo l2g_gen_fock32[];
+extern const int l2g_gen_nfock32;
+extern const int l2g_gen_lmax;
+
+
+namespace l2g_impl {
+
+
+inline thread_local std::string g_err;
+
+struct L2gError : std::runtime_error {
+  int code;
+  L2gError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess)                                                                           \
+      throw L2gError(e_ == hipErrorOutOfMemory ? L2G_E_OOM : L2G_E_NODEVICE,                      \
+                     std::string(#x) + ": " + hipGetErrorString(e_));                              \
+  } while (0)
+
+template <class F>
+inline int guarded(F&& f) {
+  try {
+    f();
+    return L2G_OK;
+  } catch (const L2gError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return L2G_E_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return L2G_E_INVALID;
+  }
+}
+
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+inline int nfun(int l, int pure) { return pure ? 2 * l + 1 : ncart(l); }
+
+
+// ---------------------------------------------------------------------------
+// normalization and solid harmonics
+// ---------------------------------------------------------------------------
+inline double double_factorial_km1(int k) {  // (k-1)!!
+  double r = 1;
+  for (int i = k - 1; i > 1; i -= 2) r *= i;
+  return r;
+}
+inline double factorial(int n) {
+  double r = 1;
+  for (int i = 2; i <= n; ++i) r *= i;
+  return r;
+}
+inline double choose(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  return factorial(n) / (factorial(k) * factorial(n - k));
+}
+inline int sgn_parity(int i) { return (i & 1) ? -1 : 1; }
+
+// ---------------------------------------------------------------------------
+// objects behind the opaque handles
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p && bytes >= b) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return;
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+inline void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(std::max<size_t>(h.size() * sizeof(T), 16));
+  if (!h.empty()) HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+
+template <class F>
+void parallel_for(int n, F&& f) {
+  const int nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  std::atomic<int> next{0};
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (int i = next++; i < n; i = next++) f(i);
+    });
+  for (auto& t : th) t.join();
+}
+
+}  // namespace l2g_impl
+
+struct l2g_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  l2g_impl::DevBuf boys;
+  int shg_gaussian = 0;
+  int ncu = 256;
+  int class_of[8][8][8][8];  // index into l2g_gen_classes
+  int fock_of[8][8][8][8];   // index into l2g_gen_fock (its own order)
+  int class32_of[8][8][8][8];  // index into l2g_gen_classes32
+  int fock32_of[8][8][8][8];   // index into l2g_gen_fock32
+  int schwarz_of[8][8];
+  // bases, plans and Fock builders created on this context and not yet
+  // destroyed; l2g_finalize refuses while any is alive
+  std::atomic<int> live{0};
+};
+
+struct l2g_basis_s {
+  l2g_context_s* ctx;
+  int nshell;
+  std::vector<int> l, pure, nprim, off;
+  std::vector<double> alpha, coeff, lnc, origin;
+  std::unordered_map<uint64_t, bool> orient;  // second_is_centre() memo
+
+  ~l2g_basis_s() {
+    if (ctx) --ctx->live;
+  }
+
+  // Which shell of a pair with equal l is the HRR build centre.  The
+  // reference keeps the caller's order for equal l (engine.impl.h:1181-1208);
+  // the integral does not depend on the choice, its rounding does: HRR
+  // (a,b) = sum_k C(b,k) AB^k (a+k,0) cancels badly when the build centre is
+  // the diffuse shell of a distant pair, because the pair's charge sits near
+  // the tight shell.  True when shell j is tighter, i.e. when the
+  // K-weighted mean of |P - B_j| = alpha_i/gamma |AB| is below that of
+  // |P - A_i| = alpha_j/gamma |AB|.
+  bool second_is_centre(int i, int j) {
+    if (i == j || l[i] != l[j]) return false;
+    const uint64_t key = (uint64_t)(uint32_t)i << 32 | (uint32_t)j;
+    auto it = orient.find(key);
+    if (it != orient.end()) return it->second;
+    double AB2 = 0;
+    for (int k = 0; k < 3; ++k) AB2 += (origin[3 * i + k] - origin[3 * j + k]) * (origin[3 * i + k] - origin[3 * j + k]);
+    double s = 0;
+    if (AB2 > 0)
+      for (int p1 = 0; p1 < nprim[i]; ++p1)
+        for (int p2 = 0; p2 < nprim[j]; ++p2) {
+          const double a1 = alpha[off[i] + p1], a2 = alpha[off[j] + p2], g = a1 + a2;
+          const double w = std::fabs(coeff[off[i] + p1] * coeff[off[j] + p2]) * std::exp(-a1 * a2 / g * AB2) / g;
+          s += w * (a2 - a1) / g;
+        }
+    const bool r = s > 0;
+    orient.emplace(key, r);
+    orient.emplace((uint64_t)(uint32_t)j << 32 | (uint32_t)i, s < 0);
+    return r;
+  }
+  // canonical bra orientation of (i, j): l_i >= l_j, ties by second_is_centre
+  bool swap_pair(int i, int j) { return l[i] < l[j] || (l[i] == l[j] && second_is_centre(i, j)); }
+};
+
+// FP32 selection of a mixed batch: a quartet runs in FP32 when its Schwarz
+// bound K_ab K_cd is below the caller's threshold and its pre-HRR bound
+// S_ab S_cd (schwarz_impl) is below this factor times the threshold, which
+// keeps eps_fp32 S_ab S_cd, the scale of the FP32 rounding that reaches an
+// integral, near eps_fp32 times the threshold.
+#ifndef L2G_FP32_HRR_FACTOR
+#define L2G_FP32_HRR_FACTOR 16.0
+#endif
+
+
+namespace l2g_impl {
+
+void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double precision, int norm, double* K,
+                  double* S);
+
+// l2g_aux.hip: in-place Fock symmetrisation of one N x N block (J into jg,
+// K into kg, G = 2J - K into g)
+void launch_fock_finish(double* jg, double* kg, double* g, long long N, hipStream_t s);
+
+
+// Shell-pair data for ordered pair (i, j), restating ShellPair::init
+// (include/libint2/shell.h:1137-1256) for Original and Conservative
+// screening, with the pair's contraction coefficients folded into K.
+struct PairBuilder {
+  const l2g_basis_s& B;
+  double ln_prec;
+  int method;
+  std::vector<int> first, np;
+  std::vector<double> AB;
+  std::vector<double4> P, PA;
+  std::vector<double2> aux;
+  std::vector<double> ns;
+  std::unordered_map<uint64_t, int> id;
+
+  // Schwarz / SchwarzInf primitive screening (shell.h:1258-1328): factors of
+  // the ordered pair (i, j), p1-major, supplied per pair by the caller
+  std::unordered_map<uint64_t, const std::vector<double>*> pfac;
+
+  PairBuilder(const l2g_basis_s& b, double lp, int m) : B(b), ln_prec(lp), method(m) {}
+
+  int get(int i, int j) {
+    const uint64_t key = (uint64_t)(uint32_t)i << 32 | (uint32_t)j;
+    auto it = id.find(key);
+    if (it != id.end()) return it->second;
+    const std::vector<double>* fac = nullptr;
+    bool fac_t = false;
+    if (method == L2G_SCREEN_SCHWARZ || method == L2G_SCREEN_SCHWARZINF) {
+      auto f = pfac.find(key);
+      if (f == pfac.end()) {
+        f = pfac.find((uint64_t)(uint32_t)j << 32 | (uint32_t)i);
+        fac_t = true;
+      }
+      if (f == pfac.end()) throw L2gError(L2G_E_STATE, "no primitive Schwarz factors for a shell pair");
+      fac = f->second;
+    }
+    const int pid = (int)first.size();
+    id.emplace(key, pid);
+    first.push_back((int)P.size());
+    const double* A = &B.origin[3 * i];
+    const double* Bo = &B.origin[3 * j];
+    double ab[3], AB2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      ab[k] = A[k] - Bo[k];
+      AB2 += ab[k] * ab[k];
+      AB.push_back(ab[k]);
+    }
+    const int n1 = B.nprim[i], n2 = B.nprim[j];
+    const int l1 = B.l[i], l2 = B.l[j];
+    int cnt = 0;
+    for (int p1 = 0; p1 < n1; ++p1)
+      for (int p2 = 0; p2 < n2; ++p2) {
+        const double a1 = B.alpha[B.off[i] + p1], a2 = B.alpha[B.off[j] + p2];
+        const double gamma = a1 + a2, oog = 1.0 / gamma;
+        const double rho = a1 * a2 * oog;
+        const double mr = -rho * AB2;
+        double ln_scr = mr + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
+        if (fac) {
+          const double f = fac_t ? (*fac)[(size_t)p2 * n1 + p1] : (*fac)[(size_t)p1 * n2 + p2];
+          ln_scr = std::log((double)(n1 * n2) * f) + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
+          if (ln_scr < ln_prec) continue;
+        }
+        if (method == L2G_SCREEN_ORIGINAL && ln_scr < ln_prec) continue;
+        double Pp[3];
+        for (int k = 0; k < 3; ++k) Pp[k] = (AB2 == 0.0) ? A[k] : (a1 * A[k] + a2 * Bo[k]) * oog;
+        double nonsph = 0;
+        if (method == L2G_SCREEN_CONSERVATIVE) {
+          double mpa = 0, mpb = 0;
+          for (int k = 0; k < 3; ++k) {
+            mpa = std::max(mpa, std::fabs(Pp[k] - A[k]));
+            mpb = std::max(mpb, std::fabs(Pp[k] - Bo[k]));
+          }
+          nonsph = std::max(std::pow(mpa, l1) * std::pow(mpb, l2),
+                            factorial(l1) * factorial(l2) * std::pow(oog, l1 + l2));
+          ln_scr += 1.777485947591722872387900 + std::log(oog) + std::log(std::max(nonsph, 1.0)) +
+                    std::log((double)(n1 * n2));
+          if (ln_scr < ln_prec) continue;
+        }
+        const double K = 5.9149671727956128778 * std::exp(mr) * oog;
+        const double c = B.coeff[B.off[i] + p1] * B.coeff[B.off[j] + p2];
+        P.push_back(double4{Pp[0], Pp[1], Pp[2], gamma});
+        PA.push_back(double4{Pp[0] - A[0], Pp[1] - A[1], Pp[2] - A[2], K * c});
+        aux.push_back(double2{ln_scr, 0.5 * oog});
+        ns.push_back(nonsph);
+        ++cnt;
+      }
+    np.push_back(cnt);
+    return pid;
+  }
+};
+
+}  // namespace l2g_impl

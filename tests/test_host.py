@@ -50,6 +50,18 @@ def test_init_without_device_fails_loudly():
     assert b"device" in la.lib().l2g_last_error()
 
 
+def test_server_without_device_fails_loudly():
+    import ctypes
+    h = ctypes.c_void_p()
+    cfg = la.ServerConfig()
+    rc = la.lib().l2g_server_create(ctypes.byref(cfg), ctypes.byref(h))
+    if rc == 0:
+        la.lib().l2g_server_destroy(h)
+        pytest.skip("a GPU is visible here")
+    assert rc == la.L2G_E_NODEVICE
+    assert not h.value
+
+
 def test_renorm_and_sph_vs_oracle():
     for l in range(5):
         a = [0.11, 1.7, 30.0]
