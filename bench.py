@@ -14,7 +14,11 @@ Prints one JSON line.  `roofline` is the (dd|dd) kernel's model FLOP rate
 (libint_amd/flop_model.py) over its HIP-event duration on the library's
 stream, against the MI355X FP64 peak; `cpu_baseline` times the CPU oracle
 (a restatement of libint's Engine::compute2) on a seeded sample of the same
-quartets with every host thread.
+quartets with every host thread.  Two more legs follow the ERI step (skip
+with --no-fock): `fock`, a J/K build (compute_2body_fock on the GPU) of a
+smaller molecule of the same recipe in FP64 and in mixed precision, and
+`serve`, a Poisson stream of Fock-build requests through the request server
+with its p50/p99 latency (`p99_fock_build_ms`).
 """
 from __future__ import annotations
 
@@ -83,6 +87,98 @@ def cpu_baseline(B, quartets, seed=5, target_s=10.0):
                        f"{dt:.1f} s with {nthreads} OpenMP threads, one Engine restatement per thread")
 
 
+def measured_traffic(nquartets):
+    """HBM bytes per launch of the (dd|dd) kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/eri_2222_workload2_traffic.md,
+    calibrated against a 1 GiB copy); None when the profile was taken on a
+    different quartet list."""
+    path = os.path.join(HERE, "profiles", "eri_2222_workload2_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 6 and f[0] == "traffic_bytes_per_launch" and f[3] == "quartets":
+                return float(f[2]) if int(f[5]) == nquartets else None
+    except OSError:
+        pass
+    return None
+
+
+def density(nbf, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.normal(0.0, 0.1, (nbf, max(1, nbf // 5)))
+    return C @ C.T
+
+
+def fock_leg(la, molecule, ctx, natoms, reps):
+    """J/K build (compute_2body_fock on the GPU) for a molecule of the same
+    recipe, FP64 and mixed (FP32 where K12 K34 < 1e-6 and S12 S34 < 16e-6):
+    median wall time per build, kernel time, and the largest difference of
+    the mixed G from the FP64 G."""
+    mol = molecule.make_molecule(2, natoms=natoms)
+    B = la.Basis(ctx, mol.basis)
+    fb = la.FockBuilder(B)
+    D = density(mol.nbf, 12)
+    out = {"workload": f"{natoms} atoms of the same recipe, nbf {mol.nbf}, D = C C^T, C ~ N(0, 0.1), "
+                       f"precision 1e-12, SchwarzInf"}
+    G = {}
+    for name, thr in (("fp64", 0.0), ("mixed", 1e-6)):
+        fb.set_fp32_threshold(thr)
+        G[name], _, _ = fb.build(D, 1e-12)  # warm-up
+        wall, kern = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            G[name], _, _ = fb.build(D, 1e-12)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            kern.append(fb.stats()["kernel_ms"])
+        st = fb.stats()
+        cls = sorted(fb.class_stats(), key=lambda c: -c["ms"])
+        out[name] = {"build_ms": float(np.median(wall)), "kernel_ms": float(np.median(kern)),
+                     "quartets": st["nquartets"], "quartets_fp32": st["nquartets_fp32"],
+                     "primitive_quartets": st["nprim"],
+                     "top_classes": [f"{c['cls']}{'/fp32' if c['fp32'] else ''}: {c['ms']:.1f} ms, "
+                                     f"{c['nquartets']} quartets, {c['nprim']} primitive quartets"
+                                     for c in cls[:6]]}
+        log(f"fock {name}: build {out[name]['build_ms']:.1f} ms, kernels {out[name]['kernel_ms']:.1f} ms")
+    out["mixed_speedup_kernel"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
+    out["mixed_speedup_build"] = out["fp64"]["build_ms"] / out["mixed"]["build_ms"]
+    out["mixed_max_abs_dG"] = float(np.abs(G["mixed"] - G["fp64"]).max())
+    del fb, B
+    return out
+
+
+def serve_leg(la, molecule, nreq, rate_hz, seed=5000):
+    """Fock-build requests for molecules of 2-12 atoms (log-uniform; the two
+    recipes alternate) arriving as a Poisson stream, served with continuous
+    batching; latency = submit to result."""
+    rng = np.random.default_rng(seed)
+    mols = []
+    for k in range(nreq):
+        n = int(round(np.exp(rng.uniform(np.log(2), np.log(12)))))
+        m = molecule.make_molecule(1 + k % 2, natoms=n)
+        mols.append((m, density(m.nbf, seed + k)))
+    gaps = rng.exponential(1.0 / rate_hz, nreq)
+    srv = la.Server(0)
+    try:
+        srv.result(srv.submit(mols[0][0].basis, mols[0][1], 1e-12))  # warm-up
+        t0 = time.perf_counter()
+        ids = []
+        for (m, D), g in zip(mols, gaps):
+            time.sleep(g)
+            ids.append(srv.submit(m.basis, D, 1e-12))
+        lat = np.array([srv.result(i)[3][1] for i in ids])
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    out = {"workload": f"{nreq} requests, Poisson {rate_hz:g}/s, 2-12 atoms of the two recipes (nbf "
+                       f"{min(m.nbf for m, _ in mols)}-{max(m.nbf for m, _ in mols)}), precision 1e-12",
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "requests_per_s": nreq / wall, "rounds": st["rounds"],
+           "max_requests_per_round": st["max_requests_per_round"]}
+    log(f"serve: p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms, {out['requests_per_s']:.1f} req/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,7 +186,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--max-quartets", type=int, default=0, help="cap the quartet list (0 = all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fock", action="store_true", help="skip the J/K build and serving legs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (the known-size kernel that calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--fock-atoms", type=int, default=20, help="atoms of the J/K build leg")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving leg")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.empty(2 ** 27, dtype=torch.float64, device="cuda")
+        x.fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)  # 1 GiB read + 1 GiB written
+        torch.cuda.synchronize()
+        print("copied 1 GiB", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
@@ -163,7 +275,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(len(q)),
             "kernel": "l2g_eri_2222",
             "kernel_ms": kms,
             "model_flops_per_launch": flops,
@@ -172,6 +284,11 @@ def main():
         },
         "setup_s": t_setup,
     }
+    del plan
+    if not args.no_fock:
+        out["fock"] = fock_leg(la, molecule, ctx, args.fock_atoms, reps=2)
+        out["serve"] = serve_leg(la, molecule, args.serve_requests, rate_hz=20.0)
+        out["p99_fock_build_ms"] = out["serve"]["p99_ms"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol.basis, q)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

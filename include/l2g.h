@@ -216,6 +216,12 @@ int l2g_fock_set_fp32_threshold(l2g_fock fock, double threshold);
 /* Number of quartets the last build evaluated in FP32. */
 int l2g_fock_stats_mixed(l2g_fock fock, int64_t* nquartets_fp32);
 
+/* Per kernel class of the last build, as l2g_eri_plan_stats: class code
+ * la*1000+lb*100+lc*10+ld (+10000 for FP32 kernels), quartets, primitive
+ * quartets and the class's kernel time in ms (HIP events). */
+int l2g_fock_class_stats(l2g_fock fock, int cap, int* nclasses, int* class_code, int64_t* nquartets,
+                         int64_t* nprim_quartets, float* ms);
+
 int l2g_fock_destroy(l2g_fock fock);
 
 /* ---- Fock-build request server ------------------------------------------ */
@@ -227,9 +233,10 @@ int l2g_fock_destroy(l2g_fock fock);
  * compute_schwarz_ints + compute_2body_fock
  * (tests/hartree-fock/hartree-fock++.cc:1231-1772).
  *
- * A background admission thread takes submitted requests in order, builds
- * their pair list, Schwarz factors and screened quartet list, and admits
- * each one as soon as a slot and device memory are free; the request's
+ * Background admission workers build the pair list, Schwarz factors and
+ * screened quartet list of several submitted requests in parallel, and
+ * admit them in submission order, each as soon as a slot and device memory
+ * are free; the request's
  * shell-pair data, density and J/K accumulators then stay resident on the
  * device until it completes.  A launch thread runs rounds of the Fock
  * kernels: each round takes up to round_quartets quartets, split across all
@@ -275,7 +282,8 @@ int l2g_request_wait(l2g_server server, int64_t id);
 int l2g_request_result(l2g_server server, int64_t id, double* J, double* K, double* G, double* ms);
 
 /* Counters: launch rounds run, the most requests that shared one round,
- * quartets evaluated. */
+ * work items launched (a quartet, or one primitive range of a deep quartet
+ * that was split across lanes). */
 int l2g_server_stats(l2g_server server, int64_t* rounds, int64_t* max_requests_per_round, int64_t* nquartets);
 
 /* Waits for every submitted request to finish, then stops the threads and

@@ -78,6 +78,7 @@ ABI = {
     "l2g_fock_set_fp32_threshold": (ctypes.c_int, [_vp, ctypes.c_double]),
     "l2g_fock_stats_mixed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_precision_bounds": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, _f64p, _f64p]),
+    "l2g_fock_class_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
     "l2g_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "l2g_request_submit": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                           ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
@@ -428,6 +429,21 @@ class FockBuilder:
         _check(lib().l2g_fock_stats_mixed(self._h, ctypes.byref(d)))
         return dict(nquartets=a.value, nprim=b.value, kernel_ms=c.value, nquartets_fp32=d.value)
 
+    def class_stats(self):
+        """Per kernel class of the last build: class, fp32, quartets,
+        primitive quartets and kernel ms."""
+        cap = 256
+        n = ctypes.c_int()
+        code = np.zeros(cap, np.int32)
+        nq = np.zeros(cap, np.int64)
+        npq = np.zeros(cap, np.int64)
+        ms = np.zeros(cap, np.float32)
+        _check(lib().l2g_fock_class_stats(self._h, cap, ctypes.byref(n), code.ctypes.data, nq.ctypes.data,
+                                          npq.ctypes.data, ms.ctypes.data))
+        k = n.value
+        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c >= 10000), nquartets=int(a), nprim=int(b), ms=float(t))
+                for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
+
     def _release(self):
         if self._h:
             lib().l2g_fock_destroy(self._h)
@@ -540,7 +556,7 @@ class Server:
     def stats(self):
         r, m, q = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _check(lib().l2g_server_stats(self._h, ctypes.byref(r), ctypes.byref(m), ctypes.byref(q)))
-        return dict(rounds=r.value, max_requests_per_round=m.value, nquartets=q.value)
+        return dict(rounds=r.value, max_requests_per_round=m.value, work_items=q.value)
 
     def close(self) -> None:
         if self._h:

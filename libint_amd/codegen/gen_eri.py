@@ -566,19 +566,18 @@ def emit_prim_loop(E, plan, graph, gi, accs):
     pair data of the next primitive are loaded at the top of an iteration and
     its Boys-table row halfway through the recursion, so both memory
     latencies overlap the FP64 work of the current primitive."""
-    E("const int npq = npb * npk;")
-    E.block("if (npq > 0)")
-    E("int ib = 0, ik = 0;")
-    E("double4 Pg = A.pp_P[pb0], PAk = A.pp_PA[pb0];")
-    E("double2 xb = A.pp_aux[pb0];")
-    E("double4 Qg = A.pp_P[pk0], QCk = A.pp_PA[pk0];")
-    E("double2 xk = A.pp_aux[pk0];")
+    E.block("if (pend > pbeg)")
+    E("int ib = pbeg / npk, ik = pbeg - ib * npk;")
+    E("double4 Pg = A.pp_P[pb0 + ib], PAk = A.pp_PA[pb0 + ib];")
+    E("double2 xb = A.pp_aux[pb0 + ib];")
+    E("double4 Qg = A.pp_P[pk0 + ik], QCk = A.pp_PA[pk0 + ik];")
+    E("double2 xk = A.pp_aux[pk0 + ik];")
     E("double pfac;")
     E("L2gBoysArg bq;")
     E.block("")
-    _emit_stage2(E, "", "0", "0", "" if gi == 0 else None)
+    _emit_stage2(E, "", "ib", "ik", "" if gi == 0 else None)
     E.end()
-    E.block("for (int p = 0; p < npq; ++p)")
+    E.block("for (int p = pbeg; p < pend; ++p)")
     E("int ikn = ik + 1, ibn = ib;")
     E("if (ikn == npk) { ikn = 0; ++ibn; }")
     E("if (ibn == npb) ibn = 0;  // past the end: any valid pair, result unused")
@@ -616,7 +615,7 @@ def emit_prim_loop(E, plan, graph, gi, accs):
 
     def mid():
         E.block("")
-        _emit_stage2(E, "n", "ibn", "ikn", "p + 1 < npq && " if gi == 0 else None)
+        _emit_stage2(E, "n", "ibn", "ikn", "p + 1 < pend && " if gi == 0 else None)
         E.end()
 
     emit_vrr_group(E, plan, graph, gi, accs, mid=mid)
@@ -665,6 +664,14 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     E("const double CDx = A.pair_AB[3 * kp], CDy = A.pair_AB[3 * kp + 1], CDz = A.pair_AB[3 * kp + 2];")
     E(f"const double* __restrict__ boysL = A.boys + (size_t){plan.L} * L2G_BOYS_NINT * 8;")
     E("const double npbk = (double)npb * (double)npk;")
+    E("// primitive quartets [pbeg, pend) of this work item (p = ib * npk + ik)")
+    E("int pbeg = 0, pend = npb * npk;")
+    if mode == 'fock':
+        E.block("if (A.q_prange)")
+        E("const int2 pr = A.q_prange[qi];")
+        E("pbeg = pr.x;")
+        E("pend = pr.y;")
+        E.end()
     if mode == 'fock':
         E("const double lnprec = A.q_lnprec[qi];")
         E("const double qprec = A.method == L2G_SCREEN_ORIGINAL || A.method == L2G_SCREEN_CONSERVATIVE ? exp(lnprec) : 0.0;")

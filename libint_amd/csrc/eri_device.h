@@ -61,6 +61,7 @@ struct L2gEriArgs {
   double* __restrict__ kg;               // unsymmetrised exchange accumulator arena
   const L2gMatDesc* __restrict__ mats;   // matrix block of each request
   const int* __restrict__ q_req;         // request of each work item (null: request 0)
+  const int2* __restrict__ q_prange;     // primitive quartets [x, y) of each work item (null: all)
   // Schwarz kernels: optional sqrt(max |[e0|e0]|) of the pre-HRR diagonal
   double* __restrict__ out2;
 };

@@ -224,6 +224,7 @@ struct FockItems {
   std::vector<int> bra, ket, meta, req;
   std::vector<double> lnp, scale;
   std::vector<int4> bf;
+  std::vector<int2> pr;  // primitive-quartet range [x, y)
   size_t size() const { return bra.size(); }
   void append(const FockItems& o, size_t b = 0, size_t e = SIZE_MAX) {
     e = std::min(e, o.size());
@@ -234,10 +235,11 @@ struct FockItems {
     lnp.insert(lnp.end(), o.lnp.begin() + b, o.lnp.begin() + e);
     scale.insert(scale.end(), o.scale.begin() + b, o.scale.begin() + e);
     bf.insert(bf.end(), o.bf.begin() + b, o.bf.begin() + e);
+    pr.insert(pr.end(), o.pr.begin() + b, o.pr.begin() + e);
     if (!o.req.empty()) req.insert(req.end(), o.req.begin() + b, o.req.begin() + e);
   }
   void clear() {
-    bra.clear(), ket.clear(), meta.clear(), req.clear(), lnp.clear(), scale.clear(), bf.clear();
+    bra.clear(), ket.clear(), meta.clear(), req.clear(), lnp.clear(), scale.clear(), bf.clear(), pr.clear();
   }
 };
 
@@ -252,16 +254,15 @@ struct FockWork {
   int64_t nq = 0, nq32 = 0, nprim = 0;
 };
 
-// Reorders the items of one class by decreasing contraction depth
-// (primitive quartets), so that neighbouring lanes run equal-length
-// primitive loops.  Stable counting sort on the weight.
-void sort_by_depth(FockItems& it, const std::vector<int>& np) {
+// Reorders the items of one class by decreasing primitive-loop length, so
+// that neighbouring lanes run equal-length loops.  Stable counting sort.
+void sort_by_depth(FockItems& it) {
   const size_t m = it.size();
   if (m < 2) return;
   int wmax = 0;
   std::vector<int> w(m);
   for (size_t i = 0; i < m; ++i) {
-    w[i] = np[it.bra[i]] * np[it.ket[i]];
+    w[i] = it.pr[i].y - it.pr[i].x;
     wmax = std::max(wmax, w[i]);
   }
   std::vector<size_t> cnt((size_t)wmax + 2, 0);
@@ -270,13 +271,49 @@ void sort_by_depth(FockItems& it, const std::vector<int>& np) {
   std::vector<size_t> ord(m);
   for (size_t i = 0; i < m; ++i) ord[cnt[(size_t)(wmax - w[i])]++] = i;
   FockItems s;
-  s.bra.resize(m), s.ket.resize(m), s.meta.resize(m), s.lnp.resize(m), s.scale.resize(m), s.bf.resize(m);
+  s.bra.resize(m), s.ket.resize(m), s.meta.resize(m), s.lnp.resize(m), s.scale.resize(m), s.bf.resize(m),
+      s.pr.resize(m);
   if (!it.req.empty()) s.req.resize(m);
   for (size_t k = 0; k < m; ++k) {
     const size_t i = ord[k];
     s.bra[k] = it.bra[i], s.ket[k] = it.ket[i], s.meta[k] = it.meta[i];
-    s.lnp[k] = it.lnp[i], s.scale[k] = it.scale[i], s.bf[k] = it.bf[i];
+    s.lnp[k] = it.lnp[i], s.scale[k] = it.scale[i], s.bf[k] = it.bf[i], s.pr[k] = it.pr[i];
     if (!it.req.empty()) s.req[k] = it.req[i];
+  }
+  it = std::move(s);
+}
+
+// Classes with fewer items than the chip has lanes to fill are split by
+// primitive-quartet range: an item whose loop is longer than the class's
+// chunk (about 1.4e5 recursion-node evaluations) becomes several items on
+// several lanes.  The digest is linear in the integrals, so each range is
+// digested on its own; the cost is one more HRR, transform and digest per
+// range.  Without this, one (dd|dd) quartet with K = 6 on every shell runs
+// ~1300 primitive quartets x 19 groups on one lane, tens of ms.
+void split_deep(FockItems& it, int cls) {
+  const size_t m = it.size();
+  if (m == 0 || m >= (size_t)(1 << 17)) return;
+  const int nodes = std::max(1, fock_info(cls).nodes);
+  const int chunk = std::max(8, 140000 / nodes);
+  size_t extra = 0;
+  for (size_t i = 0; i < m; ++i) {
+    const int len = it.pr[i].y - it.pr[i].x;
+    if (len > chunk) extra += (size_t)((len + chunk - 1) / chunk) - 1;
+  }
+  if (extra == 0) return;
+  FockItems s;
+  s.bra.reserve(m + extra), s.ket.reserve(m + extra), s.meta.reserve(m + extra), s.lnp.reserve(m + extra),
+      s.scale.reserve(m + extra), s.bf.reserve(m + extra), s.pr.reserve(m + extra);
+  if (!it.req.empty()) s.req.reserve(m + extra);
+  for (size_t i = 0; i < m; ++i) {
+    const int b = it.pr[i].x, e = it.pr[i].y, len = e - b;
+    const int nch = len > chunk ? (len + chunk - 1) / chunk : 1;
+    for (int k = 0; k < nch; ++k) {
+      s.bra.push_back(it.bra[i]), s.ket.push_back(it.ket[i]), s.meta.push_back(it.meta[i]);
+      s.lnp.push_back(it.lnp[i]), s.scale.push_back(it.scale[i]), s.bf.push_back(it.bf[i]);
+      if (!it.req.empty()) s.req.push_back(it.req[i]);
+      s.pr.push_back(int2{b + (int)((int64_t)len * k / nch), b + (int)((int64_t)len * (k + 1) / nch)});
+    }
   }
   it = std::move(s);
 }
@@ -355,6 +392,7 @@ void fock_enumerate(const FockSetup& f, const double* D, double precision, doubl
             it.lnp.push_back(qprec > 0 ? std::log(qprec) : std::numeric_limits<double>::lowest());
             it.scale.push_back((double)deg);
             it.bf.push_back(int4{f.bf1st[b1], f.bf1st[b2], f.bf1st[k1], f.bf1st[k2]});
+            it.pr.push_back(int2{0, f.np[bp] * f.np[kp]});
             ++w.nq;
             w.nprim += (int64_t)f.np[bp] * f.np[kp];
           }
@@ -376,7 +414,8 @@ void fock_enumerate(const FockSetup& f, const double* D, double precision, doubl
     o.bra.reserve(tot), o.ket.reserve(tot), o.meta.reserve(tot), o.lnp.reserve(tot), o.scale.reserve(tot),
         o.bf.reserve(tot);
     for (auto& w : part) o.append(w.cls[c]);
-    sort_by_depth(o, f.np);
+    split_deep(o, c);
+    sort_by_depth(o);
   });
 }
 
@@ -424,7 +463,7 @@ L2gEriArgs fock_args(l2g_context_s* ctx, int method, const int* pair_first, cons
 // ranges; the arrays are uploaded from host vectors that must stay alive
 // until the stream has passed the copies.
 struct ItemsDev {
-  DevBuf bra, ket, meta, req, lnp, scale, bf;
+  DevBuf bra, ket, meta, req, lnp, scale, bf, pr;
   std::vector<int64_t> first, count;  // per class
   FockItems host;                      // staging (kept until the stream is synchronised)
   void upload_from(const std::vector<FockItems>& cls, hipStream_t st) {
@@ -442,16 +481,21 @@ struct ItemsDev {
     upload(lnp, host.lnp, st);
     upload(scale, host.scale, st);
     upload(bf, host.bf, st);
+    upload(pr, host.pr, st);
     if (!host.req.empty()) upload(req, host.req, st);
   }
 };
 
 // Launches every non-empty class of `items` with the common arguments A;
 // grid and scratch as sized by the caller.
-void fock_launch_all(L2gEriArgs A, const ItemsDev& items, bool with_req, int grid, int block, hipStream_t st) {
+// With `ev` (2 events per class), records the class's launch between
+// ev[2c] and ev[2c+1].
+void fock_launch_all(L2gEriArgs A, const ItemsDev& items, bool with_req, int grid, int block, hipStream_t st,
+                     hipEvent_t* ev = nullptr) {
   for (size_t c = 0; c < items.count.size(); ++c) {
     const int64_t m = items.count[c], o = items.first[c];
     if (m == 0) continue;
+    if (ev) HIPCHK(hipEventRecord(ev[2 * c], st));
     A.bra_pair = items.bra.as<int>() + o;
     A.ket_pair = items.ket.as<int>() + o;
     A.meta = items.meta.as<int>() + o;
@@ -459,9 +503,11 @@ void fock_launch_all(L2gEriArgs A, const ItemsDev& items, bool with_req, int gri
     A.q_scale = items.scale.as<double>() + o;
     A.q_bf = items.bf.as<int4>() + o;
     A.q_req = with_req ? items.req.as<int>() + o : nullptr;
+    A.q_prange = items.pr.as<int2>() + o;
     A.n = (int)m;
     fock_info((int)c).launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
     HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[2 * c + 1], st));
   }
 }
 
@@ -475,6 +521,12 @@ struct l2g_fock_s {
   PairDev pairs;  // resident shell-pair data
   DevBuf dens, jg, kg, mats, ybuf;
   ItemsDev items;
+  std::vector<hipEvent_t> ev;     // per class: start, end of the last build
+  std::vector<int64_t> cls_nq, cls_nprim;
+  ~l2g_fock_s() {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 extern "C" {
@@ -547,11 +599,23 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
                              p.PA.as<double4>(), p.aux.as<double2>(), p.ns.as<double>(), f->dens.as<double>(),
                              f->jg.as<double>(), f->kg.as<double>(), f->mats.as<L2gMatDesc>(), f->ybuf.as<double>(),
                              ystride);
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    const int ncls = nfock_classes();
+    if (f->ev.empty()) {
+      f->ev.assign(2 * ncls + 2, nullptr);
+      for (auto& e : f->ev) HIPCHK(hipEventCreate(&e));
+    }
+    f->cls_nq.assign(ncls, 0);
+    f->cls_nprim.assign(ncls, 0);
+    for (int c = 0; c < ncls; ++c) {
+      const FockItems& it = w.cls[c];
+      for (size_t i = 0; i < it.size(); ++i) {
+        f->cls_nq[c] += it.pr[i].x == 0;  // one range of each quartet starts at 0
+        f->cls_nprim[c] += it.pr[i].y - it.pr[i].x;
+      }
+    }
+    hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
     HIPCHK(hipEventRecord(e0, st));
-    fock_launch_all(A, f->items, false, grid, block, st);
+    fock_launch_all(A, f->items, false, grid, block, st, f->ev.data());
     HIPCHK(hipEventRecord(e1, st));
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);
@@ -559,8 +623,6 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     f->last_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (J && NN) HIPCHK(hipMemcpy(J, f->jg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (K && NN) HIPCHK(hipMemcpy(K, f->kg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (G && NN) HIPCHK(hipMemcpy(G, f->dens.p, NN * sizeof(double), hipMemcpyDeviceToHost));
@@ -588,6 +650,31 @@ int l2g_fock_stats(l2g_fock f, int64_t* nquartets, int64_t* nprim_quartets, floa
     if (nquartets) *nquartets = f->last_nq;
     if (nprim_quartets) *nprim_quartets = f->last_nprim;
     if (kernel_ms) *kernel_ms = f->last_ms;
+  });
+}
+
+int l2g_fock_class_stats(l2g_fock f, int cap, int* nclasses, int* class_code, int64_t* nquartets,
+                         int64_t* nprim_quartets, float* ms) {
+  return guarded([&] {
+    if (!f || !nclasses || cap < 0) throw std::invalid_argument("l2g_fock_class_stats: bad arguments");
+    int k = 0;
+    for (int c = 0; c < (int)f->cls_nq.size(); ++c) {
+      if (f->cls_nq[c] == 0) continue;
+      if (k < cap) {
+        const L2gClassInfo& ci = fock_info(c);
+        if (class_code)
+          class_code[k] = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld + (c >= l2g_gen_nfock ? 10000 : 0);
+        if (nquartets) nquartets[k] = f->cls_nq[c];
+        if (nprim_quartets) nprim_quartets[k] = f->cls_nprim[c];
+        if (ms) {
+          float t = 0;
+          (void)hipEventElapsedTime(&t, f->ev[2 * c], f->ev[2 * c + 1]);
+          ms[k] = t;
+        }
+      }
+      ++k;
+    }
+    *nclasses = k;
   });
 }
 
@@ -651,6 +738,7 @@ using Clock = std::chrono::steady_clock;
 struct ServerRequest {
   int64_t id = 0;
   int state = L2G_REQ_QUEUED;
+  bool claimed = false;  // an admission worker is preparing it
   int error = 0;
   std::string message;
   // inputs (copied at submission)
@@ -667,7 +755,7 @@ struct ServerRequest {
   int64_t npair = 0, nprimpair = 0, nmat = 0;
   // launch progress (launch thread only)
   std::vector<size_t> cursor;
-  int64_t left = 0;
+  int64_t left = 0;  // work items not yet launched
   hipEvent_t done_ev = nullptr;
   double* result = nullptr;  // pinned host: J, K, G
   Clock::time_point t_submit, t_admit, t_done;
@@ -701,8 +789,10 @@ struct l2g_server_s {
   std::vector<ServerRequest*> active;      // admitted, quartets left to launch
   std::vector<ServerRequest*> finishing;   // all launched, finish step queued
   int64_t next_id = 1, inflight = 0;
+  int uploading = 0;  // reserved requests between reservation and activation
   bool stop = false;
-  std::thread admit_thr, exec_thr;
+  std::vector<std::thread> admit_thr;  // admission workers
+  std::thread exec_thr;
   int64_t rounds = 0, max_per_round = 0, nq = 0;
 };
 
@@ -760,7 +850,8 @@ void prepare(l2g_server_s* S, ServerRequest* r) {
   r->npair = (int64_t)r->setup.first.size();
   r->nprimpair = (int64_t)r->setup.P.size();
   r->nmat = (int64_t)r->nbf * r->nbf;
-  r->left = r->work.nq;
+  r->left = 0;  // work items: deep quartets may be split into several
+  for (const auto& it : r->work.cls) r->left += (int64_t)it.size();
   r->cursor.assign(r->work.cls.size(), 0);
 }
 
@@ -795,28 +886,37 @@ void upload_request(l2g_server_s* S, ServerRequest* r) {
   HIPCHK(hipStreamSynchronize(st));
 }
 
+// Admission worker: prepares the oldest unclaimed request (several workers
+// prepare in parallel), then admits requests strictly in submission order,
+// each as soon as a slot and arena space are free.
 void admit_loop(l2g_server_s* S) {
   (void)hipSetDevice(S->cfg.device);
   for (;;) {
     ServerRequest* r = nullptr;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv.wait(lk, [&] { return S->stop || !S->pending.empty(); });
-      if (S->pending.empty()) return;
-      r = S->pending.front();
+      auto unclaimed = [&] {
+        for (ServerRequest* q : S->pending)
+          if (!q->claimed) return q;
+        return (ServerRequest*)nullptr;
+      };
+      S->cv.wait(lk, [&] { return (S->stop && S->pending.empty()) || unclaimed(); });
+      r = unclaimed();
+      if (!r) return;
+      r->claimed = true;
     }
     int rc = guarded([&] { prepare(S, r); });
     bool ok = rc == 0;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->pending.pop_front();
+      S->cv.wait(lk, [&] { return S->pending.front() == r; });  // admission in submission order
       if (!ok) {
         fail_locked(S, r, rc, g_err);
       } else {
         // wait for a slot and arena space; a request that cannot fit an
         // empty server fails instead of waiting forever
         while (!reserve_locked(S, r)) {
-          if (S->active.empty() && S->finishing.empty()) {
+          if (S->active.empty() && S->finishing.empty() && S->uploading == 0) {
             ok = false;
             fail_locked(S, r, L2G_E_OOM, "request does not fit the server's device memory or slots");
             break;
@@ -824,6 +924,9 @@ void admit_loop(l2g_server_s* S) {
           S->cv.wait(lk);
         }
       }
+      S->pending.pop_front();
+      if (ok) ++S->uploading;
+      S->cv.notify_all();
     }
     if (!ok) {
       if (r->basis) l2g_basis_destroy(r->basis), r->basis = nullptr;
@@ -831,6 +934,7 @@ void admit_loop(l2g_server_s* S) {
     }
     rc = guarded([&] { upload_request(S, r); });
     std::unique_lock<std::mutex> lk(S->mu);
+    --S->uploading;
     if (rc) {
       release_locked(S, r);
       fail_locked(S, r, rc, g_err);
@@ -1035,7 +1139,7 @@ int l2g_server_create(const l2g_server_config* config, l2g_server* out) {
     for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&S->rev[k], hipEventDisableTiming));
     for (int k = c.max_active - 1; k >= 0; --k) S->free_slots.push_back(k);
     l2g_server_s* p = S.get();
-    p->admit_thr = std::thread(admit_loop, p);
+    for (int k = 0; k < 4; ++k) p->admit_thr.emplace_back(admit_loop, p);
     p->exec_thr = std::thread(exec_loop, p);
     *out = S.release();
   });
@@ -1153,7 +1257,8 @@ int l2g_server_destroy(l2g_server S) {
     S->stop = true;
     S->cv.notify_all();
   }
-  if (S->admit_thr.joinable()) S->admit_thr.join();
+  for (auto& t : S->admit_thr)
+    if (t.joinable()) t.join();
   if (S->exec_thr.joinable()) S->exec_thr.join();
   (void)hipSetDevice(S->cfg.device);
   (void)hipStreamSynchronize(S->exec->stream);

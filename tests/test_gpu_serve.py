@@ -11,6 +11,7 @@ import pytest
 import libint_amd as la
 import oracle
 from libint_amd import molecule
+from tests.helpers import random_shells
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +40,7 @@ def requests():
 def test_serve_alone_equals_mix():
     reqs = requests()
     say(f"case: server alone vs mix, {len(reqs)} requests, nbf {[D.shape[0] for _, D in reqs]}")
-    srv = la.Server(0, round_quartets=4000)
+    srv = la.Server(0, round_quartets=256)
     try:
         alone = []
         for B, D in reqs:
@@ -86,22 +87,23 @@ def test_serve_memory_admission_and_failures():
     order; a request larger than the whole budget fails with L2G_E_OOM and a
     request above the compiled l fails with L2G_E_LMAX, neither disturbing
     the others."""
-    small = [molecule.make_molecule(1, natoms=3) for _ in range(3)]
-    nbf = small[0].nbf
+    # uncontracted shells on close centres: every shell pair and primitive
+    # pair is kept, so the matrix arena (D, jg, kg: 24 bytes per element,
+    # 60% of the budget) binds before the primitive-pair (88 bytes, 35%) and
+    # shell-pair (32 bytes, 5%) arenas; the budget holds 1.5 requests
+    rng = np.random.default_rng(21)
+    shells = random_shells(rng, [2, 2, 2, 1, 1, 0, 2, 1], kmax=1, box=(0.0, 1.5), expo=(0.5, 2.0))
+    arrays = la.BasisArrays.from_shells(shells)
+    nbf = int(arrays.sizes().sum())
+    npairs = arrays.nshell * (arrays.nshell + 1) // 2
+    budget = 1.5 * 24 * nbf * nbf / 0.60
+    assert 0.35 * budget / 88 >= npairs and 0.05 * budget / 32 >= npairs
+    small = [arrays] * 3
     big = molecule.make_molecule(2, natoms=12)
-    # room for 1.5 small requests in the arena that binds first: matrices
-    # (D, jg, kg: 24 bytes per element, 60% of the budget), primitive pairs
-    # (88 bytes, 35%) and shell pairs (32 bytes, 5%); pair counts bounded by
-    # all pairs
-    B0 = small[0].basis
-    npairs = B0.nshell * (B0.nshell + 1) // 2
-    kp = B0.nprim.astype(np.int64)
-    nprim_pairs = int((np.outer(kp, kp)[np.tril_indices(B0.nshell)]).sum())
-    budget = 1.5 * max(24 * nbf * nbf / 0.60, 88 * nprim_pairs / 0.35, 32 * npairs / 0.05)
-    say(f"case: server admission by memory, budget {budget / 1e6:.2f} MB")
+    say(f"case: server admission by memory, budget {budget / 1e6:.3f} MB, nbf {nbf}")
     srv = la.Server(0, hbm_bytes=budget)
     try:
-        ids = [srv.submit(m.basis, density(m.nbf, 5 + k), 1e-12) for k, m in enumerate(small)]
+        ids = [srv.submit(a, density(nbf, 5 + k), 1e-12) for k, a in enumerate(small)]
         bad = srv.submit(big.basis, density(big.nbf, 1), 1e-12)
         res = [srv.result(i) for i in ids]
         with pytest.raises(la.L2gError) as e:
@@ -115,7 +117,7 @@ def test_serve_memory_admission_and_failures():
             srv.wait(srv.submit(fb, np.eye(nf), 1e-12))
         assert e2.value.code == la.L2G_E_LMAX
         # the server keeps working after failures
-        G, _, _, _ = srv.result(srv.submit(small[0].basis, density(nbf, 5), 1e-12))
+        G, _, _, _ = srv.result(srv.submit(small[0], density(nbf, 5), 1e-12))
         assert np.abs(G - res[0][0]).max() <= ALONE_TOL
     finally:
         srv.close()
