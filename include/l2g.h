@@ -66,6 +66,14 @@ int l2g_max_l(void);
  * (include/libint2/initialize.h:76-90) for this path. */
 int l2g_init(int device, l2g_context* ctx);
 
+/* Routes every class through the generic kernel (l2g_generic.hip), which
+ * otherwise serves only the classes with an f shell; on = 0 restores the
+ * generated register-resident kernels for s, p and d classes.  A testing
+ * aid with no counterpart in the reference: it lets the generic kernel be
+ * checked on every class.  Affects plans, Schwarz factors and Fock builds
+ * created afterwards. */
+int l2g_set_generic_kernels(l2g_context ctx, int on);
+
 /* Releases the context (libint2::finalize, initialize.h:105-113).  Returns
  * L2G_E_STATE, and keeps the context, while any basis, plan or Fock builder
  * created on it has not been destroyed. */

@@ -79,6 +79,7 @@ ABI = {
     "l2g_fock_stats_mixed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_precision_bounds": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, _f64p, _f64p]),
     "l2g_fock_class_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
+    "l2g_set_generic_kernels": (ctypes.c_int, [_vp, ctypes.c_int]),
     "l2g_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_vp)]),
     "l2g_request_submit": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                           ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
@@ -244,6 +245,10 @@ class Context:
     def set_solid_harmonics_ordering(self, ordering: int) -> None:
         _check(lib().l2g_set_solid_harmonics_ordering(self._h, ordering))
 
+    def set_generic_kernels(self, on: bool) -> None:
+        """Route every class through the generic kernel (a testing aid)."""
+        _check(lib().l2g_set_generic_kernels(self._h, int(on)))
+
     def synchronize(self) -> None:
         _check(lib().l2g_synchronize(self._h))
 
@@ -349,7 +354,8 @@ class EriPlan:
         _check(lib().l2g_eri_plan_stats(self._h, cap, ctypes.byref(n), code.ctypes.data, nq.ctypes.data,
                                         npq.ctypes.data, ms.ctypes.data))
         k = n.value
-        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c >= 10000), nquartets=int(a), nprim=int(b), ms=float(t))
+        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c // 10000 == 1), generic=bool(c // 10000 == 2),
+                     nquartets=int(a), nprim=int(b), ms=float(t))
                 for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
 
     def _release(self):
@@ -441,7 +447,8 @@ class FockBuilder:
         _check(lib().l2g_fock_class_stats(self._h, cap, ctypes.byref(n), code.ctypes.data, nq.ctypes.data,
                                           npq.ctypes.data, ms.ctypes.data))
         k = n.value
-        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c >= 10000), nquartets=int(a), nprim=int(b), ms=float(t))
+        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c // 10000 == 1), generic=bool(c // 10000 == 2),
+                     nquartets=int(a), nprim=int(b), ms=float(t))
                 for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
 
     def _release(self):

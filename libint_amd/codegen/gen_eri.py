@@ -876,7 +876,7 @@ def class_waves(plan, budget, waves):
 
 
 def generate(outdir, lmax=2, budget=150, nfiles=8, waves=0, only=None, budgets=None,
-             modes=('eri', 'fock', 'schwarz', 'fp32')):
+             modes=('eri', 'fock', 'schwarz', 'fp32'), generic_lmax=3):
     """Write the kernels and the dispatch tables.  `budgets` maps a class name
     ('2222') to its own (budget, waves)."""
     os.makedirs(outdir, exist_ok=True)
@@ -964,6 +964,15 @@ def generate(outdir, lmax=2, budget=150, nfiles=8, waves=0, only=None, budgets=N
         with open(fn, 'w') as f:
             f.write(new)
     written.append(fn)
+    # programs of the generic kernel: every canonical class up to generic_lmax
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import gen_generic
+    fn = os.path.join(outdir, "generic_table.cpp")
+    new = gen_generic.emit_tables(canonical_classes(generic_lmax), generic_lmax)
+    if not (os.path.exists(fn) and open(fn).read() == new):
+        with open(fn, 'w') as f:
+            f.write(new)
+    written.append(fn)
     return written, plans
 
 
@@ -978,6 +987,7 @@ if __name__ == '__main__':
     ap.add_argument("--class-cfg", default="", help="per class budget:waves, e.g. 2222=64:2,2122=80:2")
     ap.add_argument("--nfiles", type=int, default=8)
     ap.add_argument("--modes", default="eri,fock,schwarz,fp32")
+    ap.add_argument("--generic-lmax", type=int, default=3, help="highest l of the generic kernel's classes")
     a = ap.parse_args()
     per = {}
     for item in filter(None, a.class_cfg.split(",")):
@@ -985,6 +995,7 @@ if __name__ == '__main__':
         b, w = v.split(":")
         per[k] = (int(b), int(w))
     files, plans = generate(a.outdir, a.lmax, a.budget, a.nfiles, a.waves,
-                            set(filter(None, a.only.split(","))) or None, per, tuple(a.modes.split(",")))
+                            set(filter(None, a.only.split(","))) or None, per, tuple(a.modes.split(",")),
+                            a.generic_lmax)
     for p in plans:
         print(p.name, 'groups', len(p.groups), 'nodes', sum(p.group_nodes), 'peak', max(p.group_peak))

@@ -68,6 +68,24 @@ struct L2gEriArgs {
 
 typedef void (*L2gLaunchFn)(const L2gEriArgs&, int grid, int block, hipStream_t s);
 
+// One class of the generic kernel (l2g_generic.hip): offsets into the
+// program arrays written by codegen/gen_generic.py and the LDS layout of a
+// block, all in doubles of LDS or in entries of the program / index arrays.
+struct L2gGenericClass {
+  int la, lb, lc, ld, L;
+  int ne, nf;        // bra / ket components of the [e0|f0] targets
+  int m0len, targ0;  // m-vector length of (00|00); its target index (-1 unless (ss|ss))
+  int base, zero, coef, acc;
+  int y0, w0, wrow, chk;  // ket phase: rows, row workspaces, rows per pass
+  int f0, wb0, wcol, chb; // bra phase: digest accumulators, column workspaces, columns per pass
+  int lds;                // doubles of dynamic LDS per block
+  int vrr_first, lev_first;
+  int hk_first, hk_nops, hk_out, hk_nws;
+  int hb_first, hb_nops, hb_out, hb_nws;
+  int diag_first, ndiag;
+  int nvals;              // recursion values per primitive quartet
+};
+
 struct L2gClassInfo {
   int la, lb, lc, ld;
   int ngroups;

@@ -243,10 +243,15 @@ struct FockItems {
   }
 };
 
-int nfock_classes() { return l2g_gen_nfock + l2g_gen_nfock32; }
+// Fock kernel classes: generated FP64 classes, generated FP32 classes, then
+// the generic kernel's classes (FP64).
+int nfock_classes() { return l2g_gen_nfock + l2g_gen_nfock32 + l2g_gen_ngeneric; }
+bool fock_is_generic(int c) { return c >= l2g_gen_nfock + l2g_gen_nfock32; }
+const L2gGenericClass& fock_gclass(int c) { return l2g_gen_generic[c - l2g_gen_nfock - l2g_gen_nfock32]; }
 const L2gClassInfo& fock_info(int c) {
   return c < l2g_gen_nfock ? l2g_gen_fock[c] : l2g_gen_fock32[c - l2g_gen_nfock];
 }
+int fock_ybuf(int c) { return fock_is_generic(c) ? 0 : fock_info(c).ybuf_doubles; }
 
 // The screened unique quartets of one Fock build, by kernel class.
 struct FockWork {
@@ -292,9 +297,11 @@ void sort_by_depth(FockItems& it) {
 // ~1300 primitive quartets x 19 groups on one lane, tens of ms.
 void split_deep(FockItems& it, int cls) {
   const size_t m = it.size();
-  if (m == 0 || m >= (size_t)(1 << 17)) return;
-  const int nodes = std::max(1, fock_info(cls).nodes);
-  const int chunk = std::max(8, 140000 / nodes);
+  // the generic kernel runs one quartet per 256-thread block
+  const bool gen = fock_is_generic(cls);
+  if (m == 0 || m >= (size_t)(gen ? 4096 : (1 << 17))) return;
+  const int chunk = gen ? std::max(16, 2000000 / std::max(1, fock_gclass(cls).nvals))
+                        : std::max(8, 140000 / std::max(1, fock_info(cls).nodes));
   size_t extra = 0;
   for (size_t i = 0; i < m; ++i) {
     const int len = it.pr[i].y - it.pr[i].x;
@@ -373,9 +380,12 @@ void fock_enumerate(const FockSetup& f, const double* D, double precision, doubl
             const int b2 = sbk ? (stk ? s3 : s4) : (stb ? s1 : s2);
             const int k1 = sbk ? (stb ? s2 : s1) : (stk ? s4 : s3);
             const int k2 = sbk ? (stb ? s1 : s2) : (stk ? s3 : s4);
-            int cls = ctx->fock_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
-            if (cls < 0) throw L2gError(L2G_E_LMAX, "no Fock kernel for this angular momentum class");
-            if (fp32_threshold > 0 && K12 * K34 < fp32_threshold &&
+            int cls = ctx->force_generic ? -1 : ctx->fock_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+            if (cls < 0) {
+              const int g = ctx->generic_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+              if (g < 0) throw L2gError(L2G_E_LMAX, "no Fock kernel for this angular momentum class");
+              cls = l2g_gen_nfock + l2g_gen_nfock32 + g;
+            } else if (fp32_threshold > 0 && K12 * K34 < fp32_threshold &&
                 S12 * f.S[(size_t)s3 * n + s4] < L2G_FP32_HRR_FACTOR * fp32_threshold) {
               const int c32 = ctx->fock32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
               if (c32 >= 0) {
@@ -490,8 +500,8 @@ struct ItemsDev {
 // grid and scratch as sized by the caller.
 // With `ev` (2 events per class), records the class's launch between
 // ev[2c] and ev[2c+1].
-void fock_launch_all(L2gEriArgs A, const ItemsDev& items, bool with_req, int grid, int block, hipStream_t st,
-                     hipEvent_t* ev = nullptr) {
+void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsDev& items, bool with_req, int grid, int block,
+                     hipStream_t st, hipEvent_t* ev = nullptr) {
   for (size_t c = 0; c < items.count.size(); ++c) {
     const int64_t m = items.count[c], o = items.first[c];
     if (m == 0) continue;
@@ -505,8 +515,13 @@ void fock_launch_all(L2gEriArgs A, const ItemsDev& items, bool with_req, int gri
     A.q_req = with_req ? items.req.as<int>() + o : nullptr;
     A.q_prange = items.pr.as<int2>() + o;
     A.n = (int)m;
-    fock_info((int)c).launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
-    HIPCHK(hipGetLastError());
+    if (fock_is_generic((int)c)) {
+      launch_generic(kGenericFock, A, fock_gclass((int)c), ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
+                     (int)std::min<int64_t>((int64_t)ctx->ncu * 8, m), st);
+    } else {
+      fock_info((int)c).launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
+      HIPCHK(hipGetLastError());
+    }
     if (ev) HIPCHK(hipEventRecord(ev[2 * c + 1], st));
   }
 }
@@ -589,7 +604,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     for (int c = 0; c < nfock_classes(); ++c)
       if (!w.cls[c].bra.empty()) {
         maxn = std::max<int64_t>(maxn, (int64_t)w.cls[c].size());
-        ymax = std::max<int64_t>(ymax, fock_info(c).ybuf_doubles);
+        ymax = std::max<int64_t>(ymax, fock_ybuf(c));
       }
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
     const int64_t ystride = (int64_t)grid * block;
@@ -615,7 +630,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     }
     hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
     HIPCHK(hipEventRecord(e0, st));
-    fock_launch_all(A, f->items, false, grid, block, st, f->ev.data());
+    fock_launch_all(ctx, A, f->items, false, grid, block, st, f->ev.data());
     HIPCHK(hipEventRecord(e1, st));
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);
@@ -661,9 +676,17 @@ int l2g_fock_class_stats(l2g_fock f, int cap, int* nclasses, int* class_code, in
     for (int c = 0; c < (int)f->cls_nq.size(); ++c) {
       if (f->cls_nq[c] == 0) continue;
       if (k < cap) {
-        const L2gClassInfo& ci = fock_info(c);
+        int la, lb, lc, ld;
+        if (fock_is_generic(c)) {
+          const L2gGenericClass& g = fock_gclass(c);
+          la = g.la, lb = g.lb, lc = g.lc, ld = g.ld;
+        } else {
+          const L2gClassInfo& ci = fock_info(c);
+          la = ci.la, lb = ci.lb, lc = ci.lc, ld = ci.ld;
+        }
         if (class_code)
-          class_code[k] = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld + (c >= l2g_gen_nfock ? 10000 : 0);
+          class_code[k] = la * 1000 + lb * 100 + lc * 10 + ld +
+                          (fock_is_generic(c) ? 20000 : (c >= l2g_gen_nfock ? 10000 : 0));
         if (nquartets) nquartets[k] = f->cls_nq[c];
         if (nprim_quartets) nprim_quartets[k] = f->cls_nprim[c];
         if (ms) {
@@ -1020,7 +1043,7 @@ void run_round(l2g_server_s* S, const std::vector<ServerRequest*>& act) {
                            S->a_P.as<double4>(), S->a_PA.as<double4>(), S->a_aux.as<double2>(), S->a_ns.as<double>(),
                            S->a_dens.as<double>(), S->a_jg.as<double>(), S->a_kg.as<double>(),
                            S->rmats[b].as<L2gMatDesc>(), S->ybuf.as<double>(), S->ystride);
-  fock_launch_all(A, S->rbuf[b], true, S->grid, S->block, st);
+  fock_launch_all(ctx, A, S->rbuf[b], true, S->grid, S->block, st);
   HIPCHK(hipEventRecord(S->rev[b], st));
   // requests whose last quartets are in this round: finish on the device and
   // copy J, K, G to their pinned host buffers
@@ -1132,7 +1155,7 @@ int l2g_server_create(const l2g_server_config* config, l2g_server* out) {
     S->a_np.alloc(cap_pair * sizeof(int));
     S->a_AB.alloc(cap_pair * 3 * sizeof(double));
     int64_t ymax = 1;
-    for (int k = 0; k < nfock_classes(); ++k) ymax = std::max<int64_t>(ymax, fock_info(k).ybuf_doubles);
+    for (int k = 0; k < nfock_classes(); ++k) ymax = std::max<int64_t>(ymax, fock_ybuf(k));
     S->grid = S->exec->ncu * 8;
     S->ystride = (int64_t)S->grid * S->block;
     S->ybuf.alloc(ymax * S->ystride * sizeof(double));

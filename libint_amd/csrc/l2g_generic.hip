@@ -1,0 +1,381 @@
+// Generic (table-driven) kernel for the classes the register-resident
+// kernels do not cover: every class with an f shell.  One 256-thread block
+// evaluates one contracted shell quartet at a time with all intermediates in
+// LDS; the per-class work comes from the programs of codegen/gen_generic.py:
+//
+//   1. per primitive quartet: screening and prefactors (every thread, the
+//      values are block-uniform), F_m(T) by Chebyshev interpolation of each
+//      m on its own thread (the scheme of FmEval_Chebyshev7::eval,
+//      include/libint2/boys.h:345-454), then the OS VRR level by level, a
+//      thread computing whole m-vectors of [e0|f0] nodes and adding the
+//      m = 0 value of target nodes to their contracted accumulator;
+//   2. ket HRR (CD) and ket solid harmonics, one thread per bra row;
+//   3. bra HRR (AB) and bra solid harmonics, one thread per ket column,
+//      followed by the caller-order store (ERI), the J/K digest into LDS
+//      accumulators flushed with FP64 atomics (Fock), or the norm reduction
+//      (Schwarz).
+//
+// The maths is that of the generated kernels (engine.impl.h:1306-1985 for
+// the reference's steps); only the mapping onto the GPU differs.
+#include "l2g_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kModeEri = 0, kModeFock = 1, kModeSchwarz = 2;
+
+// solid-harmonic coefficients, standard order (m = -l..l), l <= 3:
+// [l][m + l][Cartesian index]
+__constant__ double c_sh[4][7][10];
+
+__device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
+
+// F_m(T) for one m: the Chebyshev-7 interpolant of the interval holding T,
+// or the asymptotic form above T_crit (boys.h:348-358)
+__device__ __forceinline__ double boys_one(double T, int m, const double* __restrict__ tab) {
+  if (T > L2G_BOYS_TCRIT) {
+    const double oox = 1.0 / T;
+    double f = 0.88622692545275801365 * sqrt(oox);
+    for (int j = 1; j <= m; ++j) f *= ((double)j - 0.5) * oox;
+    return f;
+  }
+  const double x7 = T * 7.0;
+  const double ivd = fmin(floor(x7), (double)(L2G_BOYS_NINT - 1));
+  const double xd = x7 - ivd - 0.5;
+  const double* c = tab + ((size_t)m * L2G_BOYS_NINT + (size_t)ivd) * 8;
+  double f = c[7];
+  for (int k = 6; k >= 0; --k) f = fma(f, xd, c[k]);
+  return f;
+}
+
+// out[i][j] (n1o x n2) = sum_a U_l1[i][a] in[a][j] when pure, else a copy
+__device__ void sph_first(const double* in, double* out, int l1, int n2, int pure) {
+  const int nc = ncart_d(l1);
+  if (!pure || l1 == 0) {
+    for (int k = 0; k < nc * n2; ++k) out[k] = in[k];
+    return;
+  }
+  for (int i = 0; i < 2 * l1 + 1; ++i)
+    for (int j = 0; j < n2; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < nc; ++a) s += c_sh[l1][i][a] * in[a * n2 + j];
+      out[i * n2 + j] = s;
+    }
+}
+
+// out[i][j] (n1 x n2o) = sum_b U_l2[j][b] in[i][b] when pure, else a copy
+__device__ void sph_second(const double* in, double* out, int n1, int l2, int pure) {
+  const int nc = ncart_d(l2);
+  const int no = pure && l2 > 0 ? 2 * l2 + 1 : nc;
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < no; ++j) {
+      double s;
+      if (no == nc && !(pure && l2 > 0)) {
+        s = in[i * nc + j];
+      } else {
+        s = 0.0;
+        for (int b = 0; b < nc; ++b) s += c_sh[l2][j][b] * in[i * nc + b];
+      }
+      out[i * no + j] = s;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs A, const L2gGenericClass C,
+                                                                 const int4* __restrict__ prog,
+                                                                 const int* __restrict__ idx) {
+  extern __shared__ double lds[];
+  __shared__ double red[2 * kThreads / 64];
+  const int t = threadIdx.x;
+  const int ne = C.ne, nf = C.nf;
+  const int la = C.la, lb = C.lb, lc = C.lc, ld = C.ld;
+  for (int qi = blockIdx.x; qi < A.n; qi += gridDim.x) {
+    const int bp = A.bra_pair[qi];
+    const int kp = MODE == kModeSchwarz ? bp : A.ket_pair[qi];
+    const int meta = A.meta[qi];
+    const int pb0 = A.pair_first[bp], npb = A.pair_np[bp];
+    const int pk0 = A.pair_first[kp], npk = A.pair_np[kp];
+    const double AB[3] = {A.pair_AB[3 * bp], A.pair_AB[3 * bp + 1], A.pair_AB[3 * bp + 2]};
+    const double CD[3] = {A.pair_AB[3 * kp], A.pair_AB[3 * kp + 1], A.pair_AB[3 * kp + 2]};
+    const double npbk = (double)npb * (double)npk;
+    double lnprec, qprec;
+    if (MODE == kModeFock) {
+      lnprec = A.q_lnprec[qi];
+      qprec = A.method == L2G_SCREEN_ORIGINAL || A.method == L2G_SCREEN_CONSERVATIVE ? exp(lnprec) : 0.0;
+    } else {
+      lnprec = A.ln_precision;
+      qprec = A.precision;
+    }
+    int pbeg = 0, pend = npb * npk;
+    if (MODE == kModeFock && A.q_prange) {
+      const int2 pr = A.q_prange[qi];
+      pbeg = pr.x;
+      pend = pr.y;
+    }
+    for (int i = t; i < ne * nf; i += kThreads) lds[C.acc + i] = 0.0;
+    if (t < 16) lds[C.zero + t] = 0.0;
+    __syncthreads();
+    int nsurv = 0;
+    // ---- 1. primitive quartets: prefactors, F_m, VRR, contraction
+    for (int p = pbeg; p < pend; ++p) {
+      const int ib = p / npk, ik = p - ib * npk;
+      const double4 Pg = A.pp_P[pb0 + ib], PA = A.pp_PA[pb0 + ib];
+      const double2 xb = A.pp_aux[pb0 + ib];
+      const double4 Qg = A.pp_P[pk0 + ik], QC = A.pp_PA[pk0 + ik];
+      const double2 xk = A.pp_aux[pk0 + ik];
+      const double gp = Pg.w, gq = Qg.w;
+      const double rs = rsqrt(gp + gq);
+      const double oos = rs * rs;
+      const double pf = PA.w * QC.w * rs * A.scale;
+      bool ok = xb.x + xk.x > lnprec;
+      if (A.method == L2G_SCREEN_ORIGINAL) {
+        ok = ok && !(fabs(pf) < qprec);
+      } else if (A.method == L2G_SCREEN_CONSERVATIVE) {
+        const double nsb = A.pp_nonsph[pb0 + ib], nsk = A.pp_nonsph[pk0 + ik];
+        ok = ok && !(fabs(pf) * fmax(1.0, nsb * nsk) * npbk < qprec);
+      }
+      if (!ok) continue;  // block-uniform
+      ++nsurv;
+      const double PQx = Pg.x - Qg.x, PQy = Pg.y - Qg.y, PQz = Pg.z - Qg.z;
+      const double T = gp * gq * oos * (PQx * PQx + PQy * PQy + PQz * PQz);
+      if (t < 17) {
+        const double goq = gq * oos, gop = gp * oos;
+        double v = 0.0;
+        switch (t) {
+          case 0: v = PA.x; break;
+          case 1: v = PA.y; break;
+          case 2: v = PA.z; break;
+          case 3: v = QC.x; break;
+          case 4: v = QC.y; break;
+          case 5: v = QC.z; break;
+          case 6: v = -goq * PQx; break;
+          case 7: v = -goq * PQy; break;
+          case 8: v = -goq * PQz; break;
+          case 9: v = gop * PQx; break;
+          case 10: v = gop * PQy; break;
+          case 11: v = gop * PQz; break;
+          case 12: v = xb.y; break;   // 1/(2 zeta)
+          case 13: v = xk.y; break;   // 1/(2 eta)
+          case 14: v = goq; break;    // rho/zeta
+          case 15: v = gop; break;    // rho/eta
+          default: v = 0.5 * oos;     // 1/(2 (zeta + eta))
+        }
+        lds[C.coef + t] = v;
+      }
+      if (t < C.m0len) lds[C.base + t] = pf * boys_one(T, t, A.boys);
+      __syncthreads();
+      for (int k = 1; k <= C.L; ++k) {
+        const int n0 = idx[C.lev_first + k - 1], n1 = idx[C.lev_first + k];
+        for (int nd = n0 + t; nd < n1; nd += kThreads) {
+          const int4 r0 = prog[C.vrr_first + 2 * nd];
+          const int4 r1 = prog[C.vrr_first + 2 * nd + 1];
+          const int na = r1.w & 15, nc = (r1.w >> 4) & 15, kind = r1.w >> 8;
+          const double cA = lds[C.coef + r1.z], cW = lds[C.coef + 6 + r1.z];
+          const double g = (double)na * lds[C.coef + 12 + kind], r = lds[C.coef + 14 + kind];
+          const double h = (double)nc * lds[C.coef + 16];
+          const double* X = lds + r0.y;
+          const double* Y = lds + r0.z;
+          const double* Z = lds + r0.w;
+          double* O = lds + r0.x;
+          for (int m = 0; m < r1.y; ++m) O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
+          if (r1.x >= 0) lds[C.acc + r1.x] += O[0];
+        }
+        __syncthreads();
+      }
+      if (C.targ0 >= 0 && t == 0) lds[C.acc + C.targ0] += lds[C.base];
+    }
+    __syncthreads();
+    if (MODE == kModeEri && t == 0 && A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;
+    if (MODE == kModeFock && nsurv == 0) continue;  // nothing to digest (libint's nullptr)
+    double smax = 0.0, ssq = 0.0, hmax = 0.0;
+    if (MODE == kModeSchwarz)
+      for (int i = t; i < C.ndiag; i += kThreads) hmax = fmax(hmax, fabs(lds[C.acc + idx[C.diag_first + i]]));
+    // ---- 2. ket HRR and ket solid harmonics, one thread per bra row
+    const int pk1 = (meta >> 5) & 1, pk2 = (meta >> 6) & 1;
+    const int nc3 = ncart_d(lc), nc4 = ncart_d(ld), ncc = nc3 * nc4;
+    const int nk1 = pk1 && lc > 0 ? 2 * lc + 1 : nc3, nk2 = pk2 && ld > 0 ? 2 * ld + 1 : nc4;
+    for (int r0 = 0; r0 < ne; r0 += C.chk) {
+      const int ie = r0 + t;
+      if (t < C.chk && ie < ne) {
+        double* ws = lds + C.w0 + t * C.wrow;
+        double* tmp = ws + C.hk_nws;
+        for (int j = 0; j < nf; ++j) ws[j] = lds[C.acc + ie * nf + j];
+        for (int o = 0; o < C.hk_nops; ++o) {
+          const int4 op = prog[C.hk_first + o];
+          ws[op.x] = ws[op.y] + CD[op.w] * ws[op.z];
+        }
+        for (int k = 0; k < ncc; ++k) tmp[k] = ws[idx[C.hk_out + k]];
+        sph_first(tmp, ws, lc, nc4, pk1);
+        sph_second(ws, lds + C.y0 + ie * ncc, nk1, ld, pk2);
+      }
+    }
+    __syncthreads();
+    // ---- 3. bra HRR, bra solid harmonics and the store / digest / norm
+    const int pb1 = (meta >> 3) & 1, pb2 = (meta >> 4) & 1;
+    const int nc1 = ncart_d(la), nc2 = ncart_d(lb), nab = nc1 * nc2;
+    const int n1 = pb1 && la > 0 ? 2 * la + 1 : nc1, n2 = pb2 && lb > 0 ? 2 * lb + 1 : nc2;
+    const int ncol = nk1 * nk2;
+    L2gStore st;
+    st.gauss = A.shg_gaussian;
+    st.l1 = la, st.l2 = lb, st.l3 = lc, st.l4 = ld;
+    st.p1 = pb1, st.p2 = pb2, st.p3 = pk1, st.p4 = pk2;
+    st.n1 = n1, st.n2 = n2, st.nk1 = nk1, st.nk2 = nk2;
+    st.uniform = A.uniform_cart;
+    if (MODE == kModeEri) st = make_store(meta, A, qi, la, lb, lc, ld);
+    // Fock: matrices of this item's request, accumulators J12 [n1 n2],
+    // K13 [n1 nk1], K14 [n1 nk2], K24 [n2 nk2], K23 [n2 nk1]
+    long long N = 0, moff = 0;
+    int4 bf = int4{0, 0, 0, 0};
+    double deg = 0.0;
+    double *dJ12 = nullptr, *dK13 = nullptr, *dK14 = nullptr, *dK24 = nullptr, *dK23 = nullptr;
+    if (MODE == kModeFock) {
+      const L2gMatDesc md = A.mats[A.q_req ? A.q_req[qi] : 0];
+      N = md.n;
+      moff = md.off;
+      bf = A.q_bf[qi];
+      deg = A.q_scale[qi];
+      dJ12 = lds + C.f0;
+      dK13 = dJ12 + n1 * n2;
+      dK14 = dK13 + n1 * nk1;
+      dK24 = dK14 + n1 * nk2;
+      dK23 = dK24 + n2 * nk2;
+      const int ndg = n1 * n2 + (n1 + n2) * (nk1 + nk2);
+      for (int i = t; i < ndg; i += kThreads) dJ12[i] = 0.0;
+      __syncthreads();
+    }
+    const double* Dm = A.dens + moff;
+    for (int c0 = 0; c0 < ncol; c0 += C.chb) {
+      const int col = c0 + t;
+      if (t < C.chb && col < ncol) {
+        double* ws = lds + C.wb0 + t * C.wcol;
+        double* tmp = ws + C.hb_nws;
+        for (int j = 0; j < ne; ++j) ws[j] = lds[C.y0 + j * ncc + col];
+        for (int o = 0; o < C.hb_nops; ++o) {
+          const int4 op = prog[C.hb_first + o];
+          ws[op.x] = ws[op.y] + AB[op.w] * ws[op.z];
+        }
+        for (int k = 0; k < nab; ++k) tmp[k] = ws[idx[C.hb_out + k]];
+        sph_first(tmp, ws, la, nc2, pb1);
+        sph_second(ws, tmp, n1, lb, pb2);  // values v[i1 n2 + i2] in tmp
+        const int ic = col / nk2, id = col - ic * nk2;
+        if (MODE == kModeEri) {
+          const size_t colofs = st.base + (size_t)st.pos3(ic) * st.s3 + (size_t)st.pos4(id) * st.s4;
+          const double colscale = st.uniform ? st.scale3(ic) * st.scale4(id) : 1.0;
+          for (int i1 = 0; i1 < n1; ++i1)
+            for (int i2 = 0; i2 < n2; ++i2) st.put(A.out, colofs, i1, i2, colscale, tmp[i1 * n2 + i2]);
+        } else if (MODE == kModeSchwarz) {
+          for (int k = 0; k < n1 * n2; ++k) {
+            smax = fmax(smax, fabs(tmp[k]));
+            ssq += tmp[k] * tmp[k];
+          }
+        } else {
+          const long long bk3 = bf.z + st.pos3(ic), bk4 = bf.w + st.pos4(id);
+          const double d34 = Dm[bk3 * N + bk4];
+          double jcol = 0.0;
+          for (int i1 = 0; i1 < n1; ++i1) {
+            const long long ba1 = bf.x + st.pos(la, pb1, i1);
+            const double d13 = Dm[ba1 * N + bk3], d14 = Dm[ba1 * N + bk4];
+            double k13 = 0.0, k14 = 0.0;
+            for (int i2 = 0; i2 < n2; ++i2) {
+              const long long ba2 = bf.y + st.pos(lb, pb2, i2);
+              const double v = tmp[i1 * n2 + i2];
+              jcol += Dm[ba1 * N + ba2] * v;
+              atomicAdd(&dJ12[i1 * n2 + i2], d34 * v);
+              k13 += Dm[ba2 * N + bk4] * v;
+              k14 += Dm[ba2 * N + bk3] * v;
+              atomicAdd(&dK24[i2 * nk2 + id], d13 * v);
+              atomicAdd(&dK23[i2 * nk1 + ic], d14 * v);
+            }
+            atomicAdd(&dK13[i1 * nk1 + ic], k13);
+            atomicAdd(&dK14[i1 * nk2 + id], k14);
+          }
+          unsafeAtomicAdd(&A.jg[moff + bk3 * N + bk4], deg * jcol);
+        }
+      }
+    }
+    if (MODE == kModeFock) {
+      __syncthreads();
+      double* jg = A.jg + moff;
+      double* kg = A.kg + moff;
+      for (int i = t; i < n1 * n2; i += kThreads) {
+        const int i1 = i / n2, i2 = i - i1 * n2;
+        unsafeAtomicAdd(&jg[(bf.x + st.pos(la, pb1, i1)) * N + bf.y + st.pos(lb, pb2, i2)], deg * dJ12[i]);
+      }
+      for (int i = t; i < n1 * nk1; i += kThreads) {
+        const int i1 = i / nk1, ic = i - i1 * nk1;
+        unsafeAtomicAdd(&kg[(bf.x + st.pos(la, pb1, i1)) * N + bf.z + st.pos3(ic)], deg * dK13[i]);
+      }
+      for (int i = t; i < n1 * nk2; i += kThreads) {
+        const int i1 = i / nk2, id = i - i1 * nk2;
+        unsafeAtomicAdd(&kg[(bf.x + st.pos(la, pb1, i1)) * N + bf.w + st.pos4(id)], deg * dK14[i]);
+      }
+      for (int i = t; i < n2 * nk2; i += kThreads) {
+        const int i2 = i / nk2, id = i - i2 * nk2;
+        unsafeAtomicAdd(&kg[(bf.y + st.pos(lb, pb2, i2)) * N + bf.w + st.pos4(id)], deg * dK24[i]);
+      }
+      for (int i = t; i < n2 * nk1; i += kThreads) {
+        const int i2 = i / nk1, ic = i - i2 * nk1;
+        unsafeAtomicAdd(&kg[(bf.y + st.pos(lb, pb2, i2)) * N + bf.z + st.pos3(ic)], deg * dK23[i]);
+      }
+    }
+    if (MODE == kModeSchwarz) {
+      // block reductions of max|v|, sum v^2 and the pre-HRR diagonal max
+      for (int o = 32; o > 0; o >>= 1) {
+        smax = fmax(smax, __shfl_xor(smax, o));
+        ssq += __shfl_xor(ssq, o);
+        hmax = fmax(hmax, __shfl_xor(hmax, o));
+      }
+      const int w = t >> 6, nw = kThreads >> 6;
+      if ((t & 63) == 0) {
+        red[w] = smax;
+        red[nw + w] = ssq;
+      }
+      __syncthreads();
+      if (t == 0) {
+        double m = 0.0, q = 0.0;
+        for (int i = 0; i < nw; ++i) m = fmax(m, red[i]), q += red[nw + i];
+        A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(q)) : sqrt(m);
+      }
+      __syncthreads();
+      if ((t & 63) == 0) red[w] = hmax;
+      __syncthreads();
+      if (t == 0 && A.out2) {
+        double m = 0.0;
+        for (int i = 0; i < nw; ++i) m = fmax(m, red[i]);
+        A.out2[A.out_off[qi]] = sqrt(m);
+      }
+    }
+    __syncthreads();  // LDS is reused by the next quartet
+  }
+}
+
+}  // namespace
+
+namespace l2g_impl {
+
+void generic_init(const double* sh) {
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sh), sh, sizeof(double) * 4 * 7 * 10));
+  // 160 KB per CU on gfx950, less 512 bytes for the static reduction array
+  // (codegen/gen_generic.py plans at most that many doubles)
+  const int maxlds = 160 * 1024 - 512;
+  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<kModeEri>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<kModeFock>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<kModeSchwarz>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+}
+
+void launch_generic(int mode, const L2gEriArgs& A, const L2gGenericClass& C, const int4* prog, const int* idx,
+                    int grid, hipStream_t s) {
+  const size_t shm = (size_t)C.lds * sizeof(double);
+  if (mode == kModeEri)
+    hipLaunchKernelGGL(l2g_generic_kernel<kModeEri>, dim3(grid), dim3(kThreads), shm, s, A, C, prog, idx);
+  else if (mode == kModeFock)
+    hipLaunchKernelGGL(l2g_generic_kernel<kModeFock>, dim3(grid), dim3(kThreads), shm, s, A, C, prog, idx);
+  else
+    hipLaunchKernelGGL(l2g_generic_kernel<kModeSchwarz>, dim3(grid), dim3(kThreads), shm, s, A, C, prog, idx);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace l2g_impl

@@ -81,8 +81,9 @@ const BoysTable& boys_table() {
 }  // namespace
 
 struct ClassWork {
-  int cls;  // index into l2g_gen_classes (l2g_gen_classes32 when fp32)
+  int cls;  // index into l2g_gen_classes (l2g_gen_classes32 when fp32, l2g_gen_generic when generic)
   bool fp32 = false;
+  bool generic = false;
   int64_t first, count;
   int64_t nprimq;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -121,7 +122,7 @@ const char* l2g_version(void) { return "libint_amd 0.1 (gfx950)"; }
 
 const char* l2g_last_error(void) { return g_err.c_str(); }
 
-int l2g_max_l(void) { return l2g_gen_lmax; }
+int l2g_max_l(void) { return lmax_supported(); }
 
 int l2g_shell_renorm(int l, int nprim, const double* alpha, double* coeff, double* max_ln_coeff, int enforce_unit) {
   return guarded([&] {
@@ -228,6 +229,31 @@ int l2g_init(int device, l2g_context* out) {
       const auto& c = l2g_gen_schwarz[i];
       if (c.launch) ctx->schwarz_of[c.la][c.lb] = i;
     }
+    for (auto& a : ctx->generic_of)
+      for (auto& b : a)
+        for (auto& c : b)
+          for (auto& d : c) d = -1;
+    for (int i = 0; i < l2g_gen_ngeneric; ++i) {
+      const auto& c = l2g_gen_generic[i];
+      ctx->generic_of[c.la][c.lb][c.lc][c.ld] = i;
+    }
+    ctx->gprog.alloc(std::max<size_t>((size_t)l2g_gen_generic_nprog, 1) * sizeof(int4));
+    ctx->gidx.alloc(std::max<size_t>((size_t)l2g_gen_generic_nidx, 1) * sizeof(int));
+    if (l2g_gen_generic_nprog)
+      HIPCHK(hipMemcpy(ctx->gprog.p, l2g_gen_generic_prog, (size_t)l2g_gen_generic_nprog * sizeof(int4),
+                       hipMemcpyHostToDevice));
+    if (l2g_gen_generic_nidx)
+      HIPCHK(hipMemcpy(ctx->gidx.p, l2g_gen_generic_idx, (size_t)l2g_gen_generic_nidx * sizeof(int),
+                       hipMemcpyHostToDevice));
+    // solid-harmonic coefficients of the generic kernel, standard order
+    std::vector<double> sh(4 * 7 * 10, 0.0);
+    for (int l = 0; l <= 3; ++l)
+      for (int m = -l; m <= l; ++m) {
+        int c = 0;
+        for (int x = l; x >= 0; --x)
+          for (int y = l - x; y >= 0; --y, ++c) sh[(l * 7 + m + l) * 10 + c] = l2g_solid_harmonics_coeff(l, m, x, y, l - x - y);
+      }
+    generic_init(sh.data());
     *out = ctx.release();
   });
 }
@@ -242,6 +268,13 @@ int l2g_finalize(l2g_context ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+  });
+}
+
+int l2g_set_generic_kernels(l2g_context ctx, int on) {
+  return guarded([&] {
+    if (!ctx) throw L2gError(L2G_E_STATE, "null context");
+    ctx->force_generic = on != 0;
   });
 }
 
@@ -271,9 +304,9 @@ int l2g_basis_create(l2g_context ctx, int nshell, const int* l, const int* pure,
     b->off[0] = 0;
     for (int s = 0; s < nshell; ++s) {
       if (l[s] < 0) throw std::invalid_argument("negative angular momentum");
-      if (l[s] > l2g_gen_lmax)
+      if (l[s] > lmax_supported())
         throw L2gError(L2G_E_LMAX, "angular momentum " + std::to_string(l[s]) + " exceeds the compiled limit " +
-                                       std::to_string(l2g_gen_lmax));
+                                       std::to_string(lmax_supported()));
       if (nprim[s] <= 0) throw std::invalid_argument("shell without primitives");
       b->off[s + 1] = b->off[s] + nprim[s];
     }
@@ -352,7 +385,8 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       int64_t off, q;
       int64_t w;
     };
-    std::vector<std::vector<Item>> buckets(l2g_gen_nclasses + l2g_gen_nclasses32);
+    const int nc32 = l2g_gen_nclasses32;
+    std::vector<std::vector<Item>> buckets(l2g_gen_nclasses + l2g_gen_nclasses32 + l2g_gen_ngeneric);
     plan->offsets.resize(nq);
     int64_t total = 0;
     for (int64_t q = 0; q < nq; ++q) {
@@ -366,9 +400,12 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       const int b2 = sbk ? (stk ? s[2] : s[3]) : (stb ? s[0] : s[1]);
       const int k1 = sbk ? (stb ? s[1] : s[0]) : (stk ? s[3] : s[2]);
       const int k2 = sbk ? (stb ? s[0] : s[1]) : (stk ? s[2] : s[3]);
-      int cls = ctx->class_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
-      if (cls < 0) throw L2gError(L2G_E_LMAX, "no kernel for this angular momentum class");
-      if (fp32_threshold > 0 && use_fp32(s)) {
+      int cls = ctx->force_generic ? -1 : ctx->class_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+      if (cls < 0) {
+        const int g = ctx->generic_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
+        if (g < 0) throw L2gError(L2G_E_LMAX, "no kernel for this angular momentum class");
+        cls = nc64 + nc32 + g;  // FP64 only
+      } else if (fp32_threshold > 0 && use_fp32(s)) {
         const int c32 = ctx->class32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
         if (c32 >= 0) cls = nc64 + c32;
       }
@@ -400,8 +437,9 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
     for (int cb = 0; cb < (int)buckets.size(); ++cb) {
       auto& v = buckets[cb];
       if (v.empty()) continue;
-      const bool is32 = cb >= nc64;
-      const int c = is32 ? cb - nc64 : cb;
+      const bool isgen = cb >= nc64 + nc32;
+      const bool is32 = !isgen && cb >= nc64;
+      const int c = isgen ? cb - nc64 - nc32 : (is32 ? cb - nc64 : cb);
       std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) {
         if (a.w != b.w) return a.w > b.w;
         if (a.bp != b.bp) return a.bp < b.bp;
@@ -410,6 +448,7 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       ClassWork cw;
       cw.cls = c;
       cw.fp32 = is32;
+      cw.generic = isgen;
       cw.first = (int64_t)bra.size();
       cw.count = (int64_t)v.size();
       cw.nprimq = 0;
@@ -424,7 +463,7 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       HIPCHK(hipEventCreate(&cw.ev0));
       HIPCHK(hipEventCreate(&cw.ev1));
       plan->classes.push_back(cw);
-      ymax = std::max<int64_t>(ymax, (is32 ? l2g_gen_classes32 : l2g_gen_classes)[c].ybuf_doubles);
+      if (!isgen) ymax = std::max<int64_t>(ymax, (is32 ? l2g_gen_classes32 : l2g_gen_classes)[c].ybuf_doubles);
     }
     hipStream_t st = ctx->stream;
     upload(plan->bra, bra, st);
@@ -513,17 +552,24 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     p->nz.alloc(std::max<int64_t>(p->nq, 1));
     A.nz = p->nz.as<signed char>();
     for (auto& c : p->classes) {
-      const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
       A.bra_pair = p->bra.as<int>() + c.first;
       A.ket_pair = p->ket.as<int>() + c.first;
       A.meta = p->meta.as<int>() + c.first;
       A.out_off = p->off.as<long long>() + c.first;
       A.qidx = p->qidx.as<long long>() + c.first;
       A.n = (int)c.count;
-      const int grid = (int)std::min<int64_t>(p->grid, (c.count + p->block - 1) / p->block);
       HIPCHK(hipEventRecord(c.ev0, ctx->stream));
-      ci.launch(A, grid, p->block, ctx->stream);
-      HIPCHK(hipGetLastError());
+      if (c.generic) {
+        // one block per quartet, persistent over the class
+        const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
+        launch_generic(kGenericEri, A, l2g_gen_generic[c.cls], ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid,
+                       ctx->stream);
+      } else {
+        const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
+        const int grid = (int)std::min<int64_t>(p->grid, (c.count + p->block - 1) / p->block);
+        ci.launch(A, grid, p->block, ctx->stream);
+        HIPCHK(hipGetLastError());
+      }
       HIPCHK(hipEventRecord(c.ev1, ctx->stream));
     }
   });
@@ -569,8 +615,15 @@ int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* n
     *ncls = (int)p->classes.size();
     for (int i = 0; i < (int)p->classes.size() && i < cap; ++i) {
       auto& c = p->classes[i];
-      const auto& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
-      if (code) code[i] = ci.la * 1000 + ci.lb * 100 + ci.lc * 10 + ci.ld + (c.fp32 ? 10000 : 0);
+      int la, lb, lc, ld;
+      if (c.generic) {
+        const auto& g = l2g_gen_generic[c.cls];
+        la = g.la, lb = g.lb, lc = g.lc, ld = g.ld;
+      } else {
+        const auto& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
+        la = ci.la, lb = ci.lb, lc = ci.lc, ld = ci.ld;
+      }
+      if (code) code[i] = la * 1000 + lb * 100 + lc * 10 + ld + (c.fp32 ? 10000 : 0) + (c.generic ? 20000 : 0);
       if (nquart) nquart[i] = c.count;
       if (nprimq) nprimq[i] = c.nprimq;
       if (ms) {
@@ -624,14 +677,20 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
   HIPCHK(hipSetDevice(ctx->device));
   const double lnp = precision > 0 ? std::log(precision) : std::numeric_limits<double>::lowest();
   PairBuilder pb(*B, lnp, L2G_SCREEN_ORIGINAL);
-  std::vector<std::vector<int>> pid(l2g_gen_nschwarz), meta(l2g_gen_nschwarz);
-  std::vector<std::vector<long long>> idx(l2g_gen_nschwarz);
+  // buckets: generated Schwarz kernels, then generic classes (ab|ab)
+  const int nsch = l2g_gen_nschwarz;
+  std::vector<std::vector<int>> pid(nsch + l2g_gen_ngeneric), meta(nsch + l2g_gen_ngeneric);
+  std::vector<std::vector<long long>> idx(nsch + l2g_gen_ngeneric);
   for (int64_t q = 0; q < npairs; ++q) {
     int a = pairs[2 * q], b = pairs[2 * q + 1];
     if (a < 0 || b < 0 || a >= B->nshell || b >= B->nshell) throw std::invalid_argument("shell index out of range");
     if (B->swap_pair(a, b)) std::swap(a, b);
-    const int c = ctx->schwarz_of[B->l[a]][B->l[b]];
-    if (c < 0) throw L2gError(L2G_E_LMAX, "no Schwarz kernel for this shell pair");
+    int c = ctx->force_generic ? -1 : ctx->schwarz_of[B->l[a]][B->l[b]];
+    if (c < 0) {
+      const int g = ctx->generic_of[B->l[a]][B->l[b]][B->l[a]][B->l[b]];
+      if (g < 0) throw L2gError(L2G_E_LMAX, "no Schwarz kernel for this shell pair");
+      c = nsch + g;
+    }
     const int pa = (B->pure[a] && B->l[a] > 0) ? 1 : 0, pbit = (B->pure[b] && B->l[b] > 0) ? 1 : 0;
     pid[c].push_back(pb.get(a, b));
     meta[c].push_back((pa << 3) | (pbit << 4) | (pa << 5) | (pbit << 6));
@@ -641,13 +700,13 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
   std::vector<long long> alli;
   std::vector<std::pair<int, int64_t>> ranges;
   int64_t ymax = 0, maxn = 0;
-  for (int c = 0; c < l2g_gen_nschwarz; ++c) {
+  for (int c = 0; c < (int)pid.size(); ++c) {
     if (pid[c].empty()) continue;
     ranges.push_back({c, (int64_t)allp.size()});
     allp.insert(allp.end(), pid[c].begin(), pid[c].end());
     allm.insert(allm.end(), meta[c].begin(), meta[c].end());
     alli.insert(alli.end(), idx[c].begin(), idx[c].end());
-    ymax = std::max<int64_t>(ymax, l2g_gen_schwarz[c].ybuf_doubles);
+    if (c < nsch) ymax = std::max<int64_t>(ymax, l2g_gen_schwarz[c].ybuf_doubles);
     maxn = std::max<int64_t>(maxn, (int64_t)pid[c].size());
   }
   hipStream_t st = ctx->stream;
@@ -695,8 +754,13 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
     A.meta = dm.as<int>() + r.second;
     A.out_off = di.as<long long>() + r.second;
     A.n = (int)n;
-    l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
-    HIPCHK(hipGetLastError());
+    if (c < nsch) {
+      l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
+      HIPCHK(hipGetLastError());
+    } else {
+      launch_generic(kGenericSchwarz, A, l2g_gen_generic[c - nsch], ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
+                     (int)std::min<int64_t>((int64_t)ctx->ncu * 8, n), st);
+    }
   }
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Schwarz kernel failed: ") + hipGetErrorString(e));

@@ -29,6 +29,14 @@ extern const int l2g_gen_nclasses32;
 extern const L2gClassInfo l2g_gen_fock32[];
 extern const int l2g_gen_nfock32;
 extern const int l2g_gen_lmax;
+// programs of the generic kernel (codegen/gen_generic.py)
+extern const L2gGenericClass l2g_gen_generic[];
+extern const int l2g_gen_ngeneric;
+extern const int l2g_gen_generic_lmax;
+extern const int l2g_gen_generic_prog[];
+extern const int l2g_gen_generic_nprog;
+extern const int l2g_gen_generic_idx[];
+extern const int l2g_gen_generic_nidx;
 
 
 namespace l2g_impl {
@@ -145,6 +153,9 @@ struct l2g_context_s {
   int class32_of[8][8][8][8];  // index into l2g_gen_classes32
   int fock32_of[8][8][8][8];   // index into l2g_gen_fock32
   int schwarz_of[8][8];
+  int generic_of[8][8][8][8];  // index into l2g_gen_generic
+  bool force_generic = false;  // every class through the generic kernel (testing)
+  l2g_impl::DevBuf gprog, gidx;  // device copies of the generic programs
   // bases, plans and Fock builders created on this context and not yet
   // destroyed; l2g_finalize refuses while any is alive
   std::atomic<int> live{0};
@@ -211,6 +222,14 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
 // l2g_aux.hip: in-place Fock symmetrisation of one N x N block (J into jg,
 // K into kg, G = 2J - K into g)
 void launch_fock_finish(double* jg, double* kg, double* g, long long N, hipStream_t s);
+
+// l2g_generic.hip: the table-driven kernel for classes without a generated
+// register-resident kernel (f shells); modes ERI store, Fock digest, Schwarz
+enum { kGenericEri = 0, kGenericFock = 1, kGenericSchwarz = 2 };
+void generic_init(const double* sh);  // sh[4][7][10] solid-harmonic coefficients
+void launch_generic(int mode, const L2gEriArgs& A, const L2gGenericClass& C, const int4* prog, const int* idx,
+                    int grid, hipStream_t s);
+inline int lmax_supported() { return std::max(l2g_gen_lmax, l2g_gen_generic_lmax); }
 
 
 // Shell-pair data for ordered pair (i, j), restating ShellPair::init

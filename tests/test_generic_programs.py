@@ -1,0 +1,110 @@
+"""CPU check of the generic kernel's programs (codegen/gen_generic.py): a
+numpy walk over the same records (VRR m-vector nodes in three rotating LDS
+regions, target accumulation, ket and bra HRR programs) must reproduce the
+oracle's Cartesian integrals for classes up to f.  This pins the tables the
+GPU kernel reads; the kernel's own arithmetic is checked on the GPU."""
+import numpy as np
+import pytest
+
+import libint_amd as la
+import oracle
+from libint_amd.codegen import gen_eri, gen_generic
+
+
+def cart_shell(l, prims, origin):
+    return la.Shell(l, prims, origin, pure=False)
+
+
+def emulate(prog, shells):
+    """Contracted Cartesian (ab|cd) of the canonical class `prog` for four
+    renormalised Cartesian shells, walking the program records."""
+    A, B, C, D = (np.array(s.O) for s in shells)
+    la_, lb_, lc_, ld_ = prog["la"], prog["lb"], prog["lc"], prog["ld"]
+    ne, nf, L = prog["ne"], prog["nf"], prog["L"]
+    lds = np.zeros(prog["lds"] + 64)
+    acc = np.zeros(ne * nf)
+    rec = prog["recs"]
+    lev = prog["lev_first"]
+
+    def pairs(s1, s2, P1, P2):
+        out = []
+        AB = P1 - P2
+        for a1, c1 in zip(s1.alpha, s1.coeff):
+            for a2, c2 in zip(s2.alpha, s2.coeff):
+                g = a1 + a2
+                P = (a1 * P1 + a2 * P2) / g
+                K = np.sqrt(2) * np.pi ** 1.25 * np.exp(-a1 * a2 / g * AB @ AB) / g * c1 * c2
+                out.append((g, P, K))
+        return out
+
+    for (gp, P, Kb) in pairs(shells[0], shells[1], A, B):
+        for (gq, Q, Kk) in pairs(shells[2], shells[3], C, D):
+            rho = gp * gq / (gp + gq)
+            PQ = P - Q
+            T = rho * PQ @ PQ
+            pf = Kb * Kk / np.sqrt(gp + gq)
+            coef = np.zeros(20)
+            coef[0:3], coef[3:6] = P - A, Q - C
+            coef[6:9], coef[9:12] = -gq / (gp + gq) * PQ, gp / (gp + gq) * PQ
+            coef[12], coef[13] = 0.5 / gp, 0.5 / gq
+            coef[14], coef[15], coef[16] = gq / (gp + gq), gp / (gp + gq), 0.5 / (gp + gq)
+            lds[prog["zero"]:prog["zero"] + 16] = 0
+            F = oracle.boys_reference(T, L)[0]
+            lds[prog["base"]:prog["base"] + prog["m0len"]] = pf * F[:prog["m0len"]]
+            for k in range(1, L + 1):
+                for nd in range(lev[k - 1], lev[k]):
+                    r0, r1 = rec[2 * nd], rec[2 * nd + 1]
+                    na, nc, kind = r1[3] & 15, (r1[3] >> 4) & 15, r1[3] >> 8
+                    cA, cW = coef[r1[2]], coef[6 + r1[2]]
+                    g, r, h = na * coef[12 + kind], coef[14 + kind], nc * coef[16]
+                    for m in range(r1[1]):
+                        lds[r0[0] + m] = (cA * lds[r0[1] + m] + cW * lds[r0[1] + m + 1]
+                                          + g * (lds[r0[2] + m] - r * lds[r0[2] + m + 1]) + h * lds[r0[3] + m + 1])
+                    if r1[0] >= 0:
+                        acc[r1[0]] += lds[r0[0]]
+            if prog["targ0"] >= 0:
+                acc[prog["targ0"]] += lds[prog["base"]]
+    CD, AB = C - D, A - B
+    ncc = gen_eri.ncart(lc_) * gen_eri.ncart(ld_)
+    Y = np.zeros((ne, ncc))
+    for ie in range(ne):
+        ws = np.zeros(prog["hk_nws"])
+        ws[:nf] = acc[ie * nf:(ie + 1) * nf]
+        for (d, s0, s1, ax) in prog["hk"]:
+            ws[d] = ws[s0] + CD[ax] * ws[s1]
+        Y[ie] = ws[prog["hk_out"]]
+    nab = gen_eri.ncart(la_) * gen_eri.ncart(lb_)
+    out = np.zeros((nab, ncc))
+    for col in range(ncc):
+        ws = np.zeros(prog["hb_nws"])
+        ws[:ne] = Y[:, col]
+        for (d, s0, s1, ax) in prog["hb"]:
+            ws[d] = ws[s0] + AB[ax] * ws[s1]
+        out[:, col] = ws[prog["hb_out"]]
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("cls", [(0, 0, 0, 0), (1, 0, 1, 0), (1, 1, 1, 1), (2, 0, 2, 1), (3, 0, 1, 0),
+                                 (2, 1, 3, 0), (3, 1, 2, 2), (3, 2, 3, 1)])
+def test_generic_program_matches_oracle(cls):
+    la_, lb_, lc_, ld_ = cls
+    prog = gen_generic.class_program(*cls)
+    rng = np.random.default_rng(sum(cls) + 17)
+    shells = []
+    for l in cls:
+        K = int(rng.integers(1, 3))
+        shells.append(cart_shell(l, list(zip(rng.uniform(0.3, 2.0, K), rng.uniform(0.2, 1.0, K))),
+                                 tuple(rng.uniform(-0.8, 0.8, 3))))
+    v = emulate(prog, shells)
+    arrays = la.BasisArrays.from_shells(shells)
+    o, _, _ = oracle.eri_quartets(arrays, [[0, 1, 2, 3]], precision=0.0)
+    assert v.shape == o.shape
+    d = np.abs(v - o)
+    assert (d <= 1e-12 + 1e-10 * np.abs(o)).all(), d.max()
+
+
+def test_generic_programs_fit_lds():
+    for c in gen_eri.canonical_classes(3):
+        p = gen_generic.class_program(*c)
+        assert p["lds"] <= gen_generic.LDS_DOUBLES
+        assert p["chk"] >= 1 and p["chb"] >= 1

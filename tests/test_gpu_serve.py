@@ -111,7 +111,7 @@ def test_serve_memory_admission_and_failures():
         assert e.value.code == la.L2G_E_OOM
         st = srv.stats()
         assert st["max_requests_per_round"] == 1
-        fb = la.BasisArrays.from_shells([la.Shell(3, [(1.0, 1.0)])])
+        fb = la.BasisArrays.from_shells([la.Shell(la.max_l() + 1, [(1.0, 1.0)])])
         nf = int(fb.sizes().sum())
         with pytest.raises(la.L2gError) as e2:
             srv.wait(srv.submit(fb, np.eye(nf), 1e-12))
