@@ -27,9 +27,10 @@ everything in LDS, and reads the work of each class from data written here:
   are the contracted [e0|f0] inputs, and the workspace index of every
   Cartesian output.
 
-Records (int4): VRR node = two int4, {out, X, Y, Z} and {target or -1, mlen,
-coefficient slot (kind * 3 + axis), n_a | n_c << 4 | kind << 8}; HRR op =
-{dst, src0, src1, axis}.  Offsets are in doubles of the block's LDS.
+Records (int4): VRR node = {out | mlen << 15 | (kind * 3 + axis) << 19 |
+n_a << 22 | n_c << 25 | kind << 28, X | Y << 16, Z | (target + 1) << 16, 0}
+(offsets below 2^15 doubles, target 0 = none); HRR op = {dst, src0, src1,
+axis}.  Offsets are in doubles of the block's LDS.
 """
 try:  # as libint_amd.codegen.gen_generic
     from .gen_eri import cart, ncart, build_dir, sub1, hrr_program
@@ -82,9 +83,16 @@ def class_program(la, lb, lc, ld):
         levels.setdefault(sum(e) + sum(f), []).append((e, f))
     # LDS: [acc: ne nf contracted targets][zero][coefficients][3 level
     # regions]; after the primitive loop the regions are reused
-    region = max(sum(need[n] for n in levels[k]) for k in levels)
+    # strides are odd numbers of doubles: lanes of a wave walk neighbouring
+    # m-vectors / rows in lockstep, and an even stride folds them onto few
+    # of the LDS banks
+    def odd(n):
+        return n | 1
+
+    accs = odd(nf)
+    region = max(sum(odd(need[n]) for n in levels[k]) for k in levels)
     acc = 0
-    zero = ne * nf
+    zero = ne * accs
     coef = zero + ZERO_LEN
     reg0 = coef + NCOEF
     off = {}
@@ -92,17 +100,17 @@ def class_program(la, lb, lc, ld):
         o = reg0 + (k % 3) * region
         for n in nodes:
             off[n] = o
-            o += need[n]
+            o += odd(need[n])
     eidx = {e: i for i, e in enumerate(ecomps)}
     fidx = {f: j for j, f in enumerate(fcomps)}
 
     def target(e, f):
-        return eidx[e] * nf + fidx[f] if (e in eidx and f in fidx) else -1
+        return eidx[e] * accs + fidx[f] if (e in eidx and f in fidx) else -1
 
     recs = []
     lev_first = []
     for k in range(1, L + 1):
-        lev_first.append(len(recs) // 2)
+        lev_first.append(len(recs))
         for (e, f) in levels.get(k, []):
             if sum(f) > 0:
                 kind, i = 1, build_dir(f)
@@ -116,9 +124,13 @@ def class_program(la, lb, lc, ld):
                 y = (sub1(sub1(e, i), i), f) if e[i] >= 2 else None
                 zz = None
                 na, nc = e[i] - 1, 0
-            recs.append((off[(e, f)], off[x], off[y] if y else zero, off[zz] if zz else zero))
-            recs.append((target(e, f), need[(e, f)], kind * 3 + i, na | (nc << 4) | (kind << 8)))
-    lev_first.append(len(recs) // 2)
+            o, ox = off[(e, f)], off[x]
+            oy, oz = off[y] if y else zero, off[zz] if zz else zero
+            tg = target(e, f) + 1
+            assert max(o, ox, oy, oz) < (1 << 15) and tg < (1 << 16) and need[(e, f)] < 16
+            recs.append((o | (need[(e, f)] << 15) | ((kind * 3 + i) << 19) | (na << 22) | (nc << 25) | (kind << 28),
+                         ox | (oy << 16), oz | (tg << 16), 0))
+    lev_first.append(len(recs))
     base = off[(z, z)]
     m0len = need[(z, z)]
     targ0 = target(z, z)
@@ -141,15 +153,26 @@ def class_program(la, lb, lc, ld):
     # doubles follow Y.
     ncc = ncart(lc) * ncart(ld)
     nab = ncart(la) * ncart(lb)
+    ys = odd(ncc)
     lds_vrr = reg0 + 3 * region
-    y0 = acc + ne * nf
-    w0 = y0 + ne * ncc
-    wrow = hk_nws + ncc
-    chk = min(ne, 256, (LDS_DOUBLES - w0) // wrow)
+    y0 = acc + ne * accs
+    w0 = y0 + ne * ys
+    wrow = odd(hk_nws + ncc)
+    # rows / columns per pass: as many as fit under the larger of the
+    # recursion's own footprint and a quarter of the CU's LDS, so that small
+    # classes keep several blocks per CU, unless that leaves fewer than 32
+    # rows / columns per pass; then as many as the CU's LDS holds
+    cap = max(lds_vrr, LDS_DOUBLES // 4)
+
+    def per_pass(n, start, size):
+        capped = (cap - start) // size
+        return min(n, 256, capped if capped >= min(n, 32) else (LDS_DOUBLES - start) // size)
+
+    chk = per_pass(ne, w0, wrow)
     dg = nab + (ncart(la) + ncart(lb)) * (ncart(lc) + ncart(ld))
-    wcol = hb_nws + nab
+    wcol = odd(hb_nws + nab)
     wb0 = w0 + dg
-    chb = min(ncc, 256, (LDS_DOUBLES - wb0) // wcol)
+    chb = per_pass(ncc, wb0, wcol)
     if chk < 1 or chb < 1:
         raise ValueError(f"class ({la}{lb}|{lc}{ld}) does not fit {LDS_DOUBLES} doubles of LDS")
     lds = max(lds_vrr, w0 + chk * wrow, wb0 + chb * wcol)
@@ -161,13 +184,14 @@ def class_program(la, lb, lc, ld):
                 base=base, zero=zero, coef=coef, acc=acc, y0=y0, w0=w0, wrow=wrow, chk=chk,
                 f0=f0, wb0=wb0, wcol=wcol, chb=chb, lds=lds, recs=recs, lev_first=lev_first,
                 hk=hk, hk_out=hk_out, hk_nws=hk_nws, hb=hb, hb_out=hb_out, hb_nws=hb_nws,
-                diag=[i * nf + i for i in range(ne)] if (la, lb) == (lc, ld) else [])
+                accs=accs, ys=ys,
+                diag=[i * accs + i for i in range(ne)] if (la, lb) == (lc, ld) else [])
 
 
 FIELDS = ["la", "lb", "lc", "ld", "L", "ne", "nf", "m0len", "targ0", "base", "zero", "coef", "acc",
           "y0", "w0", "wrow", "chk", "f0", "wb0", "wcol", "chb", "lds", "vrr_first", "lev_first",
           "hk_first", "hk_nops", "hk_out", "hk_nws", "hb_first", "hb_nops", "hb_out", "hb_nws",
-          "diag_first", "ndiag", "nvals"]
+          "diag_first", "ndiag", "nvals", "accs", "ys"]
 
 
 def emit_tables(classes, lmax):
@@ -207,6 +231,7 @@ def emit_tables(classes, lmax):
     if not rows:
         out.append("  {" + ", ".join("0" for _ in FIELDS) + "},")
     out.append("};")
+    prog.append((0, 0, 0, 0))  # padding: a clamped prefetch never reads past the array
     out.append(f"extern const int l2g_gen_generic_nprog = {len(prog)};")
     out.append("extern const int l2g_gen_generic_prog[] = {")
     for i in range(0, len(prog), 4):

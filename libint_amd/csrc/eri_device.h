@@ -84,6 +84,7 @@ struct L2gGenericClass {
   int hb_first, hb_nops, hb_out, hb_nws;
   int diag_first, ndiag;
   int nvals;              // recursion values per primitive quartet
+  int accs, ys;           // row strides of the accumulators and of Y (odd)
 };
 
 struct L2gClassInfo {

@@ -21,7 +21,6 @@
 
 namespace {
 
-constexpr int kThreads = 256;
 constexpr int kModeEri = 0, kModeFock = 1, kModeSchwarz = 2;
 
 // solid-harmonic coefficients, standard order (m = -l..l), l <= 3:
@@ -80,12 +79,15 @@ __device__ void sph_second(const double* in, double* out, int n1, int l2, int pu
     }
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs A, const L2gGenericClass C,
-                                                                 const int4* __restrict__ prog,
-                                                                 const int* __restrict__ idx) {
+// NT threads per block: 1024 or 512 for classes whose LDS leaves room for
+// one or two blocks per CU, so that every SIMD still holds 4 waves to hide
+// the LDS and global-memory latencies of the recursion; 256 otherwise.
+template <int MODE, int NT>
+__global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, const L2gGenericClass C,
+                                                           const int4* __restrict__ prog, const int* __restrict__ idx) {
+  constexpr int kThreads = NT;
   extern __shared__ double lds[];
-  __shared__ double red[2 * kThreads / 64];
+  __shared__ double red[2 * NT / 64];
   const int t = threadIdx.x;
   const int ne = C.ne, nf = C.nf;
   const int la = C.la, lb = C.lb, lc = C.lc, ld = C.ld;
@@ -112,7 +114,7 @@ __global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs 
       pbeg = pr.x;
       pend = pr.y;
     }
-    for (int i = t; i < ne * nf; i += kThreads) lds[C.acc + i] = 0.0;
+    for (int i = t; i < ne * C.accs; i += kThreads) lds[C.acc + i] = 0.0;
     if (t < 16) lds[C.zero + t] = 0.0;
     __syncthreads();
     int nsurv = 0;
@@ -164,22 +166,41 @@ __global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs 
       }
       if (t < C.m0len) lds[C.base + t] = pf * boys_one(T, t, A.boys);
       __syncthreads();
+      // levels; a thread's next record is loaded (at a clamped, always valid
+      // index) while it computes the current one, and its first record of
+      // the next level before the barrier
+      int nd = 0, nend = 0;  // the class's level table has L + 1 entries
+      if (C.L > 0) {
+        nd = idx[C.lev_first] + t;
+        nend = idx[C.lev_first + 1];
+      }
+      int4 rc = prog[C.vrr_first + min(nd, max(nend - 1, 0))];
       for (int k = 1; k <= C.L; ++k) {
-        const int n0 = idx[C.lev_first + k - 1], n1 = idx[C.lev_first + k];
-        for (int nd = n0 + t; nd < n1; nd += kThreads) {
-          const int4 r0 = prog[C.vrr_first + 2 * nd];
-          const int4 r1 = prog[C.vrr_first + 2 * nd + 1];
-          const int na = r1.w & 15, nc = (r1.w >> 4) & 15, kind = r1.w >> 8;
-          const double cA = lds[C.coef + r1.z], cW = lds[C.coef + 6 + r1.z];
+        const int nnext = k < C.L ? idx[C.lev_first + k + 1] : 0;
+        const int cnt = nd < nend ? (nend - nd + kThreads - 1) / kThreads : 0;
+        for (int i = 0; i < cnt; ++i) {
+          const int ndn = nd + kThreads;
+          const int4 rn = prog[C.vrr_first + min(ndn, nend - 1)];
+          const int out = rc.x & 0x7FFF, mlen = (rc.x >> 15) & 15, slot = (rc.x >> 19) & 7;
+          const int na = (rc.x >> 22) & 7, nc = (rc.x >> 25) & 7, kind = (rc.x >> 28) & 1;
+          const double cA = lds[C.coef + slot], cW = lds[C.coef + 6 + slot];
           const double g = (double)na * lds[C.coef + 12 + kind], r = lds[C.coef + 14 + kind];
           const double h = (double)nc * lds[C.coef + 16];
-          const double* X = lds + r0.y;
-          const double* Y = lds + r0.z;
-          const double* Z = lds + r0.w;
-          double* O = lds + r0.x;
-          for (int m = 0; m < r1.y; ++m) O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
-          if (r1.x >= 0) lds[C.acc + r1.x] += O[0];
+          const double* X = lds + (rc.y & 0xFFFF);
+          const double* Y = lds + (rc.y >> 16);
+          const double* Z = lds + (rc.z & 0xFFFF);
+          double* O = lds + out;
+#pragma unroll 2
+          for (int m = 0; m < mlen; ++m) O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
+          const int tg = (rc.z >> 16) - 1;
+          if (tg >= 0) lds[C.acc + tg] += O[0];
+          rc = rn;
+          nd = ndn;
         }
+        // first record of level k + 1 for this thread
+        nd = nend + t;
+        nend = nnext;
+        if (k < C.L) rc = prog[C.vrr_first + min(nd, max(nend - 1, 0))];
         __syncthreads();
       }
       if (C.targ0 >= 0 && t == 0) lds[C.acc + C.targ0] += lds[C.base];
@@ -199,14 +220,14 @@ __global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs 
       if (t < C.chk && ie < ne) {
         double* ws = lds + C.w0 + t * C.wrow;
         double* tmp = ws + C.hk_nws;
-        for (int j = 0; j < nf; ++j) ws[j] = lds[C.acc + ie * nf + j];
+        for (int j = 0; j < nf; ++j) ws[j] = lds[C.acc + ie * C.accs + j];
         for (int o = 0; o < C.hk_nops; ++o) {
           const int4 op = prog[C.hk_first + o];
           ws[op.x] = ws[op.y] + CD[op.w] * ws[op.z];
         }
         for (int k = 0; k < ncc; ++k) tmp[k] = ws[idx[C.hk_out + k]];
         sph_first(tmp, ws, lc, nc4, pk1);
-        sph_second(ws, lds + C.y0 + ie * ncc, nk1, ld, pk2);
+        sph_second(ws, lds + C.y0 + ie * C.ys, nk1, ld, pk2);
       }
     }
     __syncthreads();
@@ -249,7 +270,7 @@ __global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs 
       if (t < C.chb && col < ncol) {
         double* ws = lds + C.wb0 + t * C.wcol;
         double* tmp = ws + C.hb_nws;
-        for (int j = 0; j < ne; ++j) ws[j] = lds[C.y0 + j * ncc + col];
+        for (int j = 0; j < ne; ++j) ws[j] = lds[C.y0 + j * C.ys + col];
         for (int o = 0; o < C.hb_nops; ++o) {
           const int4 op = prog[C.hb_first + o];
           ws[op.x] = ws[op.y] + AB[op.w] * ws[op.z];
@@ -353,28 +374,47 @@ __global__ void __launch_bounds__(kThreads) l2g_generic_kernel(const L2gEriArgs 
 
 namespace l2g_impl {
 
+template <int MODE>
+void set_lds_limit(int maxlds) {
+  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<MODE, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             maxlds));
+  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<MODE, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             maxlds));
+  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<MODE, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             maxlds));
+}
+
+template <int MODE>
+void launch_mode(const L2gEriArgs& A, const L2gGenericClass& C, const int4* prog, const int* idx, int grid,
+                 hipStream_t s) {
+  const size_t shm = (size_t)C.lds * sizeof(double);
+  const size_t per_cu = (160 * 1024) / (shm + 512);
+  if (per_cu <= 1)
+    hipLaunchKernelGGL((l2g_generic_kernel<MODE, 1024>), dim3(grid), dim3(1024), shm, s, A, C, prog, idx);
+  else if (per_cu <= 3)
+    hipLaunchKernelGGL((l2g_generic_kernel<MODE, 512>), dim3(grid), dim3(512), shm, s, A, C, prog, idx);
+  else
+    hipLaunchKernelGGL((l2g_generic_kernel<MODE, 256>), dim3(grid), dim3(256), shm, s, A, C, prog, idx);
+}
+
 void generic_init(const double* sh) {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sh), sh, sizeof(double) * 4 * 7 * 10));
   // 160 KB per CU on gfx950, less 512 bytes for the static reduction array
   // (codegen/gen_generic.py plans at most that many doubles)
   const int maxlds = 160 * 1024 - 512;
-  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<kModeEri>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
-  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<kModeFock>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
-  HIPCHK(hipFuncSetAttribute((const void*)l2g_generic_kernel<kModeSchwarz>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, maxlds));
+  set_lds_limit<kModeEri>(maxlds);
+  set_lds_limit<kModeFock>(maxlds);
+  set_lds_limit<kModeSchwarz>(maxlds);
 }
 
 void launch_generic(int mode, const L2gEriArgs& A, const L2gGenericClass& C, const int4* prog, const int* idx,
                     int grid, hipStream_t s) {
-  const size_t shm = (size_t)C.lds * sizeof(double);
   if (mode == kModeEri)
-    hipLaunchKernelGGL(l2g_generic_kernel<kModeEri>, dim3(grid), dim3(kThreads), shm, s, A, C, prog, idx);
+    launch_mode<kModeEri>(A, C, prog, idx, grid, s);
   else if (mode == kModeFock)
-    hipLaunchKernelGGL(l2g_generic_kernel<kModeFock>, dim3(grid), dim3(kThreads), shm, s, A, C, prog, idx);
+    launch_mode<kModeFock>(A, C, prog, idx, grid, s);
   else
-    hipLaunchKernelGGL(l2g_generic_kernel<kModeSchwarz>, dim3(grid), dim3(kThreads), shm, s, A, C, prog, idx);
+    launch_mode<kModeSchwarz>(A, C, prog, idx, grid, s);
   HIPCHK(hipGetLastError());
 }
 

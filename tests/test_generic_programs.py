@@ -22,7 +22,8 @@ def emulate(prog, shells):
     la_, lb_, lc_, ld_ = prog["la"], prog["lb"], prog["lc"], prog["ld"]
     ne, nf, L = prog["ne"], prog["nf"], prog["L"]
     lds = np.zeros(prog["lds"] + 64)
-    acc = np.zeros(ne * nf)
+    accs = prog["accs"]
+    acc = np.zeros(ne * accs)
     rec = prog["recs"]
     lev = prog["lev_first"]
 
@@ -53,15 +54,17 @@ def emulate(prog, shells):
             lds[prog["base"]:prog["base"] + prog["m0len"]] = pf * F[:prog["m0len"]]
             for k in range(1, L + 1):
                 for nd in range(lev[k - 1], lev[k]):
-                    r0, r1 = rec[2 * nd], rec[2 * nd + 1]
-                    na, nc, kind = r1[3] & 15, (r1[3] >> 4) & 15, r1[3] >> 8
-                    cA, cW = coef[r1[2]], coef[6 + r1[2]]
+                    x, y, z, _ = rec[nd]
+                    out, mlen, slot = x & 0x7FFF, (x >> 15) & 15, (x >> 19) & 7
+                    na, nc, kind = (x >> 22) & 7, (x >> 25) & 7, (x >> 28) & 1
+                    ox, oy, oz, tg = y & 0xFFFF, y >> 16, z & 0xFFFF, (z >> 16) - 1
+                    cA, cW = coef[slot], coef[6 + slot]
                     g, r, h = na * coef[12 + kind], coef[14 + kind], nc * coef[16]
-                    for m in range(r1[1]):
-                        lds[r0[0] + m] = (cA * lds[r0[1] + m] + cW * lds[r0[1] + m + 1]
-                                          + g * (lds[r0[2] + m] - r * lds[r0[2] + m + 1]) + h * lds[r0[3] + m + 1])
-                    if r1[0] >= 0:
-                        acc[r1[0]] += lds[r0[0]]
+                    for m in range(mlen):
+                        lds[out + m] = (cA * lds[ox + m] + cW * lds[ox + m + 1]
+                                        + g * (lds[oy + m] - r * lds[oy + m + 1]) + h * lds[oz + m + 1])
+                    if tg >= 0:
+                        acc[tg] += lds[out]
             if prog["targ0"] >= 0:
                 acc[prog["targ0"]] += lds[prog["base"]]
     CD, AB = C - D, A - B
@@ -69,7 +72,7 @@ def emulate(prog, shells):
     Y = np.zeros((ne, ncc))
     for ie in range(ne):
         ws = np.zeros(prog["hk_nws"])
-        ws[:nf] = acc[ie * nf:(ie + 1) * nf]
+        ws[:nf] = acc[ie * accs:ie * accs + nf]
         for (d, s0, s1, ax) in prog["hk"]:
             ws[d] = ws[s0] + CD[ax] * ws[s1]
         Y[ie] = ws[prog["hk_out"]]
