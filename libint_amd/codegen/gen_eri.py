@@ -875,7 +875,7 @@ def class_waves(plan, budget, waves):
     return waves
 
 
-def generate(outdir, lmax=2, budget=150, nfiles=8, waves=0, only=None, budgets=None,
+def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=None,
              modes=('eri', 'fock', 'schwarz', 'fp32'), generic_lmax=3):
     """Write the kernels and the dispatch tables.  `budgets` maps a class name
     ('2222') to its own (budget, waves)."""
@@ -981,7 +981,7 @@ if __name__ == '__main__':
     ap = argparse.ArgumentParser()
     ap.add_argument("outdir")
     ap.add_argument("--lmax", type=int, default=2)
-    ap.add_argument("--budget", type=int, default=150)
+    ap.add_argument("--budget", type=int, default=160)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--only", default="", help="comma-separated class names, e.g. 2222")
     ap.add_argument("--class-cfg", default="", help="per class budget:waves, e.g. 2222=64:2,2122=80:2")
