@@ -1,0 +1,227 @@
+// Screened Fock quartet list on the GPU: the permutationally-unique quartet
+// loop of compute_2body_fock (tests/hartree-fock/hartree-fock++.cc:1628-1696)
+// over the significant shell pairs, which are listed in (s1, s2)
+// lexicographic order.  In that order the unique quartets are exactly the
+// pairs of pairs (p1, p2) with p2 <= p1, so the loop is one triangular index
+// t -> (p1, p2) that every thread decodes on its own.
+//
+// Pass 1 counts the surviving quartets of each block's contiguous t range;
+// pass 2 writes them (screening Dnorm1234 K12 K34 >= precision, per-quartet
+// precision precision / Dnorm1234, degeneracy, canonical shell order and
+// kernel class as the host enumeration in l2g_fock.cpp) at the block's
+// offset, with a sort key (class, decreasing primitive-loop length);
+// a radix sort by that key then groups the classes and orders each by depth.
+#include <hipcub/hipcub.hpp>
+
+#include "l2g_internal.h"
+
+using namespace l2g_impl;
+
+namespace {
+
+constexpr int kEnumThreads = 256;
+
+__device__ __forceinline__ void decode_tri(long long t, long long& p1, long long& p2) {
+  long long a = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (a * (a + 1) / 2 > t) --a;
+  while ((a + 1) * (a + 2) / 2 <= t) ++a;
+  p1 = a;
+  p2 = t - a * (a + 1) / 2;
+}
+
+struct Quartet {
+  int s1, s2, s3, s4;
+  double Dn1234;
+};
+
+// screening of one unique quartet; true when it survives
+__device__ __forceinline__ bool screen(const l2g_impl::FockEnumArgs& E, long long p1, long long p2, Quartet& q) {
+  q.s1 = E.ps1[p1], q.s2 = E.ps2[p1], q.s3 = E.ps1[p2], q.s4 = E.ps2[p2];
+  const long long n = E.n;
+  const double* Dn = E.Dn;
+  const double d = fmax(fmax(fmax(Dn[q.s1 * n + q.s2], Dn[q.s3 * n + q.s4]), fmax(Dn[q.s1 * n + q.s3], Dn[q.s2 * n + q.s3])),
+                        fmax(Dn[q.s1 * n + q.s4], Dn[q.s2 * n + q.s4]));
+  q.Dn1234 = d;
+  return !(d * E.pK[p1] * E.pK[p2] < E.precision);
+}
+
+__global__ void __launch_bounds__(kEnumThreads) enum_count(const l2g_impl::FockEnumArgs E, long long total,
+                                                              long long chunk, long long* __restrict__ counts) {
+  __shared__ long long red[kEnumThreads / 64];
+  const long long t0 = (long long)blockIdx.x * chunk, t1 = min(total, t0 + chunk);
+  long long c = 0;
+  for (long long t = t0 + threadIdx.x; t < t1; t += kEnumThreads) {
+    long long p1, p2;
+    decode_tri(t, p1, p2);
+    Quartet q;
+    c += screen(E, p1, p2, q) ? 1 : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long s = 0;
+    for (int i = 0; i < kEnumThreads / 64; ++i) s += red[i];
+    counts[blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kEnumThreads) enum_fill(const l2g_impl::FockEnumArgs E, long long total,
+                                                             long long chunk, const long long* __restrict__ base,
+                                                             l2g_impl::FockEnumOut O) {
+  __shared__ unsigned int pos;
+  __shared__ unsigned int hist[128];
+  __shared__ unsigned long long whist[128];
+  if (threadIdx.x == 0) pos = 0;
+  if (threadIdx.x < 128) hist[threadIdx.x] = 0, whist[threadIdx.x] = 0;
+  __syncthreads();
+  const long long t0 = (long long)blockIdx.x * chunk, t1 = min(total, t0 + chunk);
+  const long long b0 = base[blockIdx.x];
+  for (long long t = t0 + threadIdx.x; t < t1; t += kEnumThreads) {
+    long long p1, p2;
+    decode_tri(t, p1, p2);
+    Quartet q;
+    if (!screen(E, p1, p2, q)) continue;
+    const int s1 = q.s1, s2 = q.s2, s3 = q.s3, s4 = q.s4;
+    const double qprec = q.Dn1234 != 0.0 ? E.precision / q.Dn1234 : E.needed;
+    const int deg = (s1 == s2 ? 1 : 2) * (s3 == s4 ? 1 : 2) * (s1 == s3 ? (s2 == s4 ? 1 : 2) : 2);
+    const int l1 = E.l[s1], l2 = E.l[s2], l3 = E.l[s3], l4 = E.l[s4];
+    const bool stb = E.pswap[p1] != 0, stk = E.pswap[p2] != 0, sbk = (l1 + l2) > (l3 + l4);
+    const int b1 = sbk ? (stk ? s4 : s3) : (stb ? s2 : s1);
+    const int b2 = sbk ? (stk ? s3 : s4) : (stb ? s1 : s2);
+    const int k1 = sbk ? (stb ? s2 : s1) : (stk ? s4 : s3);
+    const int k2 = sbk ? (stb ? s1 : s2) : (stk ? s3 : s4);
+    const int bp = sbk ? E.pid[p2] : E.pid[p1], kp = sbk ? E.pid[p1] : E.pid[p2];
+    const int lk = ((E.l[b1] * 4 + E.l[b2]) * 4 + E.l[k1]) * 4 + E.l[k2];
+    int cls = E.clsmap[lk];  // every class up to the compiled l has a kernel; 127 flags none
+    if (E.fp32_thr > 0 && E.pK[p1] * E.pK[p2] < E.fp32_thr && E.pS[p1] * E.pS[p2] < E.hrr_factor * E.fp32_thr &&
+        E.cls32map[lk] >= 0)
+      cls = E.cls32map[lk];
+    if (cls < 0 || cls > 126) cls = 127;
+    auto pb = [&](int sh) { return (E.pure[sh] && E.l[sh] > 0) ? 1 : 0; };
+    const int w = E.np[bp] * E.np[kp];
+    const long long i = b0 + atomicAdd(&pos, 1u);
+    O.bra[i] = bp;
+    O.ket[i] = kp;
+    O.meta[i] = (pb(b1) << 3) | (pb(b2) << 4) | (pb(k1) << 5) | (pb(k2) << 6);
+    O.lnp[i] = qprec > 0 ? log(qprec) : -1.7976931348623157e308;
+    O.scale[i] = (double)deg;
+    O.bf[i] = int4{E.bf1st[b1], E.bf1st[b2], E.bf1st[k1], E.bf1st[k2]};
+    O.pr[i] = int2{0, w};
+    O.key[i] = ((unsigned)cls << 13) | (unsigned)(8191 - min(w, 8191));  // 20 bits
+    O.idx[i] = (unsigned)i;
+    atomicAdd(&hist[cls], 1u);
+    atomicAdd(&whist[cls], (unsigned long long)w);
+  }
+  __syncthreads();
+  if (threadIdx.x < 128 && hist[threadIdx.x]) {
+    atomicAdd(&O.cls_count[threadIdx.x], (unsigned long long)hist[threadIdx.x]);
+    atomicAdd(&O.cls_nprim[threadIdx.x], whist[threadIdx.x]);
+  }
+}
+
+template <class T>
+__global__ void gather_kernel(const T* __restrict__ in, const unsigned* __restrict__ perm, T* __restrict__ out,
+                              long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[perm[i]];
+}
+
+template <class T>
+void gather(const T* in, const unsigned* perm, T* out, long long n, hipStream_t s) {
+  if (n == 0) return;
+  const int block = 256;
+  hipLaunchKernelGGL(gather_kernel<T>, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, s, in, perm, out, n);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+namespace l2g_impl {
+
+void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st) {
+  const long long np = E.npairs;
+  const long long total = np * (np + 1) / 2;
+  S.n = 0;
+  S.cls_count.assign(128, 0ull);
+  S.cls_nprim.assign(128, 0ull);
+  if (total == 0) return;
+  const int nblocks = (int)std::min<long long>(8192, (total + kEnumThreads * 64 - 1) / (kEnumThreads * 64));
+  const long long chunk = (total + nblocks - 1) / nblocks;
+  S.counts.alloc((size_t)nblocks * sizeof(long long));
+  hipLaunchKernelGGL(enum_count, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.counts.as<long long>());
+  HIPCHK(hipGetLastError());
+  std::vector<long long> cnt(nblocks), base(nblocks);
+  HIPCHK(hipMemcpyAsync(cnt.data(), S.counts.p, (size_t)nblocks * sizeof(long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  long long n = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    base[b] = n;
+    n += cnt[b];
+  }
+  S.n = n;
+  if (n == 0) return;
+  if (n >= (1ll << 31)) throw L2gError(L2G_E_INVALID, "more than 2^31 screened quartets in one Fock build");
+  S.base.alloc((size_t)nblocks * sizeof(long long));
+  HIPCHK(hipMemcpyAsync(S.base.p, base.data(), (size_t)nblocks * sizeof(long long), hipMemcpyHostToDevice, st));
+  // unsorted items, then sorted
+  const size_t sz = (size_t)n;
+  DevBuf u_bra, u_ket, u_meta, u_lnp, u_scale, u_bf, u_pr, u_key, u_idx, s_key, stats;
+  u_bra.alloc(sz * sizeof(int));
+  u_ket.alloc(sz * sizeof(int));
+  u_meta.alloc(sz * sizeof(int));
+  u_lnp.alloc(sz * sizeof(double));
+  u_scale.alloc(sz * sizeof(double));
+  u_bf.alloc(sz * sizeof(int4));
+  u_pr.alloc(sz * sizeof(int2));
+  u_key.alloc(sz * sizeof(unsigned));
+  u_idx.alloc(sz * sizeof(unsigned));
+  s_key.alloc(sz * sizeof(unsigned));
+  stats.alloc(256 * sizeof(unsigned long long));
+  HIPCHK(hipMemsetAsync(stats.p, 0, 256 * sizeof(unsigned long long), st));
+  FockEnumOut O;
+  O.bra = u_bra.as<int>();
+  O.ket = u_ket.as<int>();
+  O.meta = u_meta.as<int>();
+  O.lnp = u_lnp.as<double>();
+  O.scale = u_scale.as<double>();
+  O.bf = u_bf.as<int4>();
+  O.pr = u_pr.as<int2>();
+  O.key = u_key.as<unsigned>();
+  O.idx = u_idx.as<unsigned>();
+  O.cls_count = stats.as<unsigned long long>();
+  O.cls_nprim = stats.as<unsigned long long>() + 128;
+  hipLaunchKernelGGL(enum_fill, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.base.as<long long>(), O);
+  HIPCHK(hipGetLastError());
+  // sort the item indices by (class, decreasing depth)
+  S.perm.alloc(sz * sizeof(unsigned));
+  size_t tmp_bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, O.key, s_key.as<unsigned>(), O.idx,
+                                            S.perm.as<unsigned>(), (int)n, 0, 20, st));
+  DevBuf tmp;
+  tmp.alloc(std::max<size_t>(tmp_bytes, 16));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, O.key, s_key.as<unsigned>(), O.idx,
+                                            S.perm.as<unsigned>(), (int)n, 0, 20, st));
+  const unsigned* perm = S.perm.as<unsigned>();
+  S.bra.alloc(sz * sizeof(int));
+  S.ket.alloc(sz * sizeof(int));
+  S.meta.alloc(sz * sizeof(int));
+  S.lnp.alloc(sz * sizeof(double));
+  S.scale.alloc(sz * sizeof(double));
+  S.bf.alloc(sz * sizeof(int4));
+  S.pr.alloc(sz * sizeof(int2));
+  gather(O.bra, perm, S.bra.as<int>(), n, st);
+  gather(O.ket, perm, S.ket.as<int>(), n, st);
+  gather(O.meta, perm, S.meta.as<int>(), n, st);
+  gather(O.lnp, perm, S.lnp.as<double>(), n, st);
+  gather(O.scale, perm, S.scale.as<double>(), n, st);
+  gather(O.bf, perm, S.bf.as<int4>(), n, st);
+  gather(O.pr, perm, S.pr.as<int2>(), n, st);
+  std::vector<unsigned long long> h(256);
+  HIPCHK(hipMemcpyAsync(h.data(), stats.p, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));  // the temporaries are freed on return
+  for (int c = 0; c < 128; ++c) S.cls_count[c] = h[c], S.cls_nprim[c] = h[128 + c];
+  if (S.cls_count[127]) throw L2gError(L2G_E_LMAX, "no Fock kernel for an angular momentum class");
+}
+
+}  // namespace l2g_impl

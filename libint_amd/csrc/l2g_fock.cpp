@@ -472,10 +472,26 @@ L2gEriArgs fock_args(l2g_context_s* ctx, int method, const int* pair_first, cons
 // Concatenated device copy of the work items of all classes, with the class
 // ranges; the arrays are uploaded from host vectors that must stay alive
 // until the stream has passed the copies.
+// Device work items of one launch round: arrays and per-class ranges.
+struct ItemsView {
+  const int *bra = nullptr, *ket = nullptr, *meta = nullptr, *req = nullptr;
+  const double *lnp = nullptr, *scale = nullptr;
+  const int4* bf = nullptr;
+  const int2* pr = nullptr;
+  std::vector<int64_t> first, count;  // per class
+};
+
 struct ItemsDev {
   DevBuf bra, ket, meta, req, lnp, scale, bf, pr;
   std::vector<int64_t> first, count;  // per class
   FockItems host;                      // staging (kept until the stream is synchronised)
+  ItemsView view() const {
+    ItemsView v;
+    v.bra = bra.as<int>(), v.ket = ket.as<int>(), v.meta = meta.as<int>(), v.req = req.as<int>();
+    v.lnp = lnp.as<double>(), v.scale = scale.as<double>(), v.bf = bf.as<int4>(), v.pr = pr.as<int2>();
+    v.first = first, v.count = count;
+    return v;
+  }
   void upload_from(const std::vector<FockItems>& cls, hipStream_t st) {
     host.clear();
     first.assign(cls.size(), 0);
@@ -500,20 +516,20 @@ struct ItemsDev {
 // grid and scratch as sized by the caller.
 // With `ev` (2 events per class), records the class's launch between
 // ev[2c] and ev[2c+1].
-void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsDev& items, bool with_req, int grid, int block,
+void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, bool with_req, int grid, int block,
                      hipStream_t st, hipEvent_t* ev = nullptr) {
   for (size_t c = 0; c < items.count.size(); ++c) {
     const int64_t m = items.count[c], o = items.first[c];
     if (m == 0) continue;
     if (ev) HIPCHK(hipEventRecord(ev[2 * c], st));
-    A.bra_pair = items.bra.as<int>() + o;
-    A.ket_pair = items.ket.as<int>() + o;
-    A.meta = items.meta.as<int>() + o;
-    A.q_lnprec = items.lnp.as<double>() + o;
-    A.q_scale = items.scale.as<double>() + o;
-    A.q_bf = items.bf.as<int4>() + o;
-    A.q_req = with_req ? items.req.as<int>() + o : nullptr;
-    A.q_prange = items.pr.as<int2>() + o;
+    A.bra_pair = items.bra + o;
+    A.ket_pair = items.ket + o;
+    A.meta = items.meta + o;
+    A.q_lnprec = items.lnp + o;
+    A.q_scale = items.scale + o;
+    A.q_bf = items.bf + o;
+    A.q_req = with_req ? items.req + o : nullptr;
+    A.q_prange = items.pr + o;
     A.n = (int)m;
     if (fock_is_generic((int)c)) {
       launch_generic(kGenericFock, A, fock_gclass((int)c), ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
@@ -535,7 +551,12 @@ struct l2g_fock_s {
   float last_ms = 0;
   PairDev pairs;  // resident shell-pair data
   DevBuf dens, jg, kg, mats, ybuf;
-  ItemsDev items;
+  ItemsDev items;  // split items of the classes with few quartets
+  // GPU quartet enumeration: pair list in (s1, s2) order, per-shell data,
+  // class maps, shell-block norms of D and the sorted items
+  DevBuf e_ps1, e_ps2, e_pK, e_pS, e_pid, e_pswap, e_l, e_pure, e_bf1st, e_np, e_cls, e_cls32, e_Dn;
+  int64_t npairs = 0;
+  FockEnumState enumst;
   std::vector<hipEvent_t> ev;     // per class: start, end of the last build
   std::vector<int64_t> cls_nq, cls_nprim;
   ~l2g_fock_s() {
@@ -543,6 +564,61 @@ struct l2g_fock_s {
       if (e) (void)hipEventDestroy(e);
   }
 };
+
+namespace {
+
+// Device tables of the GPU quartet enumeration (l2g_enum.hip): the
+// significant pairs in (s1, s2) order with their Schwarz and FP32-selection
+// factors, canonical pair ids and orientation, per-shell data, and the map
+// from a canonical class to its Fock kernel class.
+void upload_enum_tables(l2g_fock_s& f, hipStream_t st) {
+  const FockSetup& fs = f.setup;
+  l2g_basis_s* B = fs.B;
+  l2g_context_s* ctx = B->ctx;
+  const int n = fs.n;
+  std::vector<int> ps1, ps2, pid;
+  std::vector<double> pK, pS;
+  std::vector<unsigned char> pswap;
+  for (int s1 = 0; s1 < n; ++s1)
+    for (int s2 : fs.splist[s1]) {
+      ps1.push_back(s1);
+      ps2.push_back(s2);
+      pK.push_back(fs.K[(size_t)s1 * n + s2]);
+      pS.push_back(fs.S[(size_t)s1 * n + s2]);
+      pid.push_back(fs.pair_of[(size_t)s1 * n + s2]);
+      pswap.push_back(B->swap_pair(s1, s2) ? 1 : 0);
+    }
+  f.npairs = (int64_t)ps1.size();
+  std::vector<int> cls(256, -1), cls32(256, -1);
+  const int nf64 = l2g_gen_nfock, nf32 = l2g_gen_nfock32;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b)
+      for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 4; ++d) {
+          const int k = ((a * 4 + b) * 4 + c) * 4 + d;
+          const int spec = ctx->force_generic ? -1 : ctx->fock_of[a][b][c][d];
+          const int g = ctx->generic_of[a][b][c][d];
+          cls[k] = spec >= 0 ? spec : (g >= 0 ? nf64 + nf32 + g : -1);
+          const int c32 = ctx->force_generic ? -1 : ctx->fock32_of[a][b][c][d];
+          cls32[k] = c32 >= 0 ? nf64 + c32 : -1;
+        }
+  if (nfock_classes() > 127) throw L2gError(L2G_E_STATE, "too many Fock kernel classes for the enumeration sort key");
+  upload(f.e_ps1, ps1, st);
+  upload(f.e_ps2, ps2, st);
+  upload(f.e_pK, pK, st);
+  upload(f.e_pS, pS, st);
+  upload(f.e_pid, pid, st);
+  upload(f.e_pswap, pswap, st);
+  upload(f.e_l, B->l, st);
+  upload(f.e_pure, B->pure, st);
+  upload(f.e_bf1st, fs.bf1st, st);
+  upload(f.e_np, fs.np, st);
+  upload(f.e_cls, cls, st);
+  upload(f.e_cls32, cls32, st);
+  HIPCHK(hipStreamSynchronize(st));  // the host vectors go out of scope
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -553,6 +629,7 @@ int l2g_fock_create(l2g_basis B, double pair_threshold, double max_engine_precis
     auto f = std::make_unique<l2g_fock_s>();
     fock_setup(f->setup, B, pair_threshold, max_engine_precision, screening);
     f->pairs.upload_from(f->setup, B->ctx->stream);
+    upload_enum_tables(*f, B->ctx->stream);
     const L2gMatDesc md{0, f->setup.nbf};
     f->mats.alloc(sizeof(md));
     HIPCHK(hipMemcpyAsync(f->mats.p, &md, sizeof(md), hipMemcpyHostToDevice, B->ctx->stream));
@@ -582,13 +659,97 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     const FockSetup& fs = f->setup;
     l2g_context_s* ctx = fs.B->ctx;
     HIPCHK(hipSetDevice(ctx->device));
-    const int N = fs.nbf;
-    FockWork w;
-    fock_enumerate(fs, D, precision, f->fp32_threshold, w);
-    f->last_nq = w.nq;
-    f->last_nprim = w.nprim;
-    f->last_nq32 = w.nq32;
+    const int n = fs.n, N = fs.nbf;
     hipStream_t st = ctx->stream;
+    // shell-block infinity norms of D (compute_shellblock_norm)
+    std::vector<double> Dn((size_t)n * n, 0.0);
+    parallel_for(n, [&](int s1) {
+      for (int s2 = 0; s2 < n; ++s2) {
+        double m = 0;
+        for (int i = fs.bf1st[s1]; i < fs.bf1st[s1 + 1]; ++i)
+          for (int j = fs.bf1st[s2]; j < fs.bf1st[s2 + 1]; ++j) m = std::max(m, std::fabs(D[(size_t)i * N + j]));
+        Dn[(size_t)s1 * n + s2] = m;
+      }
+    });
+    double dmax = 0;
+    for (double v : Dn) dmax = std::max(dmax, v);
+    const double needed = dmax > 0 ? precision / dmax : precision;
+    if (needed < fs.max_engine_precision)
+      throw std::invalid_argument("requested Fock precision needs more engine precision than the shell-pair data hold");
+    upload(f->e_Dn, Dn, st);
+    // screened unique quartets on the GPU, sorted by class and depth
+    FockEnumArgs E;
+    E.ps1 = f->e_ps1.as<int>();
+    E.ps2 = f->e_ps2.as<int>();
+    E.pK = f->e_pK.as<double>();
+    E.pS = f->e_pS.as<double>();
+    E.pid = f->e_pid.as<int>();
+    E.pswap = f->e_pswap.as<unsigned char>();
+    E.l = f->e_l.as<int>();
+    E.pure = f->e_pure.as<int>();
+    E.bf1st = f->e_bf1st.as<int>();
+    E.np = f->e_np.as<int>();
+    E.Dn = f->e_Dn.as<double>();
+    E.n = n;
+    E.npairs = f->npairs;
+    E.clsmap = f->e_cls.as<int>();
+    E.cls32map = f->e_cls32.as<int>();
+    E.precision = precision;
+    E.needed = needed;
+    E.fp32_thr = f->fp32_threshold;
+    E.hrr_factor = L2G_FP32_HRR_FACTOR;
+    FockEnumState& es = f->enumst;
+    fock_enumerate_device(E, es, st);  // synchronises; Dn may go
+    const int ncls = nfock_classes();
+    ItemsView big;
+    big.bra = es.bra.as<int>(), big.ket = es.ket.as<int>(), big.meta = es.meta.as<int>();
+    big.lnp = es.lnp.as<double>(), big.scale = es.scale.as<double>(), big.bf = es.bf.as<int4>();
+    big.pr = es.pr.as<int2>();
+    big.first.assign(ncls, 0);
+    big.count.assign(ncls, 0);
+    f->last_nq = es.n;
+    f->last_nq32 = 0;
+    f->last_nprim = 0;
+    f->cls_nq.assign(ncls, 0);
+    f->cls_nprim.assign(ncls, 0);
+    // classes with few items come back to the host to be split by
+    // primitive range (split_deep) and go up again as a second item set
+    std::vector<FockItems> small(ncls);
+    int64_t o = 0;
+    for (int c = 0; c < ncls; ++c) {
+      const int64_t m = (int64_t)es.cls_count[c];
+      f->cls_nq[c] = m;
+      f->cls_nprim[c] = (int64_t)es.cls_nprim[c];
+      f->last_nprim += f->cls_nprim[c];
+      if (c >= l2g_gen_nfock && !fock_is_generic(c)) f->last_nq32 += m;
+      big.first[c] = o;
+      if (m > 0 && m < (fock_is_generic(c) ? 4096 : (1 << 17))) {
+        FockItems& it = small[c];
+        it.bra.resize(m), it.ket.resize(m), it.meta.resize(m), it.lnp.resize(m), it.scale.resize(m), it.bf.resize(m),
+            it.pr.resize(m);
+        auto down = [&](void* dst, const DevBuf& b, size_t elem) {
+          HIPCHK(hipMemcpyAsync(dst, b.as<char>() + (size_t)o * elem, (size_t)m * elem, hipMemcpyDeviceToHost, st));
+        };
+        down(it.bra.data(), es.bra, sizeof(int));
+        down(it.ket.data(), es.ket, sizeof(int));
+        down(it.meta.data(), es.meta, sizeof(int));
+        down(it.lnp.data(), es.lnp, sizeof(double));
+        down(it.scale.data(), es.scale, sizeof(double));
+        down(it.bf.data(), es.bf, sizeof(int4));
+        down(it.pr.data(), es.pr, sizeof(int2));
+      } else {
+        big.count[c] = m;
+      }
+      o += m;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    parallel_for(ncls, [&](int c) {
+      if (small[c].size() == 0) return;
+      split_deep(small[c], c);
+      sort_by_depth(small[c]);
+    });
+    f->items.upload_from(small, st);
+    const ItemsView sv = f->items.view();
     const size_t NN = (size_t)N * N;
     f->dens.alloc(std::max<size_t>(NN, 1) * sizeof(double));
     f->jg.alloc(std::max<size_t>(NN, 1) * sizeof(double));
@@ -598,14 +759,14 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       HIPCHK(hipMemsetAsync(f->jg.p, 0, NN * sizeof(double), st));
       HIPCHK(hipMemsetAsync(f->kg.p, 0, NN * sizeof(double), st));
     }
-    f->items.upload_from(w.cls, st);
     const int block = 128;
     int64_t maxn = 0, ymax = 0;
-    for (int c = 0; c < nfock_classes(); ++c)
-      if (!w.cls[c].bra.empty()) {
-        maxn = std::max<int64_t>(maxn, (int64_t)w.cls[c].size());
-        ymax = std::max<int64_t>(ymax, fock_ybuf(c));
-      }
+    for (int c = 0; c < ncls; ++c) {
+      const int64_t m = std::max(big.count[c], sv.count[c]);
+      if (m == 0) continue;
+      maxn = std::max(maxn, m);
+      ymax = std::max<int64_t>(ymax, fock_ybuf(c));
+    }
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
     const int64_t ystride = (int64_t)grid * block;
     f->ybuf.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
@@ -614,23 +775,14 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
                              p.PA.as<double4>(), p.aux.as<double2>(), p.ns.as<double>(), f->dens.as<double>(),
                              f->jg.as<double>(), f->kg.as<double>(), f->mats.as<L2gMatDesc>(), f->ybuf.as<double>(),
                              ystride);
-    const int ncls = nfock_classes();
     if (f->ev.empty()) {
       f->ev.assign(2 * ncls + 2, nullptr);
       for (auto& e : f->ev) HIPCHK(hipEventCreate(&e));
     }
-    f->cls_nq.assign(ncls, 0);
-    f->cls_nprim.assign(ncls, 0);
-    for (int c = 0; c < ncls; ++c) {
-      const FockItems& it = w.cls[c];
-      for (size_t i = 0; i < it.size(); ++i) {
-        f->cls_nq[c] += it.pr[i].x == 0;  // one range of each quartet starts at 0
-        f->cls_nprim[c] += it.pr[i].y - it.pr[i].x;
-      }
-    }
     hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
     HIPCHK(hipEventRecord(e0, st));
-    fock_launch_all(ctx, A, f->items, false, grid, block, st, f->ev.data());
+    fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data());
+    fock_launch_all(ctx, A, sv, false, grid, block, st, f->ev.data());
     HIPCHK(hipEventRecord(e1, st));
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);
@@ -1043,7 +1195,7 @@ void run_round(l2g_server_s* S, const std::vector<ServerRequest*>& act) {
                            S->a_P.as<double4>(), S->a_PA.as<double4>(), S->a_aux.as<double2>(), S->a_ns.as<double>(),
                            S->a_dens.as<double>(), S->a_jg.as<double>(), S->a_kg.as<double>(),
                            S->rmats[b].as<L2gMatDesc>(), S->ybuf.as<double>(), S->ystride);
-  fock_launch_all(ctx, A, S->rbuf[b], true, S->grid, S->block, st);
+  fock_launch_all(ctx, A, S->rbuf[b].view(), true, S->grid, S->block, st);
   HIPCHK(hipEventRecord(S->rev[b], st));
   // requests whose last quartets are in this round: finish on the device and
   // copy J, K, G to their pinned host buffers

@@ -231,6 +231,40 @@ void launch_generic(int mode, const L2gEriArgs& A, const L2gGenericClass& C, con
                     int grid, hipStream_t s);
 inline int lmax_supported() { return std::max(l2g_gen_lmax, l2g_gen_generic_lmax); }
 
+// l2g_enum.hip: the screened unique-quartet list of a Fock build on the GPU
+struct FockEnumArgs {
+  const int* ps1;               // significant pairs in (s1, s2) order, s2 <= s1
+  const int* ps2;
+  const double* pK;             // Schwarz factor of each pair
+  const double* pS;             // FP32-selection factor of each pair
+  const int* pid;               // canonical shell-pair id (kernel pair data)
+  const unsigned char* pswap;   // 1 when the canonical orientation is (s2, s1)
+  const int* l;                 // per shell
+  const int* pure;
+  const int* bf1st;
+  const int* np;                // primitive pairs per canonical pair id
+  const double* Dn;             // n x n shell-block max norms of D
+  long long n, npairs;
+  const int* clsmap;            // [4][4][4][4] canonical class -> Fock kernel class (FP64)
+  const int* cls32map;          // -> FP32 kernel class or -1
+  double precision, needed, fp32_thr, hrr_factor;
+};
+struct FockEnumOut {
+  int *bra, *ket, *meta;
+  double *lnp, *scale;
+  int4* bf;
+  int2* pr;
+  unsigned *key, *idx;
+  unsigned long long *cls_count, *cls_nprim;
+};
+// the sorted items (class-major, decreasing depth) of the last enumeration
+struct FockEnumState {
+  long long n = 0;
+  std::vector<unsigned long long> cls_count, cls_nprim;  // 128 classes
+  DevBuf counts, base, perm, bra, ket, meta, lnp, scale, bf, pr;
+};
+void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st);
+
 
 // Shell-pair data for ordered pair (i, j), restating ShellPair::init
 // (include/libint2/shell.h:1137-1256) for Original and Conservative
