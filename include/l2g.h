@@ -166,6 +166,20 @@ int l2g_eri_plan_device_output(l2g_eri_plan plan, double** d_out);
 int l2g_eri_plan_stats(l2g_eri_plan plan, int cap, int* nclasses, int* class_code, int64_t* nquartets,
                        int64_t* nprim_quartets, float* ms);
 
+/* Graph mode (on != 0): the next execution captures the plan's class
+ * launches on the context stream into a hipGraph, which later executions
+ * into the same output pointer replay with one hipGraphLaunch (a new output
+ * pointer re-captures).  The graph holds kernel nodes only.  Per-class
+ * timings in l2g_eri_plan_stats are then -1; l2g_eri_plan_elapsed gives the
+ * whole execution.  No counterpart in the reference, whose callers loop over
+ * Engine::compute per shell quartet (engine.impl.h:140-178); it removes the
+ * per-class launch overhead of repeated executions of one plan. */
+int l2g_eri_plan_set_graph(l2g_eri_plan plan, int on);
+
+/* Duration in ms of the plan's last execution (all classes; HIP events on
+ * the context stream; waits for it to finish). */
+int l2g_eri_plan_elapsed(l2g_eri_plan plan, float* ms);
+
 int l2g_eri_plan_destroy(l2g_eri_plan plan);
 
 /* Schwarz factors K(s1,s2) = sqrt(max|(s1 s2|s1 s2)|) (norm = 0) or

@@ -63,6 +63,8 @@ ABI = {
     "l2g_eri_plan_download": (ctypes.c_int, [_vp, _vp, _vp]),
     "l2g_eri_plan_device_output": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "l2g_eri_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
+    "l2g_eri_plan_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_eri_plan_elapsed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
     "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
     "l2g_schwarz": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _f64p]),
@@ -357,6 +359,17 @@ class EriPlan:
         return [dict(cls=f"{c % 10000:04d}", fp32=bool(c // 10000 == 1), generic=bool(c // 10000 == 2),
                      nquartets=int(a), nprim=int(b), ms=float(t))
                 for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
+
+    def set_graph(self, on=True):
+        """Capture the class launches into a hipGraph on the next execution
+        and replay it on later executions into the same output buffer."""
+        _check(lib().l2g_eri_plan_set_graph(self._h, int(bool(on))))
+
+    def elapsed_ms(self):
+        """Duration of the last execution (all classes), waiting for it."""
+        t = ctypes.c_float()
+        _check(lib().l2g_eri_plan_elapsed(self._h, ctypes.byref(t)))
+        return float(t.value)
 
     def _release(self):
         if self._h:

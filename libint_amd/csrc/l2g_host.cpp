@@ -105,11 +105,26 @@ struct l2g_eri_plan_s {
   DevBuf out, ybuf, nz;
   int64_t ystride = 0;
   int grid = 0, block = 128;
+  // hipGraph mode: the class launches are captured once per output buffer
+  // and replayed; per-class timings are then not recorded
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double* graph_out = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // whole execution
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr, graph = nullptr, graph_out = nullptr;
+  }
   ~l2g_eri_plan_s() {
+    drop_graph();
     for (auto& c : classes) {
       if (c.ev0) (void)hipEventDestroy(c.ev0);
       if (c.ev1) (void)hipEventDestroy(c.ev1);
     }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
   }
 };
 
@@ -551,6 +566,24 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     A.ystride = p->ystride;
     p->nz.alloc(std::max<int64_t>(p->nq, 1));
     A.nz = p->nz.as<signed char>();
+    if (!p->t0) {
+      HIPCHK(hipEventCreate(&p->t0));
+      HIPCHK(hipEventCreate(&p->t1));
+    }
+    const bool capture = p->use_graph && !(p->gexec && p->graph_out == out);
+    if (p->use_graph && !capture) {  // replay
+      HIPCHK(hipEventRecord(p->t0, ctx->stream));
+      HIPCHK(hipGraphLaunch(p->gexec, ctx->stream));
+      HIPCHK(hipEventRecord(p->t1, ctx->stream));
+      return;
+    }
+    if (capture) {
+      p->drop_graph();
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    } else {
+      HIPCHK(hipEventRecord(p->t0, ctx->stream));
+    }
+    auto launch_all = [&] {
     for (auto& c : p->classes) {
       A.bra_pair = p->bra.as<int>() + c.first;
       A.ket_pair = p->ket.as<int>() + c.first;
@@ -558,7 +591,7 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       A.out_off = p->off.as<long long>() + c.first;
       A.qidx = p->qidx.as<long long>() + c.first;
       A.n = (int)c.count;
-      HIPCHK(hipEventRecord(c.ev0, ctx->stream));
+      if (!capture) HIPCHK(hipEventRecord(c.ev0, ctx->stream));
       if (c.generic) {
         // one block per quartet, persistent over the class
         const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
@@ -570,8 +603,47 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
         ci.launch(A, grid, p->block, ctx->stream);
         HIPCHK(hipGetLastError());
       }
-      HIPCHK(hipEventRecord(c.ev1, ctx->stream));
+      if (!capture) HIPCHK(hipEventRecord(c.ev1, ctx->stream));
     }
+    };
+    if (!capture) {
+      launch_all();
+      HIPCHK(hipEventRecord(p->t1, ctx->stream));
+      return;
+    }
+    try {
+      launch_all();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(ctx->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(ctx->stream, &p->graph));
+    HIPCHK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+    p->graph_out = out;
+    HIPCHK(hipEventRecord(p->t0, ctx->stream));
+    HIPCHK(hipGraphLaunch(p->gexec, ctx->stream));
+    HIPCHK(hipEventRecord(p->t1, ctx->stream));
+  });
+}
+
+int l2g_eri_plan_set_graph(l2g_eri_plan p, int on) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    (void)hipSetDevice(p->basis->ctx->device);
+    HIPCHK(hipStreamSynchronize(p->basis->ctx->stream));
+    p->use_graph = on != 0;
+    if (!p->use_graph) p->drop_graph();
+  });
+}
+
+int l2g_eri_plan_elapsed(l2g_eri_plan p, float* ms) {
+  return guarded([&] {
+    if (!p || !ms) throw std::invalid_argument("null argument");
+    if (!p->t0) throw L2gError(L2G_E_STATE, "plan not executed");
+    HIPCHK(hipEventSynchronize(p->t1));
+    HIPCHK(hipEventElapsedTime(ms, p->t0, p->t1));
   });
 }
 
@@ -628,7 +700,7 @@ int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* n
       if (nprimq) nprimq[i] = c.nprimq;
       if (ms) {
         float t = 0;
-        if (hipEventElapsedTime(&t, c.ev0, c.ev1) != hipSuccess) t = -1;
+        if (p->use_graph || hipEventElapsedTime(&t, c.ev0, c.ev1) != hipSuccess) t = -1;
         ms[i] = t;
       }
     }

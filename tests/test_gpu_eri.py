@@ -182,3 +182,65 @@ def test_context_close_releases_children():
         plan.execute()
     with pytest.raises(la.L2gError):
         b.handle
+
+
+def test_graph_replay_matches_eager(ctx):
+    """Graph mode: the captured class launches (specialized FP64 and FP32
+    kernels, and the generic kernel for f shells) replayed several times,
+    with eager executions of other plans overwriting the output between the
+    replays, give the eager result bit for bit every time; a new output
+    pointer re-captures."""
+    rng = np.random.default_rng(21)
+    lmax = la.max_l()
+    shells = random_shells(rng, [l for l in range(lmax + 1) for _ in range(2)], kmax=2)
+    n = len(shells)
+    allq = np.array(list(itertools.product(range(n), repeat=4)), np.int32)
+    q = allq[rng.choice(len(allq), min(len(allq), 3000), replace=False)]
+    say(f"case: graph replay nshell={n} nq={len(q)}")
+    b = la.Basis(ctx, la.BasisArrays.from_shells(shells))
+    for thr in (0.0, 1e-4):
+        P = la.EriPlan(b, q, fp32_threshold=thr)
+        P.execute()
+        ref, refnz = P.download()
+        kinds = {(s["fp32"], s["generic"]) for s in P.stats()}
+        say(f"  fp32_threshold={thr}: kernel kinds {sorted(kinds)}")
+        if lmax >= 3:
+            assert (False, True) in kinds
+        # same quartets reversed: same output size, different layout
+        Z = la.EriPlan(b, q[::-1].copy(), fp32_threshold=thr)
+        other = la.EriPlan(b, allq[:500], fp32_threshold=thr)
+        P.set_graph(True)
+        for rep in range(4):
+            P.execute()
+            ctx.synchronize()
+            v, nz = P.download()
+            assert np.array_equal(v, ref), f"replay {rep}"
+            assert np.array_equal(nz, refnz)
+            assert all(s["ms"] == -1 for s in P.stats())
+            assert P.elapsed_ms() > 0
+            # eager work between replays, overwriting P's output
+            Z.execute(P.device_output())
+            other.execute()
+            ctx.synchronize()
+            assert not np.array_equal(P.download()[0], ref)
+        # a second output pointer re-captures; then back to the first
+        H = la.EriPlan(b, q, fp32_threshold=thr)
+        H.execute()
+        Z.execute(H.device_output())
+        ctx.synchronize()
+        assert not np.array_equal(H.download()[0], ref)
+        P.execute(H.device_output())
+        ctx.synchronize()
+        assert np.array_equal(H.download()[0], ref)
+        P.execute()
+        ctx.synchronize()
+        assert np.array_equal(P.download()[0], ref)
+        # graph off: eager again, with per-class timings
+        P.set_graph(False)
+        Z.execute(P.device_output())
+        P.execute()
+        ctx.synchronize()
+        assert np.array_equal(P.download()[0], ref)
+        assert all(s["ms"] >= 0 for s in P.stats())
+        for x in (P, Z, other, H):
+            x._release()
