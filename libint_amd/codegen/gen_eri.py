@@ -876,9 +876,12 @@ def class_waves(plan, budget, waves):
 
 
 def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=None,
-             modes=('eri', 'fock', 'schwarz', 'fp32'), generic_lmax=3):
+             modes=('eri', 'fock', 'schwarz', 'fp32'), generic_lmax=3, max_nodes=0, fp32_lmax=2):
     """Write the kernels and the dispatch tables.  `budgets` maps a class name
-    ('2222') to its own (budget, waves)."""
+    ('2222') to its own (budget, waves).  Classes with an l above 2 whose
+    partitioned recursion evaluates more than `max_nodes` nodes per primitive
+    quartet (0: no limit) get no kernel of their own and run on the generic
+    kernel; FP32 variants are made for classes with every l <= fp32_lmax."""
     os.makedirs(outdir, exist_ok=True)
     classes = canonical_classes(lmax)
     if only:
@@ -889,8 +892,11 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
         return budgets.get("".join(map(str, c)), (budget, waves))
 
     plans = [plan_class(*c, budget=cfg(c)[0]) for c in classes]
+    if max_nodes:
+        plans = [p for p in plans if max(p.la, p.lb, p.lc, p.ld) <= 2 or sum(p.group_nodes) <= max_nodes]
     # reduced-precision variants hold twice as many values per register
-    plans32 = [plan_class(*c, budget=2 * cfg(c)[0]) for c in classes] if 'fp32' in modes else []
+    plans32 = [plan_class(p.la, p.lb, p.lc, p.ld, budget=2 * cfg((p.la, p.lb, p.lc, p.ld))[0]) for p in plans
+               if max(p.la, p.lb, p.lc, p.ld) <= fp32_lmax] if 'fp32' in modes else []
     jobs = []  # (plan, mode, real)
     for p in plans:
         if 'eri' in modes:
@@ -988,6 +994,9 @@ if __name__ == '__main__':
     ap.add_argument("--nfiles", type=int, default=8)
     ap.add_argument("--modes", default="eri,fock,schwarz,fp32")
     ap.add_argument("--generic-lmax", type=int, default=3, help="highest l of the generic kernel's classes")
+    ap.add_argument("--max-nodes", type=int, default=0,
+                    help="classes with l > 2 and more recursion nodes than this run on the generic kernel")
+    ap.add_argument("--fp32-lmax", type=int, default=2, help="highest l with FP32 kernel variants")
     a = ap.parse_args()
     per = {}
     for item in filter(None, a.class_cfg.split(",")):
@@ -996,6 +1005,6 @@ if __name__ == '__main__':
         per[k] = (int(b), int(w))
     files, plans = generate(a.outdir, a.lmax, a.budget, a.nfiles, a.waves,
                             set(filter(None, a.only.split(","))) or None, per, tuple(a.modes.split(",")),
-                            a.generic_lmax)
+                            a.generic_lmax, a.max_nodes, a.fp32_lmax)
     for p in plans:
         print(p.name, 'groups', len(p.groups), 'nodes', sum(p.group_nodes), 'peak', max(p.group_peak))

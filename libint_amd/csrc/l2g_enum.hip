@@ -70,10 +70,10 @@ __global__ void __launch_bounds__(kEnumThreads) enum_fill(const l2g_impl::FockEn
                                                              long long chunk, const long long* __restrict__ base,
                                                              l2g_impl::FockEnumOut O) {
   __shared__ unsigned int pos;
-  __shared__ unsigned int hist[128];
-  __shared__ unsigned long long whist[128];
+  __shared__ unsigned int hist[l2g_impl::kEnumClasses];
+  __shared__ unsigned long long whist[l2g_impl::kEnumClasses];
   if (threadIdx.x == 0) pos = 0;
-  if (threadIdx.x < 128) hist[threadIdx.x] = 0, whist[threadIdx.x] = 0;
+  for (int c = threadIdx.x; c < l2g_impl::kEnumClasses; c += kEnumThreads) hist[c] = 0, whist[c] = 0;
   __syncthreads();
   const long long t0 = (long long)blockIdx.x * chunk, t1 = min(total, t0 + chunk);
   const long long b0 = base[blockIdx.x];
@@ -93,11 +93,12 @@ __global__ void __launch_bounds__(kEnumThreads) enum_fill(const l2g_impl::FockEn
     const int k2 = sbk ? (stb ? s1 : s2) : (stk ? s3 : s4);
     const int bp = sbk ? E.pid[p2] : E.pid[p1], kp = sbk ? E.pid[p1] : E.pid[p2];
     const int lk = ((E.l[b1] * 4 + E.l[b2]) * 4 + E.l[k1]) * 4 + E.l[k2];
-    int cls = E.clsmap[lk];  // every class up to the compiled l has a kernel; 127 flags none
+    // every class up to the compiled l has a kernel; the last class flags none
+    int cls = E.clsmap[lk];
     if (E.fp32_thr > 0 && E.pK[p1] * E.pK[p2] < E.fp32_thr && E.pS[p1] * E.pS[p2] < E.hrr_factor * E.fp32_thr &&
         E.cls32map[lk] >= 0)
       cls = E.cls32map[lk];
-    if (cls < 0 || cls > 126) cls = 127;
+    if (cls < 0 || cls >= l2g_impl::kEnumClasses - 1) cls = l2g_impl::kEnumClasses - 1;
     auto pb = [&](int sh) { return (E.pure[sh] && E.l[sh] > 0) ? 1 : 0; };
     const int w = E.np[bp] * E.np[kp];
     const long long i = b0 + atomicAdd(&pos, 1u);
@@ -108,16 +109,17 @@ __global__ void __launch_bounds__(kEnumThreads) enum_fill(const l2g_impl::FockEn
     O.scale[i] = (double)deg;
     O.bf[i] = int4{E.bf1st[b1], E.bf1st[b2], E.bf1st[k1], E.bf1st[k2]};
     O.pr[i] = int2{0, w};
-    O.key[i] = ((unsigned)cls << 13) | (unsigned)(8191 - min(w, 8191));  // 20 bits
+    O.key[i] = ((unsigned)cls << 13) | (unsigned)(8191 - min(w, 8191));  // kEnumKeyBits bits
     O.idx[i] = (unsigned)i;
     atomicAdd(&hist[cls], 1u);
     atomicAdd(&whist[cls], (unsigned long long)w);
   }
   __syncthreads();
-  if (threadIdx.x < 128 && hist[threadIdx.x]) {
-    atomicAdd(&O.cls_count[threadIdx.x], (unsigned long long)hist[threadIdx.x]);
-    atomicAdd(&O.cls_nprim[threadIdx.x], whist[threadIdx.x]);
-  }
+  for (int c = threadIdx.x; c < l2g_impl::kEnumClasses; c += kEnumThreads)
+    if (hist[c]) {
+      atomicAdd(&O.cls_count[c], (unsigned long long)hist[c]);
+      atomicAdd(&O.cls_nprim[c], whist[c]);
+    }
 }
 
 template <class T>
@@ -143,8 +145,8 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   const long long np = E.npairs;
   const long long total = np * (np + 1) / 2;
   S.n = 0;
-  S.cls_count.assign(128, 0ull);
-  S.cls_nprim.assign(128, 0ull);
+  S.cls_count.assign(kEnumClasses, 0ull);
+  S.cls_nprim.assign(kEnumClasses, 0ull);
   if (total == 0) return;
   const int nblocks = (int)std::min<long long>(8192, (total + kEnumThreads * 64 - 1) / (kEnumThreads * 64));
   const long long chunk = (total + nblocks - 1) / nblocks;
@@ -177,8 +179,8 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   u_key.alloc(sz * sizeof(unsigned));
   u_idx.alloc(sz * sizeof(unsigned));
   s_key.alloc(sz * sizeof(unsigned));
-  stats.alloc(256 * sizeof(unsigned long long));
-  HIPCHK(hipMemsetAsync(stats.p, 0, 256 * sizeof(unsigned long long), st));
+  stats.alloc(2 * kEnumClasses * sizeof(unsigned long long));
+  HIPCHK(hipMemsetAsync(stats.p, 0, 2 * kEnumClasses * sizeof(unsigned long long), st));
   FockEnumOut O;
   O.bra = u_bra.as<int>();
   O.ket = u_ket.as<int>();
@@ -190,18 +192,18 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   O.key = u_key.as<unsigned>();
   O.idx = u_idx.as<unsigned>();
   O.cls_count = stats.as<unsigned long long>();
-  O.cls_nprim = stats.as<unsigned long long>() + 128;
+  O.cls_nprim = stats.as<unsigned long long>() + kEnumClasses;
   hipLaunchKernelGGL(enum_fill, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.base.as<long long>(), O);
   HIPCHK(hipGetLastError());
   // sort the item indices by (class, decreasing depth)
   S.perm.alloc(sz * sizeof(unsigned));
   size_t tmp_bytes = 0;
   HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, O.key, s_key.as<unsigned>(), O.idx,
-                                            S.perm.as<unsigned>(), (int)n, 0, 20, st));
+                                            S.perm.as<unsigned>(), (int)n, 0, kEnumKeyBits, st));
   DevBuf tmp;
   tmp.alloc(std::max<size_t>(tmp_bytes, 16));
   HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, O.key, s_key.as<unsigned>(), O.idx,
-                                            S.perm.as<unsigned>(), (int)n, 0, 20, st));
+                                            S.perm.as<unsigned>(), (int)n, 0, kEnumKeyBits, st));
   const unsigned* perm = S.perm.as<unsigned>();
   S.bra.alloc(sz * sizeof(int));
   S.ket.alloc(sz * sizeof(int));
@@ -217,11 +219,12 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   gather(O.scale, perm, S.scale.as<double>(), n, st);
   gather(O.bf, perm, S.bf.as<int4>(), n, st);
   gather(O.pr, perm, S.pr.as<int2>(), n, st);
-  std::vector<unsigned long long> h(256);
-  HIPCHK(hipMemcpyAsync(h.data(), stats.p, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  std::vector<unsigned long long> h(2 * kEnumClasses);
+  HIPCHK(hipMemcpyAsync(h.data(), stats.p, 2 * kEnumClasses * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        st));
   HIPCHK(hipStreamSynchronize(st));  // the temporaries are freed on return
-  for (int c = 0; c < 128; ++c) S.cls_count[c] = h[c], S.cls_nprim[c] = h[128 + c];
-  if (S.cls_count[127]) throw L2gError(L2G_E_LMAX, "no Fock kernel for an angular momentum class");
+  for (int c = 0; c < kEnumClasses; ++c) S.cls_count[c] = h[c], S.cls_nprim[c] = h[kEnumClasses + c];
+  if (S.cls_count[kEnumClasses - 1]) throw L2gError(L2G_E_LMAX, "no Fock kernel for an angular momentum class");
 }
 
 }  // namespace l2g_impl

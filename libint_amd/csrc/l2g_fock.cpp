@@ -602,7 +602,7 @@ void upload_enum_tables(l2g_fock_s& f, hipStream_t st) {
           const int c32 = ctx->force_generic ? -1 : ctx->fock32_of[a][b][c][d];
           cls32[k] = c32 >= 0 ? nf64 + c32 : -1;
         }
-  if (nfock_classes() > 127) throw L2gError(L2G_E_STATE, "too many Fock kernel classes for the enumeration sort key");
+  if (nfock_classes() > kEnumClasses - 1) throw L2gError(L2G_E_STATE, "too many Fock kernel classes for the enumeration sort key");
   upload(f.e_ps1, ps1, st);
   upload(f.e_ps2, ps2, st);
   upload(f.e_pK, pK, st);

@@ -231,7 +231,11 @@ void launch_generic(int mode, const L2gEriArgs& A, const L2gGenericClass& C, con
                     int grid, hipStream_t s);
 inline int lmax_supported() { return std::max(l2g_gen_lmax, l2g_gen_generic_lmax); }
 
-// l2g_enum.hip: the screened unique-quartet list of a Fock build on the GPU
+// l2g_enum.hip: the screened unique-quartet list of a Fock build on the GPU.
+// Sort key: Fock kernel class (8 bits; the last value flags a class without
+// a kernel) above 13 bits of decreasing primitive-loop length.
+constexpr int kEnumClasses = 256;
+constexpr int kEnumKeyBits = 21;
 struct FockEnumArgs {
   const int* ps1;               // significant pairs in (s1, s2) order, s2 <= s1
   const int* ps2;
@@ -260,7 +264,7 @@ struct FockEnumOut {
 // the sorted items (class-major, decreasing depth) of the last enumeration
 struct FockEnumState {
   long long n = 0;
-  std::vector<unsigned long long> cls_count, cls_nprim;  // 128 classes
+  std::vector<unsigned long long> cls_count, cls_nprim;  // kEnumClasses classes
   DevBuf counts, base, perm, bra, ket, meta, lnp, scale, bf, pr;
 };
 void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st);
