@@ -295,13 +295,22 @@ void sort_by_depth(FockItems& it) {
 // digested on its own; the cost is one more HRR, transform and digest per
 // range.  Without this, one (dd|dd) quartet with K = 6 on every shell runs
 // ~1300 primitive quartets x 19 groups on one lane, tens of ms.
-void split_deep(FockItems& it, int cls) {
+void split_deep(FockItems& it, int cls, int lanes) {
   const size_t m = it.size();
   // the generic kernel runs one quartet per 256-thread block
   const bool gen = fock_is_generic(cls);
   if (m == 0 || m >= (size_t)(gen ? 4096 : (1 << 17))) return;
-  const int chunk = gen ? std::max(16, 2000000 / std::max(1, fock_gclass(cls).nvals))
-                        : std::max(8, 140000 / std::max(1, fock_info(cls).nodes));
+  int chunk = gen ? std::max(16, 2000000 / std::max(1, fock_gclass(cls).nvals))
+                  : std::max(8, 140000 / std::max(1, fock_info(cls).nodes));
+  if (!gen) {
+    // A lane runs its primitive loop serially, and each iteration waits on
+    // table and pair loads, so a class with little work in total is bound
+    // by its deepest item: split down to the class's share per lane of the
+    // chip (one wave per SIMD), at least 16 primitives per piece.
+    int64_t total = 0;
+    for (size_t i = 0; i < m; ++i) total += it.pr[i].y - it.pr[i].x;
+    chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(16, (total + lanes - 1) / std::max(1, lanes)));
+  }
   size_t extra = 0;
   for (size_t i = 0; i < m; ++i) {
     const int len = it.pr[i].y - it.pr[i].x;
@@ -424,7 +433,7 @@ void fock_enumerate(const FockSetup& f, const double* D, double precision, doubl
     o.bra.reserve(tot), o.ket.reserve(tot), o.meta.reserve(tot), o.lnp.reserve(tot), o.scale.reserve(tot),
         o.bf.reserve(tot);
     for (auto& w : part) o.append(w.cls[c]);
-    split_deep(o, c);
+    split_deep(o, c, f.B->ctx->ncu * 256);
     sort_by_depth(o);
   });
 }
@@ -745,7 +754,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     HIPCHK(hipStreamSynchronize(st));
     parallel_for(ncls, [&](int c) {
       if (small[c].size() == 0) return;
-      split_deep(small[c], c);
+      split_deep(small[c], c, ctx->ncu * 256);
       sort_by_depth(small[c]);
     });
     f->items.upload_from(small, st);
