@@ -318,14 +318,16 @@ def plan_class(la, lb, lc, ld, budget=96):
         if cost([i], allf)[1] <= budget:
             cur = [i]
             continue
+        # ket chunks keep the target order they were costed in: the
+        # evaluation order, and so the live set, follows the target order
         fcur = []
         for j in order_f:
             if not fcur or cost([i], fcur + [j])[1] <= budget:
                 fcur.append(j)
             else:
-                groups.append(([i], sorted(fcur)))
+                groups.append(([i], fcur))
                 fcur = [j]
-        groups.append(([i], sorted(fcur)))
+        groups.append(([i], fcur))
     if cur:
         groups.append((cur, allf))
     gn, gp = [], []

@@ -34,7 +34,7 @@ void launch_fock_finish(double* jg, double* kg, double* g, long long N, hipStrea
   const int block = 256;
   const long long grid = (n + block - 1) / block;
   hipLaunchKernelGGL(l2g_fock_finish_kernel, dim3((unsigned)grid), dim3(block), 0, s, jg, kg, g, N);
-  HIPCHK(hipGetLastError());
+  LAUNCHCHK("l2g_fock_finish_kernel");
 }
 
 }  // namespace l2g_impl

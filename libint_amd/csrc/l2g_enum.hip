@@ -134,12 +134,112 @@ void gather(const T* in, const unsigned* perm, T* out, long long n, hipStream_t 
   if (n == 0) return;
   const int block = 256;
   hipLaunchKernelGGL(gather_kernel<T>, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, s, in, perm, out, n);
-  HIPCHK(hipGetLastError());
+  LAUNCHCHK("gather_kernel");
+}
+
+// Primitive-range split of the sorted items: an item of class c whose
+// primitive loop is longer than chunk[c] becomes ceil(len / chunk[c])
+// consecutive pieces of near-equal length.
+__global__ void __launch_bounds__(kEnumThreads) split_count(long long n, const unsigned* __restrict__ key,
+                                                               const int2* __restrict__ pr,
+                                                               const int* __restrict__ chunk,
+                                                               unsigned* __restrict__ pieces,
+                                                               unsigned long long* __restrict__ cls_pieces) {
+  __shared__ unsigned int hist[l2g_impl::kEnumClasses];
+  for (int c = threadIdx.x; c < l2g_impl::kEnumClasses; c += kEnumThreads) hist[c] = 0;
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * kEnumThreads + threadIdx.x;
+  if (i < n) {
+    const int c = (int)(key[i] >> 13);
+    const int len = pr[i].y - pr[i].x, ch = chunk[c];
+    const unsigned np = len > ch ? (unsigned)((len + ch - 1) / ch) : 1u;
+    pieces[i] = np;
+    atomicAdd(&hist[c], np);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < l2g_impl::kEnumClasses; c += kEnumThreads)
+    if (hist[c]) atomicAdd(&cls_pieces[c], (unsigned long long)hist[c]);
+}
+
+__global__ void __launch_bounds__(kEnumThreads) split_fill(long long n, const unsigned* __restrict__ pieces,
+                                                              const unsigned long long* __restrict__ at,
+                                                              l2g_impl::FockEnumState::Items in,
+                                                              l2g_impl::FockEnumState::Items out) {
+  const long long i = (long long)blockIdx.x * kEnumThreads + threadIdx.x;
+  if (i >= n) return;
+  const unsigned np = pieces[i];
+  const unsigned long long o = at[i];
+  const int b = in.pr[i].x, len = in.pr[i].y - b;
+  for (unsigned k = 0; k < np; ++k) {
+    const unsigned long long j = o + k;
+    out.bra[j] = in.bra[i];
+    out.ket[j] = in.ket[i];
+    out.meta[j] = in.meta[i];
+    out.lnp[j] = in.lnp[i];
+    out.scale[j] = in.scale[i];
+    out.bf[j] = in.bf[i];
+    out.pr[j] = int2{b + (int)((long long)len * k / np), b + (int)((long long)len * (k + 1) / np)};
+  }
 }
 
 }  // namespace
 
 namespace l2g_impl {
+
+void fock_split_device(FockEnumState& S, const std::vector<int>& chunk, hipStream_t st) {
+  const long long n = S.n;
+  S.cls_pieces.assign(kEnumClasses, 0ull);
+  if (n == 0) return;
+  if ((int)chunk.size() != kEnumClasses) throw L2gError(L2G_E_STATE, "split: one chunk per class expected");
+  LAUNCHCHK("work before the item split");
+  DevBuf d_chunk, pieces, at, cls, tmp;
+  d_chunk.alloc(kEnumClasses * sizeof(int));
+  HIPCHK(hipMemcpyAsync(d_chunk.p, chunk.data(), kEnumClasses * sizeof(int), hipMemcpyHostToDevice, st));
+  pieces.alloc((size_t)n * sizeof(unsigned));
+  at.alloc((size_t)n * sizeof(unsigned long long));
+  cls.alloc(kEnumClasses * sizeof(unsigned long long));
+  HIPCHK(hipMemsetAsync(cls.p, 0, kEnumClasses * sizeof(unsigned long long), st));
+  const unsigned nb = (unsigned)((n + kEnumThreads - 1) / kEnumThreads);
+  hipLaunchKernelGGL(split_count, dim3(nb), dim3(kEnumThreads), 0, st, n, S.key.as<unsigned>(), S.pr.as<int2>(),
+                     d_chunk.as<int>(), pieces.as<unsigned>(), cls.as<unsigned long long>());
+  LAUNCHCHK("split_count");
+  size_t tmp_bytes = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, pieces.as<unsigned>(), at.as<unsigned long long>(),
+                                          (int)n, st));
+  tmp.alloc(std::max<size_t>(tmp_bytes, 16));
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, pieces.as<unsigned>(), at.as<unsigned long long>(),
+                                          (int)n, st));
+  HIPCHK(hipMemcpyAsync(S.cls_pieces.data(), cls.p, kEnumClasses * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  unsigned long long total = 0;
+  for (int c = 0; c < kEnumClasses; ++c) {
+    if (S.cls_pieces[c] < S.cls_count[c])
+      throw L2gError(L2G_E_STATE, "split: class " + std::to_string(c) + " has " + std::to_string(S.cls_pieces[c]) +
+                                      " pieces for " + std::to_string(S.cls_count[c]) + " items");
+    total += S.cls_pieces[c];
+  }
+  if (total >= (1ull << 31)) throw L2gError(L2G_E_INVALID, "more than 2^31 work items in one Fock build");
+  if (total == (unsigned long long)n) return;  // nothing to split
+  FockEnumState::Items in = S.items();
+  DevBuf bra, ket, meta, lnp, scale, bf, pr;
+  bra.alloc(total * sizeof(int));
+  ket.alloc(total * sizeof(int));
+  meta.alloc(total * sizeof(int));
+  lnp.alloc(total * sizeof(double));
+  scale.alloc(total * sizeof(double));
+  bf.alloc(total * sizeof(int4));
+  pr.alloc(total * sizeof(int2));
+  FockEnumState::Items out{bra.as<int>(), ket.as<int>(), meta.as<int>(), lnp.as<double>(), scale.as<double>(),
+                           bf.as<int4>(), pr.as<int2>()};
+  hipLaunchKernelGGL(split_fill, dim3(nb), dim3(kEnumThreads), 0, st, n, pieces.as<unsigned>(),
+                     at.as<unsigned long long>(), in, out);
+  LAUNCHCHK("split_fill");
+  HIPCHK(hipStreamSynchronize(st));  // the inputs are freed on return
+  std::swap(S.bra, bra), std::swap(S.ket, ket), std::swap(S.meta, meta), std::swap(S.lnp, lnp);
+  std::swap(S.scale, scale), std::swap(S.bf, bf), std::swap(S.pr, pr);
+  S.n = (long long)total;
+}
 
 void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st) {
   const long long np = E.npairs;
@@ -152,7 +252,7 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   const long long chunk = (total + nblocks - 1) / nblocks;
   S.counts.alloc((size_t)nblocks * sizeof(long long));
   hipLaunchKernelGGL(enum_count, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.counts.as<long long>());
-  HIPCHK(hipGetLastError());
+  LAUNCHCHK("enum_count");
   std::vector<long long> cnt(nblocks), base(nblocks);
   HIPCHK(hipMemcpyAsync(cnt.data(), S.counts.p, (size_t)nblocks * sizeof(long long), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -168,7 +268,8 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   HIPCHK(hipMemcpyAsync(S.base.p, base.data(), (size_t)nblocks * sizeof(long long), hipMemcpyHostToDevice, st));
   // unsorted items, then sorted
   const size_t sz = (size_t)n;
-  DevBuf u_bra, u_ket, u_meta, u_lnp, u_scale, u_bf, u_pr, u_key, u_idx, s_key, stats;
+  DevBuf u_bra, u_ket, u_meta, u_lnp, u_scale, u_bf, u_pr, u_key, u_idx, stats;
+  DevBuf& s_key = S.key;
   u_bra.alloc(sz * sizeof(int));
   u_ket.alloc(sz * sizeof(int));
   u_meta.alloc(sz * sizeof(int));
@@ -194,7 +295,7 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   O.cls_count = stats.as<unsigned long long>();
   O.cls_nprim = stats.as<unsigned long long>() + kEnumClasses;
   hipLaunchKernelGGL(enum_fill, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.base.as<long long>(), O);
-  HIPCHK(hipGetLastError());
+  LAUNCHCHK("enum_fill");
   // sort the item indices by (class, decreasing depth)
   S.perm.alloc(sz * sizeof(unsigned));
   size_t tmp_bytes = 0;

@@ -1,6 +1,7 @@
 // Fock build (compute_2body_fock restated for the GPU) and the Fock-build
 // request server of the C ABI declared in include/l2g.h.
 #include <chrono>
+#include <climits>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -295,22 +296,30 @@ void sort_by_depth(FockItems& it) {
 // digested on its own; the cost is one more HRR, transform and digest per
 // range.  Without this, one (dd|dd) quartet with K = 6 on every shell runs
 // ~1300 primitive quartets x 19 groups on one lane, tens of ms.
+// Longest primitive range of one work item of class cls, for m items with
+// `total` primitive quartets in all, on a chip of `lanes` lanes (one wave
+// per SIMD).  A lane runs its primitive loop serially, each iteration waiting
+// on table and pair loads, so a deep item would bound the whole class: items
+// are cut to half of the class's per-lane share, but not below 8-16
+// primitives (fewer for classes with large recursions), which keeps the
+// per-piece HRR and digest small against the loop.  The generic kernel runs
+// one item per block; only its classes with few items are split, to about
+// 2e6 recursion values per piece.
+int split_chunk(int cls, int64_t m, int64_t total, int lanes) {
+  if (m == 0) return INT_MAX;
+  if (fock_is_generic(cls))
+    return m < 4096 ? std::max(16, 2000000 / std::max(1, fock_gclass(cls).nvals)) : INT_MAX;
+  const int minc = std::max(8, std::min(16, 140000 / std::max(1, fock_info(cls).nodes)));
+  const int64_t share = (total + 2ll * lanes - 1) / (2ll * std::max(1, lanes));
+  return (int)std::min<int64_t>(INT_MAX, std::max<int64_t>(minc, share));
+}
+
 void split_deep(FockItems& it, int cls, int lanes) {
   const size_t m = it.size();
-  // the generic kernel runs one quartet per 256-thread block
-  const bool gen = fock_is_generic(cls);
-  if (m == 0 || m >= (size_t)(gen ? 4096 : (1 << 17))) return;
-  int chunk = gen ? std::max(16, 2000000 / std::max(1, fock_gclass(cls).nvals))
-                  : std::max(8, 140000 / std::max(1, fock_info(cls).nodes));
-  if (!gen) {
-    // A lane runs its primitive loop serially, and each iteration waits on
-    // table and pair loads, so a class with little work in total is bound
-    // by its deepest item: split down to the class's share per lane of the
-    // chip (one wave per SIMD), at least 16 primitives per piece.
-    int64_t total = 0;
-    for (size_t i = 0; i < m; ++i) total += it.pr[i].y - it.pr[i].x;
-    chunk = (int)std::min<int64_t>(chunk, std::max<int64_t>(16, (total + lanes - 1) / std::max(1, lanes)));
-  }
+  int64_t total = 0;
+  for (size_t i = 0; i < m; ++i) total += it.pr[i].y - it.pr[i].x;
+  const int chunk = split_chunk(cls, (int64_t)m, total, lanes);
+  if (chunk == INT_MAX) return;
   size_t extra = 0;
   for (size_t i = 0; i < m; ++i) {
     const int len = it.pr[i].y - it.pr[i].x;
@@ -544,8 +553,10 @@ void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, b
       launch_generic(kGenericFock, A, fock_gclass((int)c), ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
                      (int)std::min<int64_t>((int64_t)ctx->ncu * 8, m), st);
     } else {
-      fock_info((int)c).launch(A, (int)std::min<int64_t>(grid, (m + block - 1) / block), block, st);
-      HIPCHK(hipGetLastError());
+      const int g = (int)std::min<int64_t>(grid, (m + block - 1) / block);
+      fock_info((int)c).launch(A, g, block, st);
+      LAUNCHCHK(std::string("Fock kernel of class ") + std::to_string(c) + " (grid " + std::to_string(g) +
+                ", block " + std::to_string(block) + ", items " + std::to_string(m) + ")");
     }
     if (ev) HIPCHK(hipEventRecord(ev[2 * c + 1], st));
   }
@@ -560,7 +571,6 @@ struct l2g_fock_s {
   float last_ms = 0;
   PairDev pairs;  // resident shell-pair data
   DevBuf dens, jg, kg, mats, ybuf;
-  ItemsDev items;  // split items of the classes with few quartets
   // GPU quartet enumeration: pair list in (s1, s2) order, per-shell data,
   // class maps, shell-block norms of D and the sorted items
   DevBuf e_ps1, e_ps2, e_pK, e_pS, e_pid, e_pswap, e_l, e_pure, e_bf1st, e_np, e_cls, e_cls32, e_Dn;
@@ -710,55 +720,35 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     FockEnumState& es = f->enumst;
     fock_enumerate_device(E, es, st);  // synchronises; Dn may go
     const int ncls = nfock_classes();
-    ItemsView big;
-    big.bra = es.bra.as<int>(), big.ket = es.ket.as<int>(), big.meta = es.meta.as<int>();
-    big.lnp = es.lnp.as<double>(), big.scale = es.scale.as<double>(), big.bf = es.bf.as<int4>();
-    big.pr = es.pr.as<int2>();
-    big.first.assign(ncls, 0);
-    big.count.assign(ncls, 0);
     f->last_nq = es.n;
     f->last_nq32 = 0;
     f->last_nprim = 0;
     f->cls_nq.assign(ncls, 0);
     f->cls_nprim.assign(ncls, 0);
-    // classes with few items come back to the host to be split by
-    // primitive range (split_deep) and go up again as a second item set
-    std::vector<FockItems> small(ncls);
-    int64_t o = 0;
+    std::vector<int> chunk(kEnumClasses, INT_MAX);
     for (int c = 0; c < ncls; ++c) {
       const int64_t m = (int64_t)es.cls_count[c];
       f->cls_nq[c] = m;
       f->cls_nprim[c] = (int64_t)es.cls_nprim[c];
       f->last_nprim += f->cls_nprim[c];
       if (c >= l2g_gen_nfock && !fock_is_generic(c)) f->last_nq32 += m;
+      chunk[c] = split_chunk(c, m, f->cls_nprim[c], ctx->ncu * 256);
+    }
+    // deep items cut by primitive range on the device (class-major order kept)
+    fock_split_device(es, chunk, st);
+    ItemsView big;
+    big.bra = es.bra.as<int>(), big.ket = es.ket.as<int>(), big.meta = es.meta.as<int>();
+    big.lnp = es.lnp.as<double>(), big.scale = es.scale.as<double>(), big.bf = es.bf.as<int4>();
+    big.pr = es.pr.as<int2>();
+    big.first.assign(ncls, 0);
+    big.count.assign(ncls, 0);
+    int64_t o = 0;
+    for (int c = 0; c < ncls; ++c) {
+      const int64_t m = es.n == 0 ? 0 : (int64_t)es.cls_pieces[c];
       big.first[c] = o;
-      if (m > 0 && m < (fock_is_generic(c) ? 4096 : (1 << 17))) {
-        FockItems& it = small[c];
-        it.bra.resize(m), it.ket.resize(m), it.meta.resize(m), it.lnp.resize(m), it.scale.resize(m), it.bf.resize(m),
-            it.pr.resize(m);
-        auto down = [&](void* dst, const DevBuf& b, size_t elem) {
-          HIPCHK(hipMemcpyAsync(dst, b.as<char>() + (size_t)o * elem, (size_t)m * elem, hipMemcpyDeviceToHost, st));
-        };
-        down(it.bra.data(), es.bra, sizeof(int));
-        down(it.ket.data(), es.ket, sizeof(int));
-        down(it.meta.data(), es.meta, sizeof(int));
-        down(it.lnp.data(), es.lnp, sizeof(double));
-        down(it.scale.data(), es.scale, sizeof(double));
-        down(it.bf.data(), es.bf, sizeof(int4));
-        down(it.pr.data(), es.pr, sizeof(int2));
-      } else {
-        big.count[c] = m;
-      }
+      big.count[c] = m;
       o += m;
     }
-    HIPCHK(hipStreamSynchronize(st));
-    parallel_for(ncls, [&](int c) {
-      if (small[c].size() == 0) return;
-      split_deep(small[c], c, ctx->ncu * 256);
-      sort_by_depth(small[c]);
-    });
-    f->items.upload_from(small, st);
-    const ItemsView sv = f->items.view();
     const size_t NN = (size_t)N * N;
     f->dens.alloc(std::max<size_t>(NN, 1) * sizeof(double));
     f->jg.alloc(std::max<size_t>(NN, 1) * sizeof(double));
@@ -771,7 +761,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     const int block = 128;
     int64_t maxn = 0, ymax = 0;
     for (int c = 0; c < ncls; ++c) {
-      const int64_t m = std::max(big.count[c], sv.count[c]);
+      const int64_t m = big.count[c];
       if (m == 0) continue;
       maxn = std::max(maxn, m);
       ymax = std::max<int64_t>(ymax, fock_ybuf(c));
@@ -791,7 +781,6 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
     HIPCHK(hipEventRecord(e0, st));
     fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data());
-    fock_launch_all(ctx, A, sv, false, grid, block, st, f->ev.data());
     HIPCHK(hipEventRecord(e1, st));
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);

@@ -415,7 +415,7 @@ void launch_generic(int mode, const L2gEriArgs& A, const L2gGenericClass& C, con
     launch_mode<kModeFock>(A, C, prog, idx, grid, s);
   else
     launch_mode<kModeSchwarz>(A, C, prog, idx, grid, s);
-  HIPCHK(hipGetLastError());
+  LAUNCHCHK("generic kernel");
 }
 
 }  // namespace l2g_impl

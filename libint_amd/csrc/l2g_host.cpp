@@ -601,7 +601,7 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
         const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
         const int grid = (int)std::min<int64_t>(p->grid, (c.count + p->block - 1) / p->block);
         ci.launch(A, grid, p->block, ctx->stream);
-        HIPCHK(hipGetLastError());
+        LAUNCHCHK(std::string("ERI kernel of class ") + std::to_string(c.cls));
       }
       if (!capture) HIPCHK(hipEventRecord(c.ev1, ctx->stream));
     }
@@ -828,7 +828,7 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
     A.n = (int)n;
     if (c < nsch) {
       l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
-      HIPCHK(hipGetLastError());
+      LAUNCHCHK(std::string("Schwarz kernel of class ") + std::to_string(c));
     } else {
       launch_generic(kGenericSchwarz, A, l2g_gen_generic[c - nsch], ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
                      (int)std::min<int64_t>((int64_t)ctx->ncu * 8, n), st);

@@ -57,6 +57,15 @@ struct L2gError : std::runtime_error {
                      std::string(#x) + ": " + hipGetErrorString(e_));                              \
   } while (0)
 
+// error check after a kernel launch, naming the kernel
+#define LAUNCHCHK(name)                                                                             \
+  do {                                                                                              \
+    hipError_t e_ = hipGetLastError();                                                              \
+    if (e_ != hipSuccess)                                                                           \
+      throw L2gError(e_ == hipErrorOutOfMemory ? L2G_E_OOM : L2G_E_LAUNCH,                        \
+                     std::string("launch of ") + (name) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
 template <class F>
 inline int guarded(F&& f) {
   try {
@@ -103,6 +112,22 @@ inline int sgn_parity(int i) { return (i & 1) ? -1 : 1; }
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  // owning: moved (std::swap included), never copied
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p, bytes = o.bytes;
+      o.p = nullptr, o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -263,11 +288,25 @@ struct FockEnumOut {
 };
 // the sorted items (class-major, decreasing depth) of the last enumeration
 struct FockEnumState {
-  long long n = 0;
+  long long n = 0;  // items (after fock_split_device: pieces)
   std::vector<unsigned long long> cls_count, cls_nprim;  // kEnumClasses classes
-  DevBuf counts, base, perm, bra, ket, meta, lnp, scale, bf, pr;
+  std::vector<unsigned long long> cls_pieces;            // per class, after the split
+  DevBuf counts, base, perm, key, bra, ket, meta, lnp, scale, bf, pr;
+  struct Items {
+    int *bra, *ket, *meta;
+    double *lnp, *scale;
+    int4* bf;
+    int2* pr;
+  };
+  Items items() {
+    return Items{bra.as<int>(), ket.as<int>(), meta.as<int>(), lnp.as<double>(), scale.as<double>(), bf.as<int4>(),
+                 pr.as<int2>()};
+  }
 };
 void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st);
+// Splits the enumerated items by primitive range, chunk[c] primitives at most
+// per piece of class c (kEnumClasses entries); pieces stay class-major.
+void fock_split_device(FockEnumState& S, const std::vector<int>& chunk, hipStream_t st);
 
 
 // Shell-pair data for ordered pair (i, j), restating ShellPair::init

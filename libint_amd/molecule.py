@@ -63,13 +63,15 @@ def place_atoms(rng: np.random.Generator, natoms: int, box: float, min_sep: floa
 
 
 def make_molecule(workload: int, natoms: int | None = None, seed: int | None = None,
-                  max_l: int | None = None) -> Molecule:
+                  max_l: int | None = None, same_density: bool = False) -> Molecule:
     """Molecule of a workload recipe.  seed defaults to 1000 + workload.
-    max_l drops shells above that angular momentum (for reduced tests)."""
+    max_l drops shells above that angular momentum (for reduced tests).
+    same_density scales the cube edge with natoms^(1/3) so that a smaller
+    molecule keeps the recipe's atom density (else the edge is the recipe's)."""
     spec = WORKLOADS[workload]
     n = spec["natoms"] if natoms is None else natoms
     rng = np.random.Generator(np.random.PCG64(1000 + workload if seed is None else seed))
-    coords = place_atoms(rng, n, spec["box"])
+    coords = place_atoms(rng, n, box_edge(workload, n) if same_density else spec["box"])
     lo, hi = spec["expo"]
     l_, pure_, np_, al_, co_, org_, atom_ = [], [], [], [], [], [], []
     for a in range(n):
@@ -89,6 +91,12 @@ def make_molecule(workload: int, natoms: int | None = None, seed: int | None = N
             atom_.append(a)
     b = BasisArrays(l_, pure_, np_, np.concatenate(al_), np.concatenate(co_), np.array(org_))
     return Molecule(coords, b, np.array(atom_), workload)
+
+
+def box_edge(workload: int, natoms: int) -> float:
+    """Cube edge (Angstrom) holding natoms at the recipe's atom density."""
+    spec = WORKLOADS[workload]
+    return spec["box"] * (natoms / spec["natoms"]) ** (1.0 / 3.0)
 
 
 def unique_quartets(nshell: int, shells: np.ndarray | None = None) -> np.ndarray:
