@@ -109,17 +109,20 @@ def density(nbf, seed):
     return C @ C.T
 
 
-def fock_leg(la, molecule, ctx, natoms, reps):
-    """J/K build (compute_2body_fock on the GPU) for a molecule of the same
+def fock_leg(la, molecule, ctx, natoms, reps, workload=2, same_density=False):
+    """J/K build (compute_2body_fock on the GPU) for a molecule of a workload
     recipe, FP64 and mixed (FP32 where K12 K34 < 1e-6 and S12 S34 < 16e-6):
     median wall time per build, kernel time, and the largest difference of
     the mixed G from the FP64 G."""
-    mol = molecule.make_molecule(2, natoms=natoms)
+    mol = molecule.make_molecule(workload, natoms=natoms, same_density=same_density)
     B = la.Basis(ctx, mol.basis)
     fb = la.FockBuilder(B)
     D = density(mol.nbf, 12)
-    out = {"workload": f"{natoms} atoms of the same recipe, nbf {mol.nbf}, D = C C^T, C ~ N(0, 0.1), "
-                       f"precision 1e-12, SchwarzInf"}
+    spec = molecule.WORKLOADS[workload]
+    box = molecule.box_edge(workload, natoms) if same_density else spec["box"]
+    shells = "s4p3d2f1, K~U{1..8}, exponents [0.03,2e4]" if workload >= 3 else "s4p3d2, K~U{1..6}"
+    out = {"workload": f"{natoms} atoms in a {box:.2f} A cube ({shells}), nbf {mol.nbf}, D = C C^T, "
+                       f"C ~ N(0, 0.1), precision 1e-12, SchwarzInf"}
     G = {}
     for name, thr in (("fp64", 0.0), ("mixed", 1e-6)):
         fb.set_fp32_threshold(thr)
@@ -138,7 +141,7 @@ def fock_leg(la, molecule, ctx, natoms, reps):
                      "top_classes": [f"{c['cls']}{'/fp32' if c['fp32'] else ''}: {c['ms']:.1f} ms, "
                                      f"{c['nquartets']} quartets, {c['nprim']} primitive quartets"
                                      for c in cls[:6]]}
-        log(f"fock {name}: build {out[name]['build_ms']:.1f} ms, kernels {out[name]['kernel_ms']:.1f} ms")
+        log(f"fock {natoms} atoms (recipe {workload}) {name}: build {out[name]['build_ms']:.1f} ms, kernels {out[name]['kernel_ms']:.1f} ms")
     out["mixed_speedup_kernel"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
     out["mixed_speedup_build"] = out["fp64"]["build_ms"] / out["mixed"]["build_ms"]
     out["mixed_max_abs_dG"] = float(np.abs(G["mixed"] - G["fp64"]).max())
@@ -191,6 +194,8 @@ def main():
                     help="only run a plain 1 GiB device copy (the known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--fock-atoms", type=int, default=20, help="atoms of the J/K build leg")
+    ap.add_argument("--fock-f-atoms", type=int, default=10,
+                    help="atoms of the f-shell J/K build leg (0: skip), at the 200-atom recipe's density")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving leg")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -287,6 +292,8 @@ def main():
     del plan
     if not args.no_fock:
         out["fock"] = fock_leg(la, molecule, ctx, args.fock_atoms, reps=2)
+        if args.fock_f_atoms:
+            out["fock_f"] = fock_leg(la, molecule, ctx, args.fock_f_atoms, reps=2, workload=4, same_density=True)
         out["serve"] = serve_leg(la, molecule, args.serve_requests, rate_hz=20.0)
         out["p99_fock_build_ms"] = out["serve"]["p99_ms"]
     if not args.no_cpu:
