@@ -656,7 +656,14 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     E.ind += 1
     E("const int tid = blockIdx.x * blockDim.x + threadIdx.x;")
     E("const int nthr = gridDim.x * blockDim.x;")
-    E.block("for (int qi = tid; qi < A.n; qi += nthr)")
+    if mode == 'schwarz':
+        E("// A.coop: a wave per pair, each lane taking a contiguous 1/64 of the")
+        E("// primitive quartets; the group accumulators are summed over the wave")
+        E("const bool coop = A.coop != 0;")
+        E("const int lane = threadIdx.x & 63;")
+        E.block("for (int qi = coop ? (tid >> 6) : tid; qi < A.n; qi += coop ? (nthr >> 6) : nthr)")
+    else:
+        E.block("for (int qi = tid; qi < A.n; qi += nthr)")
     E("const int bp = A.bra_pair[qi];")
     E("const int kp = " + ("bp;" if mode == 'schwarz' else "A.ket_pair[qi];"))
     E("const int meta = A.meta[qi];")
@@ -673,6 +680,12 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         E("const int2 pr = A.q_prange[qi];")
         E("pbeg = pr.x;")
         E("pend = pr.y;")
+        E.end()
+    if mode == 'schwarz':
+        E.block("if (coop)")
+        E("const int len = pend - pbeg, b0 = pbeg + (int)(((long long)len * lane) >> 6);")
+        E("pend = pbeg + (int)(((long long)len * (lane + 1)) >> 6);")
+        E("pbeg = b0;")
         E.end()
     if mode == 'fock':
         E("const double lnprec = A.q_lnprec[qi];")
@@ -710,6 +723,10 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
             E(f"{_R} {a} = 0;")
         emit_prim_loop(E, plan, graph, gi, accs)
         if mode == 'schwarz':
+            E.block("if (coop)")
+            for a in accs:
+                E(f"{a} = l2g_wave_sum({a});")
+            E.end()
             for r, ie in enumerate(grp):
                 for t, jf in enumerate(gfs):
                     if plan.ecomps[ie] == plan.fcomps[jf]:
@@ -848,8 +865,10 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         bra_column("col", lambda ie: f"({_R})yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
         E.end()
     if mode == 'schwarz':
+        E.block("if (!coop || lane == 0)")
         E("A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(ssq)) : sqrt(smax);")
         E("if (A.out2) A.out2[A.out_off[qi]] = sqrt(hmax);")
+        E.end()
     if mode == 'fock':
         for i1 in range(ncart(la)):
             for i2 in range(ncart(lb)):

@@ -27,6 +27,13 @@ struct L2gMatDesc {
   long long n;
 };
 
+// sum over the 64 lanes of the wave (butterfly), returned in every lane
+template <class T>
+__device__ __forceinline__ T l2g_wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 struct L2gEriArgs {
   const int* __restrict__ bra_pair;
   const int* __restrict__ ket_pair;
@@ -47,6 +54,7 @@ struct L2gEriArgs {
   int shg_gaussian;
   int uniform_cart;
   int schwarz_norm;  // Schwarz kernels: 0 = max-abs norm, 1 = Frobenius norm
+  int coop;          // Schwarz kernels: one wave per pair, its lanes splitting the primitive loop
   double* __restrict__ out;
   double* __restrict__ ybuf;
   long long ystride;

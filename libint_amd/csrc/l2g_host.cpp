@@ -795,7 +795,8 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
   upload(pNs, pb.ns, st);
   dK.alloc(npairs * sizeof(double) * (S ? 2 : 1));
   const int block = 128;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
+  // ybuf columns for the largest launch: one lane per pair, or a wave per pair
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((64 * maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
   const int64_t ystride = (int64_t)grid * block;
   yb.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
   L2gEriArgs A;
@@ -827,7 +828,17 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
     A.out_off = di.as<long long>() + r.second;
     A.n = (int)n;
     if (c < nsch) {
-      l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>(grid, (n + block - 1) / block), block, st);
+      // deep pairs: one wave per pair splits the (p|p) primitive loop, so a
+      // contracted pair's latency is not its whole serial loop
+      double mean = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        const double q = (double)pb.np[pid[c][i]];
+        mean += q * q;
+      }
+      mean /= (double)n;
+      A.coop = mean >= 16.0 ? 1 : 0;
+      const int64_t want = A.coop ? (64 * n + block - 1) / block : (n + block - 1) / block;
+      l2g_gen_schwarz[c].launch(A, (int)std::min<int64_t>((int64_t)ctx->ncu * 8, want), block, st);
       LAUNCHCHK(std::string("Schwarz kernel of class ") + std::to_string(c));
     } else {
       launch_generic(kGenericSchwarz, A, l2g_gen_generic[c - nsch], ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
