@@ -196,7 +196,8 @@ def main():
     ap.add_argument("--fock-atoms", type=int, default=20, help="atoms of the J/K build leg")
     ap.add_argument("--fock-f-atoms", type=int, default=10,
                     help="atoms of the f-shell J/K build leg (0: skip), at the 200-atom recipe's density")
-    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=150, help="requests of the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the serving leg (1/s)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -294,7 +295,7 @@ def main():
         out["fock"] = fock_leg(la, molecule, ctx, args.fock_atoms, reps=2)
         if args.fock_f_atoms:
             out["fock_f"] = fock_leg(la, molecule, ctx, args.fock_f_atoms, reps=2, workload=4, same_density=True)
-        out["serve"] = serve_leg(la, molecule, args.serve_requests, rate_hz=20.0)
+        out["serve"] = serve_leg(la, molecule, args.serve_requests, rate_hz=args.serve_rate)
         out["p99_fock_build_ms"] = out["serve"]["p99_ms"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol.basis, q)
