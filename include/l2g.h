@@ -61,6 +61,11 @@ const char* l2g_last_error(void);
  * (libint: LIBINT2_MAX_AM_eri, engine.impl.h:620-622). */
 int l2g_max_l(void);
 
+/* Largest l for which FP32 kernel variants exist: in mixed precision a
+ * quartet with a shell above it runs in FP64 whatever its bounds.  No
+ * counterpart in the reference (FP64 only). */
+int l2g_fp32_max_l(void);
+
 /* Opens a device context: selects the HIP device, creates a stream, builds
  * and uploads the Boys-function table.  Replaces libint2::initialize()
  * (include/libint2/initialize.h:76-90) for this path. */

@@ -45,6 +45,7 @@ ABI = {
     "l2g_version": (ctypes.c_char_p, []),
     "l2g_last_error": (ctypes.c_char_p, []),
     "l2g_max_l": (ctypes.c_int, []),
+    "l2g_fp32_max_l": (ctypes.c_int, []),
     "l2g_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "l2g_finalize": (ctypes.c_int, [_vp]),
     "l2g_set_solid_harmonics_ordering": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -144,6 +145,12 @@ def _check(rc: int) -> None:
 
 def max_l() -> int:
     return lib().l2g_max_l()
+
+
+def fp32_max_l() -> int:
+    """Largest l with FP32 kernel variants (mixed precision runs quartets
+    with a higher l in FP64)."""
+    return lib().l2g_fp32_max_l()
 
 
 def shell_renorm(l: int, alpha, coeff, enforce_unit: bool = True):

@@ -139,6 +139,15 @@ const char* l2g_last_error(void) { return g_err.c_str(); }
 
 int l2g_max_l(void) { return lmax_supported(); }
 
+int l2g_fp32_max_l(void) {
+  int m = -1;
+  for (int i = 0; i < l2g_gen_nclasses32; ++i) {
+    const L2gClassInfo& c = l2g_gen_classes32[i];
+    if (c.launch) m = std::max(m, std::max(std::max(c.la, c.lb), std::max(c.lc, c.ld)));
+  }
+  return m;
+}
+
 int l2g_shell_renorm(int l, int nprim, const double* alpha, double* coeff, double* max_ln_coeff, int enforce_unit) {
   return guarded([&] {
     if (l < 0 || nprim <= 0 || !alpha || !coeff) throw std::invalid_argument("l2g_shell_renorm: bad arguments");

@@ -25,14 +25,17 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_mixed_eri_per_class_errors(ctx):
-    mol = molecule.make_molecule(2, natoms=6)
+@pytest.mark.parametrize("workload,natoms,nq", [(2, 6, 60000), (3, 3, 12000)])
+def test_mixed_eri_per_class_errors(ctx, workload, natoms, nq):
+    """Workload 2 (s-d) and the high-angular-momentum recipe (s-f, K <= 8,
+    at workload 3's atom density)."""
+    mol = molecule.make_molecule(workload, natoms=natoms, same_density=workload == 3)
     B = mol.basis
     b = la.Basis(ctx, B)
     allq = molecule.unique_quartets(B.nshell)
     rng = np.random.default_rng(4)
-    q = allq[rng.choice(len(allq), 60000, replace=False)]
-    say(f"case: mixed ERI, {len(q)} quartets of workload-2 recipe (6 atoms)")
+    q = allq[rng.choice(len(allq), min(nq, len(allq)), replace=False)]
+    say(f"case: mixed ERI, {len(q)} quartets of the workload-{workload} recipe ({natoms} atoms)")
     plan = la.EriPlan(b, q, fp32_threshold=THR)
     offs = plan.offsets()
     plan.execute()
@@ -50,11 +53,25 @@ def test_mixed_eri_per_class_errors(ctx):
     S[i, j] = S[j, i] = Sv
     KK = K[q[:, 0], q[:, 1]] * K[q[:, 2], q[:, 3]]
     SS = S[q[:, 0], q[:, 1]] * S[q[:, 2], q[:, 3]]
-    sel = (KK < THR) & (SS < la.FP32_HRR_FACTOR * THR)
+    # quartets with a shell above the FP32 variants' l run in FP64 regardless
+    sel = (KK < THR) & (SS < la.FP32_HRR_FACTOR * THR) & (B.l[q].max(axis=1) <= la.fp32_max_l())
     sizes = B.sizes()
     n = sizes[q[:, 0]] * sizes[q[:, 1]] * sizes[q[:, 2]] * sizes[q[:, 3]]
     is32 = np.repeat(sel, n)
     say(f"  {n32} of {len(q)} quartets in FP32 ({int((KK < THR).sum())} below the Schwarz threshold)")
+    # per-class errors of every class, FP32 and FP64 parts
+    ends0 = offs + n
+    cls0 = np.array(["".join(map(str, r)) for r in B.l[q]])
+    for c in sorted(set(cls0)):
+        for part, msk in (("fp64", (cls0 == c) & ~sel), ("fp32", (cls0 == c) & sel)):
+            if not msk.any():
+                continue
+            gm = np.concatenate([g[a:e] for a, e in zip(offs[msk], ends0[msk])])
+            om = np.concatenate([o[a:e] for a, e in zip(offs[msk], ends0[msk])])
+            big = np.abs(om) > 1e-14
+            rel = (np.abs(gm - om)[big] / np.abs(om[big])).max(initial=0)
+            say(f"  class ({c[:2]}|{c[2:]}) {part} x{int(msk.sum())}: max abs {np.abs(gm - om).max():.2e}, "
+                f"max rel (|ref|>1e-14) {rel:.2e}")
     assert n32 > 0 and n32 == int(sel.sum())
     # FP64 quartets at the integral bar, oracle misses settled by brute force
     q64 = ~sel
@@ -69,15 +86,6 @@ def test_mixed_eri_per_class_errors(ctx):
     e32 = np.array([np.abs(g[a:e] - o[a:e]).max() for a, e in zip(offs[sel], ends[sel])])
     ratio = e32 / np.maximum(EPS32 * SS[sel], 1e-300)
     say(f"  FP32 part: max abs {e32.max():.3e}; max err/(eps32 S S) {ratio[e32 > 1e-16].max(initial=0):.2f}")
-    ls = B.l[q]
-    cls = np.array([f"{a}{bb}{c}{d}" for a, bb, c, d in ls])
-    for c in sorted(set(cls[sel])):
-        m = (cls == c) & sel
-        gm = np.concatenate([g[a:e] for a, e in zip(offs[m], ends[m])])
-        om = np.concatenate([o[a:e] for a, e in zip(offs[m], ends[m])])
-        rel = np.abs(gm - om) / np.maximum(np.abs(om), 1e-300)
-        say(f"  class ({c[0]}{c[1]}|{c[2]}{c[3]}) fp32 x{int(m.sum())}: max abs {np.abs(gm - om).max():.2e} "
-            f"max rel (|ref|>1e-14) {rel[np.abs(om) > 1e-14].max(initial=0):.2e}")
     assert nf64 == 0
     # FP32 bar: 1e-5 of the threshold absolute, and within 64 eps_fp32 S S
     assert e32.max() <= 1e-5 * THR
