@@ -713,6 +713,9 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     if mode == 'schwarz':
         E("double hmax = 0.0;  // max |[e0|e0]| over the pre-HRR diagonal")
     xoff = ne * ncol_max  # raw-row region of the scratch column (split rows)
+    # ERI output staged in the scratch column after the row regions
+    staged = mode == 'eri' and not in_regs
+    stage_off = xoff + (ne * nf if plan.split_rows else 0)
     for gi, (grp, gfs) in enumerate(plan.groups):
         ngf = len(gfs)
         accs = [f"acc{gi}_{t}" for t in range(len(grp) * ngf)]
@@ -827,7 +830,9 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
                 for i1 in range(d1):
                     for i2 in range(d2):
                         v = vals[i1 * d2 + i2]
-                        if mode == 'eri':
+                        if mode == 'eri' and staged:
+                            E(f"st.put_staged(yb, A.ystride, {stage_off}, colofs, {i1}, {i2}, colscale, {v});")
+                        elif mode == 'eri':
                             E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {v});")
                         else:
                             E(f"smax = fmax(smax, fabs({v})); ssq += {v} * {v};")
@@ -864,6 +869,8 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         E.block("for (int col = 0; col < ncol; ++col)")
         bra_column("col", lambda ie: f"({_R})yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
         E.end()
+    if mode == 'eri' and staged:
+        E(f"st.flush(A.out, yb, A.ystride, {stage_off});")
     if mode == 'schwarz':
         E.block("if (!coop || lane == 0)")
         E("A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(ssq)) : sqrt(smax);")
@@ -975,6 +982,8 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
         for p, k, in_regs in tab:
             ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld) + \
                 (len(p.ecomps) * len(p.fcomps) if p.split_rows else 0)
+            if not in_regs and name in ("l2g_gen_classes", "l2g_gen_classes32"):
+                ybuf += ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld)  # staged output
             lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
                          f"{sum(p.group_nodes)}, &{k}_launch}},")
         if not tab:

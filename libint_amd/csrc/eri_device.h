@@ -249,6 +249,27 @@ struct L2gStore {
     }
     out[colofs + (size_t)pos(l1, p1, i1) * s1 + (size_t)pos(l2, p2, i2) * s2] = val;
   }
+  // Staged store: the value goes to its final position within the quartet's
+  // block, but in the lane's scratch column (yb[k * ystride], k counted from
+  // `stage`); flush() then writes the block to `out` in address order.  A
+  // lane produces its block one ket column at a time, so direct stores leave
+  // every cache line of the block half-written until the last column, and
+  // the lines are evicted and written back more than once.
+  __device__ __forceinline__ void put_staged(double* yb, size_t ystride, int stage, size_t colofs, int i1, int i2,
+                                             double colscale, double v) const {
+    double val = v;
+    if (uniform) {
+      val *= colscale * (p1 ? 1.0 : l2g_uniform_cart_scale(l1, i1)) * (p2 ? 1.0 : l2g_uniform_cart_scale(l2, i2));
+    }
+    const size_t k = colofs - base + (size_t)pos(l1, p1, i1) * s1 + (size_t)pos(l2, p2, i2) * s2;
+    yb[((size_t)stage + k) * ystride] = val;
+  }
+  __device__ __forceinline__ void flush(double* __restrict__ out, const double* yb, size_t ystride, int stage) const {
+    const int n = n1 * n2 * nk1 * nk2;
+    const double* src = yb + (size_t)stage * ystride;
+    double* dst = out + base;
+    for (int k = 0; k < n; ++k) dst[k] = src[(size_t)k * ystride];
+  }
 };
 
 __device__ __forceinline__ L2gStore make_store(int meta, const L2gEriArgs& A, int qi, int l1, int l2, int l3,
