@@ -149,6 +149,66 @@ def fock_leg(la, molecule, ctx, natoms, reps, workload=2, same_density=False):
     return out
 
 
+def eri_f_leg(la, molecule, flop_model, ctx, nquartets, reps=3, seed=21):
+    """Batched ERIs of the high-angular-momentum recipe (120 atoms in a 20 A
+    cube, s4p3d2f1, K~U{1..8}): a seeded sample of quartets with
+    K_ab K_cd >= 1e-12 over every class up to (ff|ff), FP64 and mixed
+    (FP32 where K_ab K_cd < 1e-6 and S_ab S_cd < 16e-6, l <= the FP32 kernels'
+    l), and FP64 with the opt-in wave split (outside the per-integral parity
+    bar); quartets/s and model TFLOP/s of the whole batch, and the classes
+    that take the most time."""
+    mol = molecule.make_molecule(3)
+    B = mol.basis
+    b = la.Basis(ctx, B)
+    i, j = np.tril_indices(B.nshell)
+    K = la.schwarz(b, np.stack([i, j], 1))
+    keep = K >= 1e-9
+    i, j, K = i[keep], j[keep], K[keep]
+    rng = np.random.default_rng(seed)
+    picks = []
+    got = 0
+    while got < nquartets:
+        p1 = rng.integers(0, len(K), 4 * nquartets)
+        p2 = rng.integers(0, len(K), 4 * nquartets)
+        ok = K[p1] * K[p2] >= 1e-12
+        picks.append((p1[ok], p2[ok]))
+        got += int(ok.sum())
+    p1 = np.concatenate([p[0] for p in picks])[:nquartets]
+    p2 = np.concatenate([p[1] for p in picks])[:nquartets]
+    q = np.stack([i[p1], j[p1], i[p2], j[p2]], 1).astype(np.int32)
+    out = {"workload": f"120 atoms, 20 A cube, s4p3d2f1/atom, K~U{{1..8}}, exponents [0.03,2e4]; "
+                       f"{nquartets} quartets drawn (seed {seed}) with K_ab K_cd >= 1e-12, all classes to (ff|ff)"}
+    for name, thr, wave in (("fp64", 0.0, False), ("fp64_wave_split", 0.0, True), ("mixed", 1e-6, False)):
+        plan = la.EriPlan(b, q, fp32_threshold=thr)
+        if wave:  # opt-in: reordered contraction sums for classes of few deep quartets
+            plan.set_wave_split(True)
+        d = plan.device_output()
+        plan.execute(d)
+        ctx.synchronize()
+        ms = []
+        for _ in range(reps):
+            plan.execute(d)
+            ctx.synchronize()
+            ms.append(plan.elapsed_ms())
+        st = plan.stats()
+        t = float(np.median(ms))
+        fl = sum(flop_model.class_flops(*(int(x) for x in c["cls"]), c["nquartets"], c["nprim"]) for c in st)
+        top = sorted(st, key=lambda c: -c["ms"])[:6]
+        out[name] = {"ms": t, "quartets_per_s": nquartets / (t * 1e-3), "model_tflops": fl / (t * 1e-3) / 1e12,
+                     "quartets_fp32": sum(c["nquartets"] for c in st if c["fp32"]),
+                     "top_classes": [f"{c['cls']}{'/fp32' if c['fp32'] else ''}{'/generic' if c['generic'] else ''}"
+                                     f"{'/wave' if c['wave'] else ''}: "
+                                     f"{c['ms']:.1f} ms, {c['nquartets']} quartets, "
+                                     f"{flop_model.class_flops(*(int(x) for x in c['cls']), c['nquartets'], c['nprim']) / (c['ms'] * 1e9):.2f} TFLOP/s"
+                                     for c in top]}
+        log(f"eri f-recipe {name}: {t:.1f} ms, {out[name]['quartets_per_s']:.3e} quartets/s, "
+            f"{out[name]['model_tflops']:.2f} TFLOP/s model")
+        del plan
+    out["mixed_speedup"] = out["fp64"]["ms"] / out["mixed"]["ms"]
+    del b
+    return out
+
+
 def serve_leg(la, molecule, nreq, rate_hz, seed=5000):
     """Fock-build requests for molecules of 2-12 atoms (log-uniform; the two
     recipes alternate) arriving as a Poisson stream, served with continuous
@@ -196,6 +256,8 @@ def main():
     ap.add_argument("--fock-atoms", type=int, default=20, help="atoms of the J/K build leg")
     ap.add_argument("--fock-f-atoms", type=int, default=10,
                     help="atoms of the f-shell J/K build leg (0: skip), at the 200-atom recipe's density")
+    ap.add_argument("--eri-f-quartets", type=int, default=400000,
+                    help="quartets of the high-angular-momentum ERI leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=150, help="requests of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the serving leg (1/s)")
     args = ap.parse_args()
@@ -291,6 +353,8 @@ def main():
         "setup_s": t_setup,
     }
     del plan
+    if not args.no_fock and args.eri_f_quartets:
+        out["eri_f"] = eri_f_leg(la, molecule, flop_model, ctx, args.eri_f_quartets)
     if not args.no_fock:
         out["fock"] = fock_leg(la, molecule, ctx, args.fock_atoms, reps=2)
         if args.fock_f_atoms:

@@ -167,7 +167,9 @@ int l2g_eri_plan_device_output(l2g_eri_plan plan, double** d_out);
  * (up to cap), the class (la,lb,lc,ld) packed as la*1000+lb*100+lc*10+ld,
  * the number of quartets, the number of primitive quartets (before the
  * per-primitive screen) and the duration in ms of its last execution
- * (HIP events on the context stream). */
+ * (HIP events on the context stream).  The code adds 10000 for an FP32
+ * kernel, 20000 for the generic kernel and 100000 for a wave per quartet
+ * (l2g_eri_plan_set_wave_split). */
 int l2g_eri_plan_stats(l2g_eri_plan plan, int cap, int* nclasses, int* class_code, int64_t* nquartets,
                        int64_t* nprim_quartets, float* ms);
 
@@ -180,6 +182,16 @@ int l2g_eri_plan_stats(l2g_eri_plan plan, int cap, int* nclasses, int* class_cod
  * Engine::compute per shell quartet (engine.impl.h:140-178); it removes the
  * per-class launch overhead of repeated executions of one plan. */
 int l2g_eri_plan_set_graph(l2g_eri_plan plan, int on);
+
+/* Wave split (on != 0; off by default): classes of few, deeply contracted
+ * quartets run a wave per quartet whose 64 lanes split the primitive loop,
+ * instead of one lane per quartet, so that a handful of long contractions no
+ * longer serialise on a few lanes.  The contraction is then summed in another
+ *  order than libint's serial primitive loop (engine.impl.h:1306-1766), which
+ * changes ill-conditioned integrals (large cancelling primitive terms) by up
+ * to ~1e-10 relative: outside the 1e-12 parity bar of the default mode.
+ * No counterpart in the reference. */
+int l2g_eri_plan_set_wave_split(l2g_eri_plan plan, int on);
 
 /* Duration in ms of the plan's last execution (all classes; HIP events on
  * the context stream; waits for it to finish). */

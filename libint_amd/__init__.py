@@ -65,6 +65,7 @@ ABI = {
     "l2g_eri_plan_device_output": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "l2g_eri_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
     "l2g_eri_plan_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_eri_plan_set_wave_split": (ctypes.c_int, [_vp, ctypes.c_int]),
     "l2g_eri_plan_elapsed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
     "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
@@ -363,7 +364,8 @@ class EriPlan:
         _check(lib().l2g_eri_plan_stats(self._h, cap, ctypes.byref(n), code.ctypes.data, nq.ctypes.data,
                                         npq.ctypes.data, ms.ctypes.data))
         k = n.value
-        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c // 10000 == 1), generic=bool(c // 10000 == 2),
+        return [dict(cls=f"{c % 10000:04d}", fp32=bool(c % 100000 // 10000 == 1),
+                     generic=bool(c % 100000 // 10000 == 2), wave=bool(c >= 100000),
                      nquartets=int(a), nprim=int(b), ms=float(t))
                 for c, a, b, t in zip(code[:k], nq[:k], npq[:k], ms[:k])]
 
@@ -371,6 +373,12 @@ class EriPlan:
         """Capture the class launches into a hipGraph on the next execution
         and replay it on later executions into the same output buffer."""
         _check(lib().l2g_eri_plan_set_graph(self._h, int(bool(on))))
+
+    def set_wave_split(self, on=True):
+        """Classes of few, deeply contracted quartets run a wave per quartet
+        (lanes split the primitive loop); sums in another order than libint,
+        so ill-conditioned integrals move by up to ~1e-10 relative."""
+        _check(lib().l2g_eri_plan_set_wave_split(self._h, int(bool(on))))
 
     def elapsed_ms(self):
         """Duration of the last execution (all classes), waiting for it."""

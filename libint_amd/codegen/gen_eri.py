@@ -656,8 +656,8 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     E.ind += 1
     E("const int tid = blockIdx.x * blockDim.x + threadIdx.x;")
     E("const int nthr = gridDim.x * blockDim.x;")
-    if mode == 'schwarz':
-        E("// A.coop: a wave per pair, each lane taking a contiguous 1/64 of the")
+    if mode in ('schwarz', 'eri'):
+        E("// A.coop: a wave per work item, each lane taking a contiguous 1/64 of the")
         E("// primitive quartets; the group accumulators are summed over the wave")
         E("const bool coop = A.coop != 0;")
         E("const int lane = threadIdx.x & 63;")
@@ -681,7 +681,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         E("pbeg = pr.x;")
         E("pend = pr.y;")
         E.end()
-    if mode == 'schwarz':
+    if mode in ('schwarz', 'eri'):
         E.block("if (coop)")
         E("const int len = pend - pbeg, b0 = pbeg + (int)(((long long)len * lane) >> 6);")
         E("pend = pbeg + (int)(((long long)len * (lane + 1)) >> 6);")
@@ -725,11 +725,12 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         for a in accs:
             E(f"{_R} {a} = 0;")
         emit_prim_loop(E, plan, graph, gi, accs)
-        if mode == 'schwarz':
+        if mode in ('schwarz', 'eri'):
             E.block("if (coop)")
             for a in accs:
                 E(f"{a} = l2g_wave_sum({a});")
             E.end()
+        if mode == 'schwarz':
             for r, ie in enumerate(grp):
                 for t, jf in enumerate(gfs):
                     if plan.ecomps[ie] == plan.fcomps[jf]:
@@ -769,7 +770,8 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
         E.end()  # group block
     # bra HRR + bra sph + permuted store, column by column over the ket
     if mode == 'eri':
-        E("if (A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
+        E("if (coop) nsurv = l2g_wave_sum(nsurv);")
+        E("if (A.nz && (!coop || lane == 0)) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
     elif mode == 'schwarz':
         E("double smax = 0.0, ssq = 0.0;")
     else:
@@ -831,7 +833,9 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
                     for i2 in range(d2):
                         v = vals[i1 * d2 + i2]
                         if mode == 'eri' and staged:
-                            E(f"st.put_staged(yb, A.ystride, {stage_off}, colofs, {i1}, {i2}, colscale, {v});")
+                            # a wave-cooperative item spreads its columns over the lanes
+                            E(f"if (coop) st.put(A.out, colofs, {i1}, {i2}, colscale, {v}); "
+                              f"else st.put_staged(yb, A.ystride, {stage_off}, colofs, {i1}, {i2}, colscale, {v});")
                         elif mode == 'eri':
                             E(f"st.put(A.out, colofs, {i1}, {i2}, colscale, {v});")
                         else:
@@ -862,15 +866,21 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
 
     if in_regs:
         for col in range(ncol_max):
-            E.block(f"if ({col} < ncol)")
+            if mode == 'eri':
+                E.block(f"if ({col} < ncol && (!coop || lane == {col % 64}))")
+            else:
+                E.block(f"if ({col} < ncol)")
             bra_column(str(col), lambda ie, c=col: f"Y[{ie}][{c}]")
             E.end()
     else:
-        E.block("for (int col = 0; col < ncol; ++col)")
+        if mode == 'eri':  # a wave-cooperative item spreads its columns over the lanes
+            E.block("for (int col = coop ? lane : 0; col < ncol; col += coop ? 64 : 1)")
+        else:
+            E.block("for (int col = 0; col < ncol; ++col)")
         bra_column("col", lambda ie: f"({_R})yb[((size_t)({ie * ncol_max}) + col) * A.ystride]")
         E.end()
     if mode == 'eri' and staged:
-        E(f"st.flush(A.out, yb, A.ystride, {stage_off});")
+        E(f"if (!coop) st.flush(A.out, yb, A.ystride, {stage_off});")
     if mode == 'schwarz':
         E.block("if (!coop || lane == 0)")
         E("A.out[A.out_off[qi]] = A.schwarz_norm ? sqrt(sqrt(ssq)) : sqrt(smax);")

@@ -84,6 +84,7 @@ struct ClassWork {
   int cls;  // index into l2g_gen_classes (l2g_gen_classes32 when fp32, l2g_gen_generic when generic)
   bool fp32 = false;
   bool generic = false;
+  bool coop = false;  // a wave per quartet (few, deeply contracted quartets)
   int64_t first, count;
   int64_t nprimq;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -103,8 +104,9 @@ struct l2g_eri_plan_s {
   DevBuf bra, ket, off, meta, qidx;
   DevBuf pair_first, pair_np, pair_AB, ppP, ppPA, ppAux, ppNs;
   DevBuf out, ybuf, nz;
-  int64_t ystride = 0;
+  int64_t ystride = 0, ymax = 0;
   int grid = 0, block = 128;
+  bool wave_split = false;  // classes of few deep quartets run a wave per quartet
   // hipGraph mode: the class launches are captured once per output buffer
   // and replayed; per-class timings are then not recorded
   bool use_graph = false;
@@ -127,6 +129,11 @@ struct l2g_eri_plan_s {
     if (t1) (void)hipEventDestroy(t1);
   }
 };
+
+namespace {
+void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx);  // launch layout, defined below
+}  // namespace
+
 
 // ===========================================================================
 // C ABI
@@ -502,19 +509,42 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
     upload(plan->ppPA, pb.PA, st);
     upload(plan->ppAux, pb.aux, st);
     upload(plan->ppNs, pb.ns, st);
-    // persistent grid: enough blocks to fill every CU several times over
-    plan->block = 128;
-    const int64_t maxblocks = (int64_t)ctx->ncu * 8;
-    int64_t need = 0;
-    for (const auto& c : plan->classes) need = std::max(need, (c.count + plan->block - 1) / plan->block);
-    plan->grid = (int)std::max<int64_t>(1, std::min(need, maxblocks));
-    plan->ystride = (int64_t)plan->grid * plan->block;
-    plan->ybuf.alloc(std::max<int64_t>(ymax * plan->ystride, 1) * sizeof(double));
+    plan->ymax = ymax;
+    plan_layout(plan.get(), ctx);
     HIPCHK(hipStreamSynchronize(st));
     ++ctx->live;
     *out = plan.release();
   });
 }
+
+}  // extern "C"
+
+namespace {
+
+// Launch layout of a plan: a persistent grid with enough blocks to fill
+// every CU several times over, and the lane-private scratch for it.  With
+// wave_split, a class of few quartets (fewer than a quarter of the chip's
+// lanes at one wave per SIMD) with deep primitive loops runs a wave per
+// quartet whose lanes split the loop; one lane per quartet would leave most
+// of the chip idle while a few lanes run their loops serially.  The split
+// sums each contraction in another order than libint's serial loop.
+void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
+  p->block = 128;
+  const int64_t maxblocks = (int64_t)ctx->ncu * 8;
+  int64_t need = 0;
+  for (auto& c : p->classes) {
+    c.coop = p->wave_split && !c.generic && c.count * 4 <= (int64_t)ctx->ncu * 256 && c.nprimq >= 16 * c.count;
+    const int64_t lanes = c.coop ? 64 * c.count : c.count;
+    need = std::max(need, (lanes + p->block - 1) / p->block);
+  }
+  p->grid = (int)std::max<int64_t>(1, std::min(need, maxblocks));
+  p->ystride = (int64_t)p->grid * p->block;
+  p->ybuf.alloc(std::max<int64_t>(p->ymax * p->ystride, 1) * sizeof(double));
+}
+
+}  // namespace
+
+extern "C" {
 
 int l2g_eri_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening, double scale,
                         int uniform_cart, l2g_eri_plan* out) {
@@ -608,7 +638,9 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
                        ctx->stream);
       } else {
         const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
-        const int grid = (int)std::min<int64_t>(p->grid, (c.count + p->block - 1) / p->block);
+        A.coop = c.coop ? 1 : 0;
+        const int64_t lanes = c.coop ? 64 * c.count : c.count;
+        const int grid = (int)std::min<int64_t>(p->grid, (lanes + p->block - 1) / p->block);
         ci.launch(A, grid, p->block, ctx->stream);
         LAUNCHCHK(std::string("ERI kernel of class ") + std::to_string(c.cls));
       }
@@ -644,6 +676,18 @@ int l2g_eri_plan_set_graph(l2g_eri_plan p, int on) {
     HIPCHK(hipStreamSynchronize(p->basis->ctx->stream));
     p->use_graph = on != 0;
     if (!p->use_graph) p->drop_graph();
+  });
+}
+
+int l2g_eri_plan_set_wave_split(l2g_eri_plan p, int on) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    l2g_context_s* ctx = p->basis->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    p->wave_split = on != 0;
+    p->drop_graph();
+    plan_layout(p, ctx);
   });
 }
 
@@ -704,7 +748,9 @@ int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* n
         const auto& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
         la = ci.la, lb = ci.lb, lc = ci.lc, ld = ci.ld;
       }
-      if (code) code[i] = la * 1000 + lb * 100 + lc * 10 + ld + (c.fp32 ? 10000 : 0) + (c.generic ? 20000 : 0);
+      if (code)
+        code[i] = la * 1000 + lb * 100 + lc * 10 + ld + (c.fp32 ? 10000 : 0) + (c.generic ? 20000 : 0) +
+                  (c.coop ? 100000 : 0);
       if (nquart) nquart[i] = c.count;
       if (nprimq) nprimq[i] = c.nprimq;
       if (ms) {

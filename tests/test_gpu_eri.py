@@ -244,3 +244,49 @@ def test_graph_replay_matches_eager(ctx):
         assert all(s["ms"] >= 0 for s in P.stats())
         for x in (P, Z, other, H):
             x._release()
+
+
+def test_wave_split(ctx):
+    """Opt-in wave per quartet for classes of few deep quartets: classes left
+    in lane mode are bit-identical to the default plan; wave classes sum the
+    contraction in another order, so they are held to 1e-10 absolute or
+    1e-8 relative against the oracle (ill-conditioned f-shell quartets move
+    by up to ~1e-11 absolute), not to the per-integral parity bar."""
+    mol = molecule.make_molecule(3, natoms=3, same_density=True)
+    B = mol.basis
+    b = la.Basis(ctx, B)
+    allq = molecule.unique_quartets(B.nshell)
+    rng = np.random.default_rng(4)
+    q = allq[rng.choice(len(allq), 12000, replace=False)]
+    say(f"case: wave split, {len(q)} quartets of the workload-3 recipe (3 atoms)")
+    ref = la.EriPlan(b, q)
+    ref.execute()
+    r, rnz = ref.download()
+    plan = la.EriPlan(b, q)
+    plan.set_wave_split(True)
+    plan.execute()
+    g, gnz = plan.download()
+    st = plan.stats()
+    wave = {s["cls"] for s in st if s["wave"]}
+    say(f"  wave classes: {sorted(wave)}")
+    assert wave
+    assert np.array_equal(gnz, rnz)
+    offs = plan.offsets()
+    sizes = B.sizes()
+    n = sizes[q[:, 0]] * sizes[q[:, 1]] * sizes[q[:, 2]] * sizes[q[:, 3]]
+    # canonical class of each quartet: bra pair ordered (la >= lb), the
+    # lighter pair first
+    ls = B.l[q]
+    bra = np.sort(ls[:, :2], axis=1)[:, ::-1]
+    ket = np.sort(ls[:, 2:], axis=1)[:, ::-1]
+    swap = bra.sum(1) > ket.sum(1)
+    c1 = np.where(swap[:, None], ket, bra)
+    c2 = np.where(swap[:, None], bra, ket)
+    cls = np.array([f"{a}{b_}{c}{d}" for (a, b_), (c, d) in zip(c1, c2)])
+    inw = np.isin(cls, list(wave))
+    iw = np.repeat(inw, n)
+    assert np.array_equal(g[~iw], r[~iw])
+    o, _, _ = oracle.eri_quartets(B, q)
+    nfail, mabs, mrel = compare(g[iw], o[iw], 1e-10, 1e-8)
+    say(f"  wave part ({int(inw.sum())} quartets): max abs {mabs:.2e}, fails {nfail}")
+    assert nfail == 0
