@@ -534,12 +534,30 @@ struct ItemsDev {
 // grid and scratch as sized by the caller.
 // With `ev` (2 events per class), records the class's launch between
 // ev[2c] and ev[2c+1].
+// Launches every class of an item set.  With slots (plan_slots over the
+// classes), each class gets its own grid and scratch region, and the classes
+// too small to fill the chip run on the side streams beside each other;
+// without, every class runs in order on st with the scratch in A.
 void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, bool with_req, int grid, int block,
-                     hipStream_t st, hipEvent_t* ev = nullptr) {
+                     hipStream_t st, hipEvent_t* ev = nullptr, const std::vector<ClassSlot>* slots = nullptr) {
+  double* const ybase = A.ybuf;
+  bool sides = false;
+  if (slots)
+    for (size_t c = 0; c < items.count.size(); ++c) sides |= items.count[c] > 0 && (*slots)[c].side >= 0;
+  if (sides) fork_sides(*ctx);
   for (size_t c = 0; c < items.count.size(); ++c) {
     const int64_t m = items.count[c], o = items.first[c];
     if (m == 0) continue;
-    if (ev) HIPCHK(hipEventRecord(ev[2 * c], st));
+    hipStream_t cs = st;
+    int g = (int)std::min<int64_t>(grid, (m + block - 1) / block);
+    if (slots) {
+      const ClassSlot& sl = (*slots)[c];
+      if (sides && sl.side >= 0) cs = ctx->side[sl.side];
+      A.ybuf = ybase + sl.yoff;
+      A.ystride = sl.ystride;
+      g = sl.grid;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[2 * c], cs));
     A.bra_pair = items.bra + o;
     A.ket_pair = items.ket + o;
     A.meta = items.meta + o;
@@ -551,15 +569,15 @@ void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, b
     A.n = (int)m;
     if (fock_is_generic((int)c)) {
       launch_generic(kGenericFock, A, fock_gclass((int)c), ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
-                     (int)std::min<int64_t>((int64_t)ctx->ncu * 8, m), st);
+                     (int)std::min<int64_t>((int64_t)ctx->ncu * 8, m), cs);
     } else {
-      const int g = (int)std::min<int64_t>(grid, (m + block - 1) / block);
-      fock_info((int)c).launch(A, g, block, st);
+      fock_info((int)c).launch(A, g, block, cs);
       LAUNCHCHK(std::string("Fock kernel of class ") + std::to_string(c) + " (grid " + std::to_string(g) +
                 ", block " + std::to_string(block) + ", items " + std::to_string(m) + ")");
     }
-    if (ev) HIPCHK(hipEventRecord(ev[2 * c + 1], st));
+    if (ev) HIPCHK(hipEventRecord(ev[2 * c + 1], cs));
   }
+  if (sides) join_sides(*ctx);
 }
 
 }  // namespace
@@ -759,16 +777,22 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       HIPCHK(hipMemsetAsync(f->kg.p, 0, NN * sizeof(double), st));
     }
     const int block = 128;
-    int64_t maxn = 0, ymax = 0;
+    int64_t ymax = 0;
+    std::vector<int64_t> lanes(ncls, 0), yd(ncls, 0);
     for (int c = 0; c < ncls; ++c) {
       const int64_t m = big.count[c];
       if (m == 0) continue;
-      maxn = std::max(maxn, m);
-      ymax = std::max<int64_t>(ymax, fock_ybuf(c));
+      // the generic kernel runs a block per item and no lane scratch
+      lanes[c] = fock_is_generic(c) ? m * 256 : m;
+      yd[c] = fock_ybuf(c);
+      ymax = std::max<int64_t>(ymax, yd[c]);
     }
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + block - 1) / block, (int64_t)ctx->ncu * 8));
-    const int64_t ystride = (int64_t)grid * block;
-    f->ybuf.alloc(std::max<int64_t>(ymax * ystride, 1) * sizeof(double));
+    std::vector<ClassSlot> slots;
+    const int64_t ytotal = plan_slots(slots, lanes, yd, ctx->ncu, block, l2g_context_s::kSide,
+                                      ymax * (int64_t)ctx->ncu * 8 * block + (int64_t(1) << 29));
+    f->ybuf.alloc(std::max<int64_t>(ytotal, 1) * sizeof(double));
+    const int grid = (int)ctx->ncu * 8;
+    const int64_t ystride = 0;  // per class, from its slot
     const PairDev& p = f->pairs;
     L2gEriArgs A = fock_args(ctx, fs.method, p.first.as<int>(), p.np.as<int>(), p.AB.as<double>(), p.P.as<double4>(),
                              p.PA.as<double4>(), p.aux.as<double2>(), p.ns.as<double>(), f->dens.as<double>(),
@@ -780,7 +804,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     }
     hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
     HIPCHK(hipEventRecord(e0, st));
-    fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data());
+    fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data(), &slots);
     HIPCHK(hipEventRecord(e1, st));
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);
@@ -1193,7 +1217,17 @@ void run_round(l2g_server_s* S, const std::vector<ServerRequest*>& act) {
                            S->a_P.as<double4>(), S->a_PA.as<double4>(), S->a_aux.as<double2>(), S->a_ns.as<double>(),
                            S->a_dens.as<double>(), S->a_jg.as<double>(), S->a_kg.as<double>(),
                            S->rmats[b].as<L2gMatDesc>(), S->ybuf.as<double>(), S->ystride);
-  fock_launch_all(ctx, A, S->rbuf[b].view(), true, S->grid, S->block, st);
+  const ItemsView rv = S->rbuf[b].view();
+  std::vector<int64_t> lanes(rv.count.size(), 0), yd(rv.count.size(), 0);
+  for (size_t c = 0; c < rv.count.size(); ++c) {
+    if (rv.count[c] == 0) continue;
+    lanes[c] = fock_is_generic((int)c) ? rv.count[c] * 256 : rv.count[c];
+    yd[c] = fock_ybuf((int)c);
+  }
+  std::vector<ClassSlot> slots;  // within the server's scratch (sized for the full grid)
+  (void)plan_slots(slots, lanes, yd, ctx->ncu, S->block, l2g_context_s::kSide,
+                   (int64_t)(S->ybuf.bytes / sizeof(double)));
+  fock_launch_all(ctx, A, rv, true, S->grid, S->block, st, nullptr, &slots);
   HIPCHK(hipEventRecord(S->rev[b], st));
   // requests whose last quartets are in this round: finish on the device and
   // copy J, K, G to their pinned host buffers

@@ -86,6 +86,7 @@ struct ClassWork {
   bool generic = false;
   bool coop = false;  // a wave per quartet (few, deeply contracted quartets)
   int64_t first, count;
+  ClassSlot slot;     // grid, scratch region and stream of the launch
   int64_t nprimq;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float ms = 0;
@@ -104,8 +105,8 @@ struct l2g_eri_plan_s {
   DevBuf bra, ket, off, meta, qidx;
   DevBuf pair_first, pair_np, pair_AB, ppP, ppPA, ppAux, ppNs;
   DevBuf out, ybuf, nz;
-  int64_t ystride = 0, ymax = 0;
-  int grid = 0, block = 128;
+  int64_t ymax = 0;  // largest scratch doubles per lane over the classes
+  int block = 128;
   bool wave_split = false;  // classes of few deep quartets run a wave per quartet
   // hipGraph mode: the class launches are captured once per output buffer
   // and replayed; per-class timings are then not recorded
@@ -232,6 +233,11 @@ int l2g_init(int device, l2g_context* out) {
     HIPCHK(hipGetDeviceProperties(&prop, device));
     ctx->ncu = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (int k = 0; k < l2g_context_s::kSide; ++k) {
+      HIPCHK(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     const auto& t = boys_table().t;
     ctx->boys.alloc(t.size() * sizeof(double));
     HIPCHK(hipMemcpy(ctx->boys.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -297,6 +303,11 @@ int l2g_finalize(l2g_context ctx) {
                                       " basis/plan/Fock object(s) of this context are still alive; destroy them first");
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int k = 0; k < l2g_context_s::kSide; ++k) {
+      if (ctx->side[k]) (void)hipStreamSynchronize(ctx->side[k]), (void)hipStreamDestroy(ctx->side[k]);
+      if (ctx->join[k]) (void)hipEventDestroy(ctx->join[k]);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });
@@ -530,16 +541,19 @@ namespace {
 // sums each contraction in another order than libint's serial loop.
 void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
   p->block = 128;
-  const int64_t maxblocks = (int64_t)ctx->ncu * 8;
-  int64_t need = 0;
+  std::vector<int64_t> lanes, yd;
   for (auto& c : p->classes) {
     c.coop = p->wave_split && !c.generic && c.count * 4 <= (int64_t)ctx->ncu * 256 && c.nprimq >= 16 * c.count;
-    const int64_t lanes = c.coop ? 64 * c.count : c.count;
-    need = std::max(need, (lanes + p->block - 1) / p->block);
+    // the generic kernel runs a block per quartet and no lane scratch
+    lanes.push_back(c.generic ? c.count * 256 : (c.coop ? 64 * c.count : c.count));
+    yd.push_back(c.generic ? 0 : (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].ybuf_doubles);
   }
-  p->grid = (int)std::max<int64_t>(1, std::min(need, maxblocks));
-  p->ystride = (int64_t)p->grid * p->block;
-  p->ybuf.alloc(std::max<int64_t>(p->ymax * p->ystride, 1) * sizeof(double));
+  std::vector<ClassSlot> slots;
+  // side regions up to 4 GiB of scratch beyond the shared one
+  const int64_t total = plan_slots(slots, lanes, yd, ctx->ncu, p->block, l2g_context_s::kSide,
+                                   p->ymax * (int64_t)ctx->ncu * 8 * p->block + (int64_t(1) << 29));
+  for (size_t i = 0; i < p->classes.size(); ++i) p->classes[i].slot = slots[i];
+  p->ybuf.alloc(std::max<int64_t>(total, 1) * sizeof(double));
 }
 
 }  // namespace
@@ -601,8 +615,6 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     A.shg_gaussian = ctx->shg_gaussian;
     A.uniform_cart = p->uniform;
     A.out = out;
-    A.ybuf = p->ybuf.as<double>();
-    A.ystride = p->ystride;
     p->nz.alloc(std::max<int64_t>(p->nq, 1));
     A.nz = p->nz.as<signed char>();
     if (!p->t0) {
@@ -622,30 +634,38 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     } else {
       HIPCHK(hipEventRecord(p->t0, ctx->stream));
     }
+    // classes too small to fill the chip run beside each other on the side
+    // streams (not under capture, where the launches stay in order)
+    bool sides = false;
+    for (auto& c : p->classes) sides |= c.slot.side >= 0;
+    sides &= !capture;
     auto launch_all = [&] {
+    if (sides) fork_sides(*ctx);
     for (auto& c : p->classes) {
+      hipStream_t cs = sides && c.slot.side >= 0 ? ctx->side[c.slot.side] : ctx->stream;
+      A.ybuf = p->ybuf.as<double>() + c.slot.yoff;
+      A.ystride = c.slot.ystride;
       A.bra_pair = p->bra.as<int>() + c.first;
       A.ket_pair = p->ket.as<int>() + c.first;
       A.meta = p->meta.as<int>() + c.first;
       A.out_off = p->off.as<long long>() + c.first;
       A.qidx = p->qidx.as<long long>() + c.first;
       A.n = (int)c.count;
-      if (!capture) HIPCHK(hipEventRecord(c.ev0, ctx->stream));
+      if (!capture) HIPCHK(hipEventRecord(c.ev0, cs));
       if (c.generic) {
         // one block per quartet, persistent over the class
         const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
-        launch_generic(kGenericEri, A, l2g_gen_generic[c.cls], ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid,
-                       ctx->stream);
+        launch_generic(kGenericEri, A, l2g_gen_generic[c.cls], ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid, cs);
       } else {
         const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
         A.coop = c.coop ? 1 : 0;
-        const int64_t lanes = c.coop ? 64 * c.count : c.count;
-        const int grid = (int)std::min<int64_t>(p->grid, (lanes + p->block - 1) / p->block);
-        ci.launch(A, grid, p->block, ctx->stream);
+        // exactly the slot's grid: the scratch region holds grid * block lanes
+        ci.launch(A, c.slot.grid, p->block, cs);
         LAUNCHCHK(std::string("ERI kernel of class ") + std::to_string(c.cls));
       }
-      if (!capture) HIPCHK(hipEventRecord(c.ev1, ctx->stream));
+      if (!capture) HIPCHK(hipEventRecord(c.ev1, cs));
     }
+    if (sides) join_sides(*ctx);
     };
     if (!capture) {
       launch_all();

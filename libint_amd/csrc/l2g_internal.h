@@ -165,11 +165,71 @@ void parallel_for(int n, F&& f) {
   for (auto& t : th) t.join();
 }
 
+// Launch slots of a batch of classes.  A class whose lanes fit one wave per
+// SIMD of the chip cannot fill it alone: such classes run beside each other
+// on the side streams, each with its own lane-private scratch region; the
+// larger classes run in order on the main stream and share one region.
+// lanes[i] is the class's lanes (0: class absent), ydoubles[i] its scratch
+// doubles per lane; a class launches exactly slots[i].grid blocks of `block`
+// threads (with ystride = grid * block).  Returns the scratch doubles needed;
+// when the side regions would exceed cap_doubles every class runs on the
+// main stream (side = -1) in the shared region.
+struct ClassSlot {
+  int64_t yoff = 0, ystride = 0;
+  int grid = 1, side = -1;
+};
+inline int64_t plan_slots(std::vector<ClassSlot>& slots, const std::vector<int64_t>& lanes,
+                          const std::vector<int64_t>& ydoubles, int ncu, int block, int nside, int64_t cap_doubles) {
+  const size_t n = lanes.size();
+  slots.assign(n, ClassSlot());
+  const int64_t maxblocks = (int64_t)ncu * 8, layer = (int64_t)ncu * 256;
+  int64_t shared = 0, sides = 0;
+  int nsmall = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (lanes[i] <= 0) continue;
+    ClassSlot& s = slots[i];
+    s.grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxblocks, (lanes[i] + block - 1) / block));
+    s.ystride = (int64_t)s.grid * block;
+    if (lanes[i] <= layer && nside > 0) {
+      s.side = nsmall++ % nside;
+      s.yoff = -1;  // placed below
+      sides += ydoubles[i] * s.ystride;
+    } else {
+      shared = std::max(shared, ydoubles[i] * s.ystride);
+    }
+  }
+  if (nsmall && shared + sides > cap_doubles) {  // sequential: one shared region
+    for (size_t i = 0; i < n; ++i) {
+      if (lanes[i] <= 0) continue;
+      slots[i].side = -1, slots[i].yoff = 0;
+      shared = std::max(shared, ydoubles[i] * slots[i].ystride);
+    }
+    return shared;
+  }
+  int64_t off = shared;
+  for (size_t i = 0; i < n; ++i)
+    if (lanes[i] > 0 && slots[i].side >= 0) {
+      slots[i].yoff = off;
+      off += ydoubles[i] * slots[i].ystride;
+    }
+  return off;
+}
+
+// Forks the side streams from the main stream (they start after the work
+// queued on it so far) and, at the end, joins them back.
+inline void fork_sides(const l2g_context_s& ctx);
+inline void join_sides(const l2g_context_s& ctx);
+
 }  // namespace l2g_impl
 
 struct l2g_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // side streams for classes that run beside each other (l2g_impl::plan_slots),
+  // forked from and joined back into `stream` with events
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr, join[kSide] = {};
   l2g_impl::DevBuf boys;
   int shg_gaussian = 0;
   int ncu = 256;
@@ -185,6 +245,19 @@ struct l2g_context_s {
   // destroyed; l2g_finalize refuses while any is alive
   std::atomic<int> live{0};
 };
+namespace l2g_impl {
+inline void fork_sides(const l2g_context_s& ctx) {
+  HIPCHK(hipEventRecord(ctx.fork, ctx.stream));
+  for (int k = 0; k < l2g_context_s::kSide; ++k) HIPCHK(hipStreamWaitEvent(ctx.side[k], ctx.fork, 0));
+}
+inline void join_sides(const l2g_context_s& ctx) {
+  for (int k = 0; k < l2g_context_s::kSide; ++k) {
+    HIPCHK(hipEventRecord(ctx.join[k], ctx.side[k]));
+    HIPCHK(hipStreamWaitEvent(ctx.stream, ctx.join[k], 0));
+  }
+}
+}  // namespace l2g_impl
+
 
 struct l2g_basis_s {
   l2g_context_s* ctx;
