@@ -6,7 +6,9 @@
 //
 // Kept from the reference, with the same meaning:
 //   * Shell with renormalised coefficients (shell.h:720-1012); Shell::unit()
-//   * Operator::coulomb, BraKet::xx_xx, ScreeningMethod (shell.h:1041-1059),
+//   * Operator::coulomb, BraKet::xx_xx and the 3-/2-centre xs_xx, xx_xs, xs_xs
+//     (compute with 3 or 2 shells, the missing centre Shell::unit();
+//     engine.impl.h:140-178), ScreeningMethod (shell.h:1041-1059),
 //     CartesianShellNormalization, default_screening_method()
 //   * initialize() / finalize() / initialized() (initialize.h:76-113),
 //     set_solid_harmonics_ordering (:117-119)
@@ -189,7 +191,7 @@ class Engine {
     (void)params;
     assert(max_nprim > 0);
     if (oper != Operator::coulomb) throw std::invalid_argument("libint2_amd: only Operator::coulomb is provided");
-    if (braket_ != BraKet::xx_xx) throw std::invalid_argument("libint2_amd: only BraKet::xx_xx is provided");
+    if (braket_ == BraKet::x_x) throw std::invalid_argument("libint2_amd: 1-body brakets are not provided");
     if (deriv_order != 0) throw std::invalid_argument("libint2_amd: only deriv_order 0 is provided");
     assert(initialized() && "libint2::initialize() must be called before constructing an Engine");
     if (max_l > l2g_max_l()) throw lmax_exceeded("coulomb", l2g_max_l(), max_l);
@@ -220,9 +222,11 @@ class Engine {
   }
   Engine& set(BraKet b) {
     if (default_) throw using_default_initialized();
-    if (b != BraKet::xx_xx) throw std::invalid_argument("libint2_amd: only BraKet::xx_xx is provided");
+    if (b == BraKet::x_x || b == BraKet::invalid) throw std::invalid_argument("libint2_amd: 1-body brakets are not provided");
+    braket_ = b;
     return *this;
   }
+  BraKet braket() const { return braket_; }
   Engine& prescale_by(scalar_type s) {
     scale_ = s;
     return *this;
@@ -247,6 +251,27 @@ class Engine {
   /// one shell quartet, caller order (s1 s2|s3 s4), libint's layout
   const target_ptr_vec& compute(const Shell& s1, const Shell& s2, const Shell& s3, const Shell& s4) {
     if (default_) throw using_default_initialized();
+    assert(braket_ == BraKet::xx_xx && "# of arguments to compute() does not match the braket type");
+    return compute_quartet(s1, s2, s3, s4);
+  }
+  /// 3-centre: (a|cd) for BraKet::xs_xx, (ab|c) for BraKet::xx_xs; the
+  /// missing centre is Shell::unit(), as Engine::compute (engine.impl.h:140-178)
+  const target_ptr_vec& compute(const Shell& s1, const Shell& s2, const Shell& s3) {
+    if (default_) throw using_default_initialized();
+    assert((braket_ == BraKet::xs_xx || braket_ == BraKet::xx_xs) &&
+           "# of arguments to compute() does not match the braket type");
+    return braket_ == BraKet::xs_xx ? compute_quartet(s1, Shell::unit(), s2, s3)
+                                    : compute_quartet(s1, s2, s3, Shell::unit());
+  }
+  /// 2-centre: (a|c) for BraKet::xs_xs
+  const target_ptr_vec& compute(const Shell& s1, const Shell& s2) {
+    if (default_) throw using_default_initialized();
+    assert(braket_ == BraKet::xs_xs && "# of arguments to compute() does not match the braket type");
+    return compute_quartet(s1, Shell::unit(), s2, Shell::unit());
+  }
+
+ private:
+  const target_ptr_vec& compute_quartet(const Shell& s1, const Shell& s2, const Shell& s3, const Shell& s4) {
     const Shell* sh[4] = {&s1, &s2, &s3, &s4};
     for (auto* s : sh) {
       assert(s->ncontr() == 1 && "generally-contracted shells are not yet supported");
@@ -281,13 +306,15 @@ class Engine {
     return targets_;
   }
 
+
+ public:
   template <Operator op, BraKet bk, size_t der>
   const target_ptr_vec& compute2(const Shell& b1, const Shell& b2, const Shell& k1, const Shell& k2,
                                  const void* spbra = nullptr, const void* spket = nullptr) {
     static_assert(op == Operator::coulomb && bk == BraKet::xx_xx && der == 0,
                   "libint2_amd provides compute2<coulomb, xx_xx, 0> only");
     assert(((spbra == nullptr) == (spket == nullptr)) && "Engine::compute2 -- expects zero or two ShellPair objects");
-    return compute(b1, b2, k1, k2);
+    return compute_quartet(b1, b2, k1, k2);
   }
 
   /// Batched extension: every quartet (indices into `basis`, caller order)

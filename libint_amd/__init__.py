@@ -199,6 +199,13 @@ class Shell:
         self.coeff = c
         self.O = np.array(origin, np.float64).reshape(3)
 
+    @staticmethod
+    def unit() -> "Shell":
+        """libint2::Shell::unit() (shell.h:904-909): l = 0, exponent 0,
+        coefficient 1 (not renormalised), at the origin; stands for the
+        missing centre of 3- and 2-centre integrals."""
+        return Shell(0, [(0.0, 1.0)], pure=False, renorm=False)
+
     def size(self) -> int:
         return 2 * self.l + 1 if self.pure else (self.l + 1) * (self.l + 2) // 2
 
@@ -508,8 +515,16 @@ class Engine:
     a flat array in libint's layout, or None where libint returns nullptr.
     One Engine per thread, as libint requires (engine.h:560-579)."""
 
+    # BraKet of the reference's python binding (python/src/libint2/libint2.cc:212-217):
+    # where compute()'s shells go in the 4-centre quartet; None = Shell.unit()
+    BRAKETS = {"XXXX": (0, 1, 2, 3), "XSXX": (0, None, 1, 2), "XXXS": (0, 1, 2, None),
+               "XSXS": (0, None, 1, None)}
+
     def __init__(self, max_nprim: int, max_l: int, precision: float = np.finfo(float).eps,
-                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None):
+                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None, braket: str = "XXXX"):
+        if braket not in self.BRAKETS:
+            raise ValueError(f"braket must be one of {sorted(self.BRAKETS)}")
+        self.braket = braket
         if max_nprim <= 0:
             raise ValueError("max_nprim must be positive")
         if max_l > globals()["max_l"]():
@@ -530,13 +545,21 @@ class Engine:
         self.scale = float(s)
         return self
 
-    def compute(self, s1: Shell, s2: Shell, s3: Shell, s4: Shell):
-        for s in (s1, s2, s3, s4):
+    def compute(self, *shells: Shell):
+        """compute(s1, s2, s3, s4) for XXXX; compute(a, c, d) = (a|cd) for
+        XSXX and compute(a, b, c) = (ab|c) for XXXS (3-centre); compute(a, c)
+        = (a|c) for XSXS (2-centre), the missing centres being Shell.unit()
+        as Engine::compute does (engine.impl.h:140-178)."""
+        slots = self.BRAKETS[self.braket]
+        if len(shells) != sum(k is not None for k in slots):
+            raise ValueError(f"{self.braket} takes {sum(k is not None for k in slots)} shells")
+        for s in shells:
             if s.l > self.max_l_:
                 raise LmaxExceeded(-2, "the angular momentum limit is exceeded")
             if s.nprim() > self.max_nprim:
                 raise ValueError("shell has more primitives than max_nprim")
-        b = Basis(self.ctx, BasisArrays.from_shells([s1, s2, s3, s4]))
+        quad = [shells[k] if k is not None else Shell.unit() for k in slots]
+        b = Basis(self.ctx, BasisArrays.from_shells(quad))
         vals, _, nz = compute_quartets(b, [[0, 1, 2, 3]], self.precision, self.screening, self.scale,
                                        self.uniform_cart)
         return vals if nz[0] else None

@@ -6,6 +6,7 @@
 //   facade_test       -> everything (needs a GPU)
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <libint2_amd/engine.h>
 
 using libint2::BraKet;
@@ -72,6 +73,31 @@ int main(int argc, char** argv) {
   auto all = e.compute_batch(basis, qs, &off);
   e.compute(basis[2], basis[3], basis[2], basis[3]);
   for (size_t i = 0; i < 5 * 6 * 5 * 6; ++i) CHECK(all[off[4] + i] == buf[0][i]);
+  // 3- and 2-centre brakets: the missing centre is Shell::unit() (printed
+  // as shell 4 for the oracle)
+  const double eps = std::numeric_limits<double>::epsilon();
+  Engine e3(Operator::coulomb, 2, 2, 0, eps, 0, BraKet::xs_xx);
+  e3.compute(basis[2], basis[0], basis[1]);
+  CHECK(e3.results()[0] != nullptr);
+  {
+    const size_t n = basis[2].size() * basis[0].size() * basis[1].size();
+    std::printf("Q 2 4 0 1 %zu\n", n);
+    for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", e3.results()[0][i]);
+  }
+  Engine e2(Operator::coulomb, 2, 2, 0, eps, 0, BraKet::xs_xs);
+  e2.compute(basis[1], basis[2]);
+  CHECK(e2.results()[0] != nullptr);
+  {
+    const size_t n = basis[1].size() * basis[2].size();
+    std::printf("Q 1 4 2 4 %zu\n", n);
+    for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", e2.results()[0][i]);
+  }
+  // python/tests/test_libint2.py:39-42: ||(s s|s)|| = 3.6563211198 for
+  // s = Shell(0, [(1, 10)])
+  Shell su{{1.0}, {{0, false, {10.0}}}, {{0, 0, 0}}};
+  Engine ex(Operator::coulomb, 1, 0, 0, 1e-15, 0, BraKet::xs_xx);
+  ex.compute(su, su, su);
+  CHECK(ex.results()[0] != nullptr && std::abs(ex.results()[0][0] - 3.6563211198) < 1e-9);
   // fully screened quartet -> nullptr (engine.impl.h:1781-1784)
   Shell t1{{80.0}, {{0, false, {1.0}}}, {{0, 0, 0}}};
   Shell t2{{80.0}, {{0, false, {1.0}}}, {{6.0, 0, 0}}};

@@ -40,7 +40,7 @@ def test_facade_results_match_oracle():
         (2, [2.2], [1.0], (-0.3, 0.8, 0.1), True),
         (2, [0.7, 1.9], [0.3, 0.9], (0.4, -0.5, 0.6), False),
     ]
-    sh = [la.Shell(l, list(zip(a, c)), o, pure=p) for (l, a, c, o, p) in shells]
+    sh = [la.Shell(l, list(zip(a, c)), o, pure=p) for (l, a, c, o, p) in shells] + [la.Shell.unit()]
     arrays = la.BasisArrays.from_shells(sh)
     lines = r.stdout.split("\n")
     i = 0
@@ -56,4 +56,4 @@ def test_facade_results_match_oracle():
             i += n + 1
         else:
             i += 1
-    assert checked == 5
+    assert checked == 7  # 5 quartets, one 3-centre and one 2-centre set

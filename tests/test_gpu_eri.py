@@ -290,3 +290,34 @@ def test_wave_split(ctx):
     nfail, mabs, mrel = compare(g[iw], o[iw], 1e-10, 1e-8)
     say(f"  wave part ({int(inw.sum())} quartets): max abs {mabs:.2e}, fails {nfail}")
     assert nfail == 0
+
+
+def test_three_and_two_centre(ctx):
+    """3-centre (a|cd) and 2-centre (a|c) Coulomb integrals: the 4-centre
+    kernels with Shell.unit() (exponent 0, coefficient 1) as the missing
+    centre, as Engine::compute does for BraKet xs_xx / xs_xs
+    (engine.impl.h:140-178); every class to l = 3 against the oracle, and
+    the reference's python golden ||(s s|s)|| = 3.6563211198
+    (python/tests/test_libint2.py:39-42)."""
+    rng = np.random.default_rng(41)
+    lmax = la.max_l()
+    shells = random_shells(rng, [l for l in range(lmax + 1) for _ in range(2)], kmax=3)
+    n = len(shells)
+    u = n
+    arrays = la.BasisArrays.from_shells(shells + [la.Shell.unit()])
+    q3 = [(a, u, c, d) for a in range(n) for c in range(n) for d in range(n)]
+    q2 = [(a, u, c, u) for a in range(n) for c in range(n)]
+    q = np.array(q3 + q2, np.int32)
+    say(f"case: 3-centre {len(q3)} + 2-centre {len(q2)} sets")
+    g, o, gnz, onz = gpu_vs_oracle(ctx, arrays, q)
+    nfail, mabs, mrel = compare(g, o, ABS, REL)
+    say(f"  max abs {mabs:.3e} max rel {mrel:.3e} fails {nfail}")
+    assert nfail == 0
+    assert np.array_equal(gnz, onz)
+    s = la.Shell(0, [(1.0, 10.0)])
+    for bk in ("XSXX", "XXXS"):
+        v = la.Engine(1, 0, ctx=ctx, braket=bk).compute(s, s, s)
+        assert abs(np.linalg.norm(v) - 3.6563211198) < 1e-9
+    v = la.Engine(1, 0, ctx=ctx, braket="XSXS").compute(s, s)
+    o2, _, _ = oracle.eri_quartets(la.BasisArrays.from_shells([s, la.Shell.unit()]), [[0, 1, 0, 1]])
+    assert compare(v, o2, ABS, REL)[0] == 0

@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 
+import libint_amd as la
 import oracle
 from tests.helpers import compare
 
@@ -258,3 +259,20 @@ def test_screening_null_result():
     v, _, nz = oracle.eri_quartets(b, [[0, 1, 0, 1], [0, 0, 0, 0]], precision=1e-10)
     assert nz[0] == 0 and v[0] == 0.0
     assert nz[1] == 1 and v[1] > 0
+
+
+def test_oracle_three_centre_golden():
+    """3-centre (s s|s) through Shell.unit() (exponent 0, coefficient 1) as
+    the missing centre: the reference's python golden
+    ||(s s|s)|| = 3.6563211198 for s = Shell(0, [(1, 10)])
+    (python/tests/test_libint2.py:39-42)."""
+    s = la.Shell(0, [(1.0, 10.0)])
+    arrays = la.BasisArrays.from_shells([s, la.Shell.unit()])
+    v, _, _ = oracle.eri_quartets(arrays, [[0, 1, 0, 0]], precision=0.0)
+    assert abs(np.linalg.norm(v) - 3.6563211198) < 1e-9
+    # and the 2-centre (p|p) equals the 4-centre integral with unit partners
+    p = la.Shell(1, [(1.3, 0.5), (0.4, 0.6)], (0.2, 0.0, 0.1))
+    arrays = la.BasisArrays.from_shells([p, la.Shell.unit(), p])
+    v2, _, _ = oracle.eri_quartets(arrays, [[0, 1, 2, 1]], precision=0.0)
+    v3, _, _ = oracle.eri_quartets(arrays, [[1, 0, 1, 2]], precision=0.0)
+    assert np.allclose(v2, v3.reshape(3, 3).T.reshape(-1), rtol=1e-13, atol=1e-15)
