@@ -183,6 +183,18 @@ int l2g_eri_plan_stats(l2g_eri_plan plan, int cap, int* nclasses, int* class_cod
  * per-class launch overhead of repeated executions of one plan. */
 int l2g_eri_plan_set_graph(l2g_eri_plan plan, int on);
 
+/* Operator of the plan: L2G_OPER_COULOMB (default), L2G_OPER_ERF_COULOMB
+ * (erf(omega r)/r) or L2G_OPER_ERFC_COULOMB (erfc(omega r)/r), libint's
+ * Operator::coulomb / erf_coulomb / erfc_coulomb (engine.h:158-167) with
+ * omega its parameter.  The core integrals follow erf_coulomb_gm_eval and
+ * erfc_coulomb_gm_eval (boys.h:1913-1985); the primitive screen stays on the
+ * Coulomb prefactor, as in engine.impl.h:1362-1386.  erfc is evaluated as
+ * the Coulomb integrals minus the erf ones (two passes per class). */
+#define L2G_OPER_COULOMB 0
+#define L2G_OPER_ERF_COULOMB 1
+#define L2G_OPER_ERFC_COULOMB 2
+int l2g_eri_plan_set_operator(l2g_eri_plan plan, int oper, double omega);
+
 /* Wave split (on != 0; off by default): classes of few, deeply contracted
  * quartets run a wave per quartet whose 64 lanes split the primitive loop,
  * instead of one lane per quartet, so that a handful of long contractions no

@@ -66,6 +66,7 @@ ABI = {
     "l2g_eri_plan_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp, _vp]),
     "l2g_eri_plan_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "l2g_eri_plan_set_wave_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_eri_plan_set_operator": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "l2g_eri_plan_elapsed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
     "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
@@ -381,6 +382,11 @@ class EriPlan:
         and replay it on later executions into the same output buffer."""
         _check(lib().l2g_eri_plan_set_graph(self._h, int(bool(on))))
 
+    def set_operator(self, oper: int, omega: float = 0.0):
+        """COULOMB, ERF_COULOMB or ERFC_COULOMB with attenuation omega
+        (libint's Operator::coulomb / erf_coulomb / erfc_coulomb)."""
+        _check(lib().l2g_eri_plan_set_operator(self._h, int(oper), float(omega)))
+
     def set_wave_split(self, on=True):
         """Classes of few, deeply contracted quartets run a wave per quartet
         (lanes split the primitive loop); sums in another order than libint,
@@ -427,6 +433,10 @@ def precision_bounds(basis: Basis, pairs):
 
 
 FP32_HRR_FACTOR = 16.0
+
+# Operator of an ERI plan (l2g.h L2G_OPER_*; libint's Operator::coulomb,
+# erf_coulomb, erfc_coulomb, engine.h:158-167)
+COULOMB, ERF_COULOMB, ERFC_COULOMB = 0, 1, 2
 
 
 class FockBuilder:

@@ -55,6 +55,8 @@ struct L2gEriArgs {
   int uniform_cart;
   int schwarz_norm;  // Schwarz kernels: 0 = max-abs norm, 1 = Frobenius norm
   int coop;          // Schwarz kernels: one wave per pair, its lanes splitting the primitive loop
+  double omega;      // > 0: erf-attenuated Coulomb erf(omega r)/r (0: Coulomb)
+  int store_sub;     // ERI kernels: subtract from the output instead of storing
   double* __restrict__ out;
   double* __restrict__ ybuf;
   long long ystride;
@@ -231,6 +233,7 @@ struct L2gStore {
   int l1, l2, l3, l4;
   int p1, p2, p3, p4;
   int gauss, uniform;
+  int sub;  // subtract from the output (erfc = Coulomb - erf, second pass)
 
   __device__ __forceinline__ int pos(int l, int pure, int s) const {
     if (!pure || !gauss) return s;
@@ -247,7 +250,8 @@ struct L2gStore {
     if (uniform) {
       val *= colscale * (p1 ? 1.0 : l2g_uniform_cart_scale(l1, i1)) * (p2 ? 1.0 : l2g_uniform_cart_scale(l2, i2));
     }
-    out[colofs + (size_t)pos(l1, p1, i1) * s1 + (size_t)pos(l2, p2, i2) * s2] = val;
+    double* o = out + colofs + (size_t)pos(l1, p1, i1) * s1 + (size_t)pos(l2, p2, i2) * s2;
+    *o = sub ? *o - val : val;
   }
   // Staged store: the value goes to its final position within the quartet's
   // block, but in the lane's scratch column (yb[k * ystride], k counted from
@@ -268,7 +272,10 @@ struct L2gStore {
     const int n = n1 * n2 * nk1 * nk2;
     const double* src = yb + (size_t)stage * ystride;
     double* dst = out + base;
-    for (int k = 0; k < n; ++k) dst[k] = src[(size_t)k * ystride];
+    if (sub)
+      for (int k = 0; k < n; ++k) dst[k] -= src[(size_t)k * ystride];
+    else
+      for (int k = 0; k < n; ++k) dst[k] = src[(size_t)k * ystride];
   }
 };
 
@@ -281,6 +288,7 @@ __device__ __forceinline__ L2gStore make_store(int meta, const L2gEriArgs& A, in
   st.p2 = (meta >> 4) & 1;
   st.p3 = (meta >> 5) & 1;
   st.p4 = (meta >> 6) & 1;
+  st.sub = A.store_sub;
   const int n1 = st.p1 ? 2 * l1 + 1 : l2g_ncart(l1);
   const int n2 = st.p2 ? 2 * l2 + 1 : l2g_ncart(l2);
   const int n3 = st.p3 ? 2 * l3 + 1 : l2g_ncart(l3);

@@ -164,7 +164,15 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
         }
         lds[C.coef + t] = v;
       }
-      if (t < C.m0len) lds[C.base + t] = pf * boys_one(T, t, A.boys);
+      if (t < C.m0len) {
+        // erf_coulomb (boys.h:1913-1944): G_m = s^(m+1/2) F_m(s T), s = w^2/(w^2 + rho)
+        if (A.omega > 0) {
+          const double w2 = A.omega * A.omega, s = w2 / (w2 + gp * gq * oos);
+          lds[C.base + t] = pf * pow(s, t + 0.5) * boys_one(T * s, t, A.boys);
+        } else {
+          lds[C.base + t] = pf * boys_one(T, t, A.boys);
+        }
+      }
       __syncthreads();
       // levels; a thread's next record is loaded (at a clamped, always valid
       // index) while it computes the current one, and its first record of

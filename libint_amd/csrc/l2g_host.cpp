@@ -85,6 +85,7 @@ struct ClassWork {
   bool fp32 = false;
   bool generic = false;
   bool coop = false;  // a wave per quartet (few, deeply contracted quartets)
+  int gidx = -1;      // the class's program in the generic kernel's tables
   int64_t first, count;
   ClassSlot slot;     // grid, scratch region and stream of the launch
   int64_t nprimq;
@@ -108,6 +109,8 @@ struct l2g_eri_plan_s {
   int64_t ymax = 0;  // largest scratch doubles per lane over the classes
   int block = 128;
   bool wave_split = false;  // classes of few deep quartets run a wave per quartet
+  int oper = 0;             // 0 Coulomb, 1 erf-attenuated, 2 erfc-attenuated
+  double omega = 0;
   // hipGraph mode: the class launches are captured once per output buffer
   // and replayed; per-class timings are then not recorded
   bool use_graph = false;
@@ -491,6 +494,10 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       cw.cls = c;
       cw.fp32 = is32;
       cw.generic = isgen;
+      {
+        const L2gClassInfo* ci = isgen ? nullptr : &(is32 ? l2g_gen_classes32 : l2g_gen_classes)[c];
+        cw.gidx = isgen ? c : ctx->generic_of[ci->la][ci->lb][ci->lc][ci->ld];
+      }
       cw.first = (int64_t)bra.size();
       cw.count = (int64_t)v.size();
       cw.nprimq = 0;
@@ -639,9 +646,24 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     bool sides = false;
     for (auto& c : p->classes) sides |= c.slot.side >= 0;
     sides &= !capture;
+    // attenuated operators run every class on the generic kernel (whose
+    // core-integral step takes omega); the register-resident kernels stay
+    // Coulomb-only, so their primitive loop carries no operator branch.
+    // Passes (attenuation, subtract): Coulomb; erf = one attenuated pass
+    // (omega = 0 gives exact zeros as Coulomb minus Coulomb); erfc =
+    // Coulomb minus the erf pass, both per class on the class's stream
+    std::vector<std::pair<double, int>> passes;
+    if (p->oper == 0) passes = {{0.0, 0}};
+    else if (p->oper == 1) passes = p->omega > 0 ? std::vector<std::pair<double, int>>{{p->omega, 0}}
+                                                 : std::vector<std::pair<double, int>>{{0.0, 0}, {0.0, 1}};
+    else passes = p->omega > 0 ? std::vector<std::pair<double, int>>{{0.0, 0}, {p->omega, 1}}
+                               : std::vector<std::pair<double, int>>{{0.0, 0}};
     auto launch_all = [&] {
     if (sides) fork_sides(*ctx);
     for (auto& c : p->classes) {
+    for (size_t ps = 0; ps < passes.size(); ++ps) {
+      A.omega = passes[ps].first;
+      A.store_sub = passes[ps].second;
       hipStream_t cs = sides && c.slot.side >= 0 ? ctx->side[c.slot.side] : ctx->stream;
       A.ybuf = p->ybuf.as<double>() + c.slot.yoff;
       A.ystride = c.slot.ystride;
@@ -651,11 +673,12 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       A.out_off = p->off.as<long long>() + c.first;
       A.qidx = p->qidx.as<long long>() + c.first;
       A.n = (int)c.count;
-      if (!capture) HIPCHK(hipEventRecord(c.ev0, cs));
-      if (c.generic) {
+      if (!capture && ps == 0) HIPCHK(hipEventRecord(c.ev0, cs));
+      if (c.generic || (p->oper != 0 && p->omega > 0)) {
         // one block per quartet, persistent over the class
+        if (c.gidx < 0) throw L2gError(L2G_E_LMAX, "no generic-kernel program for this class");
         const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
-        launch_generic(kGenericEri, A, l2g_gen_generic[c.cls], ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid, cs);
+        launch_generic(kGenericEri, A, l2g_gen_generic[c.gidx], ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid, cs);
       } else {
         const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
         A.coop = c.coop ? 1 : 0;
@@ -663,7 +686,8 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
         ci.launch(A, c.slot.grid, p->block, cs);
         LAUNCHCHK(std::string("ERI kernel of class ") + std::to_string(c.cls));
       }
-      if (!capture) HIPCHK(hipEventRecord(c.ev1, cs));
+      if (!capture && ps + 1 == passes.size()) HIPCHK(hipEventRecord(c.ev1, cs));
+    }
     }
     if (sides) join_sides(*ctx);
     };
@@ -696,6 +720,19 @@ int l2g_eri_plan_set_graph(l2g_eri_plan p, int on) {
     HIPCHK(hipStreamSynchronize(p->basis->ctx->stream));
     p->use_graph = on != 0;
     if (!p->use_graph) p->drop_graph();
+  });
+}
+
+int l2g_eri_plan_set_operator(l2g_eri_plan p, int oper, double omega) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    if (oper < 0 || oper > 2 || !(omega >= 0)) throw std::invalid_argument("l2g_eri_plan_set_operator: bad operator or omega");
+    l2g_context_s* ctx = p->basis->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    p->oper = oper;
+    p->omega = omega;
+    p->drop_graph();
   });
 }
 

@@ -54,6 +54,7 @@ def lib():
             build()
         L = ctypes.CDLL(_LIBPATH)
         L.oracle_set_shg_ordering.argtypes = [ctypes.c_int]
+        L.oracle_set_operator.argtypes = [ctypes.c_int, ctypes.c_double]
         L.oracle_boys_cheb7.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p]
         L.oracle_boys_reference.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p]
         L.oracle_boys_table.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -135,9 +136,13 @@ def solidharm_coeff(l, m, lx, ly, lz) -> float:
     return lib().oracle_solidharm_coeff(l, m, lx, ly, lz)
 
 
+COULOMB, ERF_COULOMB, ERFC_COULOMB = 0, 1, 2
+
+
 def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGINAL,
-                 uniform_cart=False, nthreads=0):
-    """Restated Engine::compute2 over quartets (caller order).
+                 uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0):
+    """Restated Engine::compute2 over quartets (caller order), for
+    Operator::coulomb, erf_coulomb or erfc_coulomb (omega: the attenuation).
 
     Returns (flat values, offsets[nq], nonzero[nq])."""
     ob = OBasis.of(basis)
@@ -145,22 +150,34 @@ def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGI
     nq = q.shape[0]
     offs = np.zeros(nq, np.int64)
     L = lib()
-    total = L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening,
-                                  int(uniform_cart), None, offs, None, nthreads)
-    out = np.zeros(max(total, 1))
-    nz = np.zeros(nq, np.int8)
-    L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart),
-                          out.ctypes.data, offs, nz.ctypes.data, nthreads)
+    if L.oracle_set_operator(int(oper), float(omega)) != 0:
+        raise ValueError("bad operator or omega")
+    try:
+        total = L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening,
+                                      int(uniform_cart), None, offs, None, nthreads)
+        out = np.zeros(max(total, 1))
+        nz = np.zeros(nq, np.int8)
+        L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart),
+                              out.ctypes.data, offs, nz.ctypes.data, nthreads)
+    finally:
+        L.oracle_set_operator(COULOMB, 0.0)
     return out[:total], offs, nz
 
 
-def eri_brute(basis, quartets, nthreads=0):
+def eri_brute(basis, quartets, nthreads=0, oper=COULOMB, omega=0.0):
+    """McMurchie-Davidson in long double (an independent derivation); for
+    erf_coulomb the Hermite integrals take the attenuated exponent."""
     ob = OBasis.of(basis)
     q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
     L = lib()
-    total = L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), None, nthreads)
-    out = np.zeros(max(total, 1))
-    L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), out.ctypes.data, nthreads)
+    if L.oracle_set_operator(int(oper), float(omega)) != 0:
+        raise ValueError("bad operator or omega")
+    try:
+        total = L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), None, nthreads)
+        out = np.zeros(max(total, 1))
+        L.oracle_eri_brute(*ob.args(), q.shape[0], q.reshape(-1), out.ctypes.data, nthreads)
+    finally:
+        L.oracle_set_operator(COULOMB, 0.0)
     return out[:total]
 
 

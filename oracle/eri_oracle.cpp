@@ -584,6 +584,32 @@ static void hrr_transfer(int l1, int l2, const double AB[3], const double* X, in
   std::memcpy(Y, cur.data(), sizeof(double) * ncart(l1) * ncart(l2) * rest);
 }
 
+// Core integrals of the operator (Engine's core_eval_pack, engine.impl.h:
+// 1437-1479): Coulomb F_m(T); erf_coulomb and erfc_coulomb as
+// erf_coulomb_gm_eval / erfc_coulomb_gm_eval (boys.h:1913-1985):
+//   erf:  G_m = s^(m+1/2) F_m(s T),  s = omega^2 / (omega^2 + rho)  (0 if omega = 0)
+//   erfc: G_m = F_m(T) - s^(m+1/2) F_m(s T)
+static int g_oper = 0;  // 0 coulomb, 1 erf_coulomb, 2 erfc_coulomb
+static double g_omega = 0;
+template <class Boys>
+static void core_ints(const Boys& boys, double* Gm, double rho, double T, int mmax) {
+  if (g_oper == 0) {
+    boys.eval(Gm, T, mmax);
+    return;
+  }
+  if (g_oper == 2)
+    boys.eval(Gm, T, mmax);
+  else
+    for (int m = 0; m <= mmax; ++m) Gm[m] = 0.0;
+  if (g_omega > 0) {
+    const double o2 = g_omega * g_omega, s = o2 / (o2 + rho);
+    double Fs[64];
+    boys.eval(Fs, T * s, mmax);
+    double f = std::sqrt(s);
+    for (int m = 0; m <= mmax; ++m, f *= s) Gm[m] += (g_oper == 2 ? -f : f) * Fs[m];
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Engine restatement
 // ----------------------------------------------------------------------------
@@ -693,7 +719,7 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
       }
       const double rho = gp * gq * oogpq;
       const double T = PQ2 * rho;
-      boys.eval(Fm, T, L);
+      core_ints(boys, Fm, rho, T, L);
       for (int m = 0; m <= L; ++m) gm[(size_t)m * maxp + p] = Fm[m] * pfac;
       const double gpo = oogpq * gp, gqo = oogpq * gq;
       double W[3];
@@ -914,6 +940,8 @@ static long double md_primitive(const int* a, const int* b, const int* c, const 
     const int T = imax + jmax + 1;
     return E[((size_t)i * (jmax + 1) + j) * (T + 1) + t];
   };
+  // Hermite sum with the Coulomb exponent al (or the attenuated one)
+  auto hsum = [&](long double al) -> long double {
   // R^n_{tuv}
   std::vector<long double> Fb(Lt + 1);
   boys_reference(al * (XPQ[0] * XPQ[0] + XPQ[1] * XPQ[1] + XPQ[2] * XPQ[2]), Lt, Fb.data());
@@ -957,6 +985,18 @@ static long double md_primitive(const int* a, const int* b, const int* c, const 
               sum += eb * ek * sgn * R[idx(0, t + tau, u + nu, v + phi)];
             }
       }
+  return sum;
+  };
+  // erf(omega r)/r: the Hermite integrals with al_w = al omega^2/(al + omega^2)
+  // in place of al, times sqrt(al_w/al) (McMurchie-Davidson with the
+  // attenuated kernel); erfc = Coulomb - erf
+  long double sum = 0;
+  if (g_oper != 1) sum = hsum(al);
+  if (g_oper != 0 && g_omega > 0) {
+    const long double w2 = (long double)g_omega * g_omega, alw = al * w2 / (al + w2);
+    const long double e = sqrtl(alw / al) * hsum(alw);
+    sum += g_oper == 1 ? e : -e;
+  }
   const long double PI = 3.141592653589793238462643383279502884L;
   return 2 * powl(PI, 2.5L) / (p * q * sqrtl(p + q)) * sum;
 }
@@ -1210,6 +1250,13 @@ Basis make_basis(int nshell, const int* l, const int* pure, const int* nprim, co
 }  // namespace
 
 extern "C" {
+
+int oracle_set_operator(int oper, double omega) {
+  if (oper < 0 || oper > 2 || omega < 0) return -1;
+  g_oper = oper;
+  g_omega = omega;
+  return 0;
+}
 
 int oracle_set_shg_ordering(int o) {
   g_shg_ordering = o;

@@ -321,3 +321,40 @@ def test_three_and_two_centre(ctx):
     v = la.Engine(1, 0, ctx=ctx, braket="XSXS").compute(s, s)
     o2, _, _ = oracle.eri_quartets(la.BasisArrays.from_shells([s, la.Shell.unit()]), [[0, 1, 0, 1]])
     assert compare(v, o2, ABS, REL)[0] == 0
+
+
+@pytest.mark.parametrize("oper", [la.ERF_COULOMB, la.ERFC_COULOMB])
+def test_range_separated(ctx, oper):
+    """erf_coulomb and erfc_coulomb (engine.h:158-167) at two attenuations,
+    every class to l = 3, against the oracle at the per-integral bar (the
+    oracle's core integrals are pinned by quadrature and by the
+    McMurchie-Davidson brute force); omega = 0 gives zeros for erf and the
+    Coulomb integrals for erfc."""
+    rng = np.random.default_rng(61)
+    lmax = la.max_l()
+    shells = random_shells(rng, [l for l in range(lmax + 1) for _ in range(2)], kmax=2)
+    n = len(shells)
+    allq = np.array(list(itertools.product(range(n), repeat=4)), np.int32)
+    q = allq[rng.choice(len(allq), 1500, replace=False)]
+    arrays = la.BasisArrays.from_shells(shells)
+    b = la.Basis(ctx, arrays)
+    for w in (0.4, 2.5):
+        say(f"case: operator {oper} omega {w}, {len(q)} quartets")
+        plan = la.EriPlan(b, q)
+        plan.set_operator(oper, w)
+        plan.execute()
+        g, gnz = plan.download()
+        o, _, onz = oracle.eri_quartets(arrays, q, oper=oper, omega=w)
+        nfail, mabs, mrel = compare(g, o, ABS, REL)
+        say(f"  max abs {mabs:.3e} max rel {mrel:.3e} fails {nfail}")
+        assert nfail == 0
+        assert np.array_equal(gnz, onz)
+    plan = la.EriPlan(b, q)
+    plan.set_operator(oper, 0.0)
+    plan.execute()
+    g, _ = plan.download()
+    if oper == la.ERF_COULOMB:
+        assert not g.any()
+    else:
+        c, _, _ = la.compute_quartets(b, q)
+        assert np.array_equal(g, c)

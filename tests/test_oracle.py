@@ -14,7 +14,7 @@ import pytest
 
 import libint_amd as la
 import oracle
-from tests.helpers import compare
+from tests.helpers import compare, random_shells
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EPS = np.finfo(float).eps
@@ -276,3 +276,50 @@ def test_oracle_three_centre_golden():
     v2, _, _ = oracle.eri_quartets(arrays, [[0, 1, 2, 1]], precision=0.0)
     v3, _, _ = oracle.eri_quartets(arrays, [[1, 0, 1, 2]], precision=0.0)
     assert np.allclose(v2, v3.reshape(3, 3).T.reshape(-1), rtol=1e-13, atol=1e-15)
+
+
+def test_oracle_range_separated_ss_quadrature():
+    """erf_coulomb (ss|ss) against an independent quadrature of
+    erf(w r)/r = 2/sqrt(pi) int_0^w exp(-t^2 r^2) dt over closed-form
+    Gaussian-geminal integrals; erfc_coulomb as Coulomb minus it."""
+    from scipy.special import roots_legendre
+    rng = np.random.default_rng(51)
+    for _ in range(4):
+        ex = rng.uniform(0.2, 3.0, 4)
+        cen = rng.uniform(-1, 1, (4, 3))
+        sh = [la.Shell(0, [(x, 1.0)], tuple(o)) for x, o in zip(ex, cen)]
+        arr = la.BasisArrays.from_shells(sh)
+        a, b, c, d = ex
+        A, B, C, D = cen
+        p, q = a + b, c + d
+        P, Q = (a * A + b * B) / p, (c * C + d * D) / q
+        N = np.prod([(2 * x / np.pi) ** 0.75 for x in ex])
+        K = np.exp(-a * b / p * np.sum((A - B) ** 2) - c * d / q * np.sum((C - D) ** 2))
+        for w in (0.3, 1.1, 4.0):
+            x, wt = roots_legendre(300)
+            t, wt = 0.5 * w * (x + 1), 0.5 * w * wt
+            den = p * q + t * t * (p + q)
+            I = (np.pi ** 2 / (p * q)) ** 1.5 * (p * q / den) ** 1.5 * np.exp(-p * q * t * t / den * np.sum((P - Q) ** 2))
+            ref = N * K * 2 / np.sqrt(np.pi) * np.sum(wt * I)
+            v, _, _ = oracle.eri_quartets(arr, [[0, 1, 2, 3]], precision=0.0, oper=oracle.ERF_COULOMB, omega=w)
+            vc, _, _ = oracle.eri_quartets(arr, [[0, 1, 2, 3]], precision=0.0)
+            ve, _, _ = oracle.eri_quartets(arr, [[0, 1, 2, 3]], precision=0.0, oper=oracle.ERFC_COULOMB, omega=w)
+            assert abs(v[0] - ref) < 1e-14 + 1e-12 * abs(ref)
+            assert abs(ve[0] - (vc[0] - ref)) < 1e-14 + 1e-12 * abs(vc[0])
+
+
+def test_oracle_range_separated_vs_brute_force():
+    """erf_coulomb and erfc_coulomb from the restated OS path against the
+    McMurchie-Davidson brute force with the attenuated Hermite exponent, for
+    every class up to l = 2 plus f, at the brute-force bar."""
+    rng = np.random.default_rng(52)
+    shells = random_shells(rng, [0, 1, 2, 3, 1, 2], kmax=2)
+    arr = la.BasisArrays.from_shells(shells)
+    n = len(shells)
+    q = np.array([(i, j, k, l) for i in range(n) for j in range(n) for k in range(n) for l in range(n)
+                  if (i * 7 + j * 5 + k * 3 + l) % 11 == 0], np.int32)
+    for oper in (oracle.ERF_COULOMB, oracle.ERFC_COULOMB):
+        for w in (0.4, 2.5):
+            o, _, _ = oracle.eri_quartets(arr, q, precision=0.0, oper=oper, omega=w)
+            bf = oracle.eri_brute(arr, q, oper=oper, omega=w)
+            assert compare(o, bf, 5e-14, 1e-9)[0] == 0, (oper, w)
