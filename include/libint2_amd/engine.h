@@ -6,7 +6,8 @@
 //
 // Kept from the reference, with the same meaning:
 //   * Shell with renormalised coefficients (shell.h:720-1012); Shell::unit()
-//   * Operator::coulomb, BraKet::xx_xx and the 3-/2-centre xs_xx, xx_xs, xs_xs
+//   * Operator::coulomb, erf_coulomb and erfc_coulomb (params = omega),
+//     BraKet::xx_xx and the 3-/2-centre xs_xx, xx_xs, xs_xs
 //     (compute with 3 or 2 shells, the missing centre Shell::unit();
 //     engine.impl.h:140-178), ScreeningMethod (shell.h:1041-1059),
 //     CartesianShellNormalization, default_screening_method()
@@ -33,12 +34,14 @@
 #include <string>
 #include <vector>
 
+#include <type_traits>
+
 #include "../l2g.h"
 
 namespace libint2 {
 
 // ---------------------------------------------------------------- enums
-enum class Operator { overlap = 0, kinetic, nuclear, coulomb = 7, invalid = -1 };
+enum class Operator { overlap = 0, kinetic, nuclear, coulomb = 7, erf_coulomb, erfc_coulomb, invalid = -1 };
 enum class BraKet { x_x = 0, xx_xx = 1, xs_xx, xx_xs, xs_xs, invalid = -1 };
 enum class ScreeningMethod {
   Original = L2G_SCREEN_ORIGINAL,
@@ -184,13 +187,16 @@ class Engine {
   };
 
   Engine() = default;
+  /// params: omega (the attenuation) for erf_coulomb / erfc_coulomb, as
+  /// operator_traits<erf_coulomb>::oper_params_type; ignored for coulomb
+  template <typename Params = int>
   Engine(Operator oper, size_t max_nprim, int max_l, int deriv_order = 0,
-         scalar_type precision = std::numeric_limits<scalar_type>::epsilon(), int params = 0,
+         scalar_type precision = std::numeric_limits<scalar_type>::epsilon(), Params params = Params(),
          BraKet braket = BraKet::invalid, ScreeningMethod screening_method = default_screening_method())
       : oper_(oper), braket_(braket == BraKet::invalid ? BraKet::xx_xx : braket), screening_(screening_method) {
-    (void)params;
     assert(max_nprim > 0);
-    if (oper != Operator::coulomb) throw std::invalid_argument("libint2_amd: only Operator::coulomb is provided");
+    check_operator(oper);
+    set_params(params);
     if (braket_ == BraKet::x_x) throw std::invalid_argument("libint2_amd: 1-body brakets are not provided");
     if (deriv_order != 0) throw std::invalid_argument("libint2_amd: only deriv_order 0 is provided");
     assert(initialized() && "libint2::initialize() must be called before constructing an Engine");
@@ -217,7 +223,14 @@ class Engine {
   }
   Engine& set(Operator o) {
     if (default_) throw using_default_initialized();
-    if (o != Operator::coulomb) throw std::invalid_argument("libint2_amd: only Operator::coulomb is provided");
+    check_operator(o);
+    oper_ = o;
+    return *this;
+  }
+  /// omega of erf_coulomb / erfc_coulomb (engine.h:685-690)
+  template <typename Params>
+  Engine& set_params(const Params& params) {
+    if constexpr (std::is_arithmetic_v<Params>) omega_ = (double)params;
     return *this;
   }
   Engine& set(BraKet b) {
@@ -296,6 +309,7 @@ class Engine {
     l2g_eri_plan p = nullptr;
     detail::check(l2g_eri_plan_create(b, 1, q, precision_, method_for_plan(), scale_, uniform_ ? 1 : 0, &p));
     std::unique_ptr<l2g_eri_plan_s, int (*)(l2g_eri_plan)> hp(p, &l2g_eri_plan_destroy);
+    detail::check(l2g_eri_plan_set_operator(p, plan_operator(), omega_));
     int64_t n = 0;
     detail::check(l2g_eri_plan_size(p, &n));
     buffer_.resize((size_t)std::max<int64_t>(n, 1));
@@ -346,6 +360,7 @@ class Engine {
     detail::check(l2g_eri_plan_create(b, (int64_t)quartets.size(), q.data(), precision_, method_for_plan(), scale_,
                                       uniform_ ? 1 : 0, &p));
     std::unique_ptr<l2g_eri_plan_s, int (*)(l2g_eri_plan)> hp(p, &l2g_eri_plan_destroy);
+    detail::check(l2g_eri_plan_set_operator(p, plan_operator(), omega_));
     int64_t n = 0;
     detail::check(l2g_eri_plan_size(p, &n));
     std::vector<double> out((size_t)std::max<int64_t>(n, 1));
@@ -371,7 +386,17 @@ class Engine {
     return (int)screening_;
   }
 
+  static void check_operator(Operator o) {
+    if (o != Operator::coulomb && o != Operator::erf_coulomb && o != Operator::erfc_coulomb)
+      throw std::invalid_argument("libint2_amd: only Operator::coulomb, erf_coulomb and erfc_coulomb are provided");
+  }
+  int plan_operator() const {
+    return oper_ == Operator::erf_coulomb ? L2G_OPER_ERF_COULOMB
+                                          : (oper_ == Operator::erfc_coulomb ? L2G_OPER_ERFC_COULOMB : L2G_OPER_COULOMB);
+  }
+
   Operator oper_ = Operator::invalid;
+  double omega_ = 0;
   BraKet braket_ = BraKet::invalid;
   ScreeningMethod screening_ = ScreeningMethod::Original;
   size_t max_nprim_ = 0;

@@ -92,6 +92,15 @@ int main(int argc, char** argv) {
     std::printf("Q 1 4 2 4 %zu\n", n);
     for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", e2.results()[0][i]);
   }
+  // erf-attenuated Coulomb, omega = 0.7 as the Engine's params (printed as R)
+  Engine er(Operator::erf_coulomb, 2, 2, 0, eps, 0.7);
+  er.compute(basis[0], basis[1], basis[2], basis[3]);
+  CHECK(er.results()[0] != nullptr);
+  {
+    const size_t n = basis[0].size() * basis[1].size() * basis[2].size() * basis[3].size();
+    std::printf("R 0 1 2 3 %zu\n", n);
+    for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", er.results()[0][i]);
+  }
   // python/tests/test_libint2.py:39-42: ||(s s|s)|| = 3.6563211198 for
   // s = Shell(0, [(1, 10)])
   Shell su{{1.0}, {{0, false, {10.0}}}, {{0, 0, 0}}};

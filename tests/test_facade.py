@@ -46,14 +46,16 @@ def test_facade_results_match_oracle():
     i = 0
     checked = 0
     while i < len(lines):
-        if lines[i].startswith("Q "):
-            _, a, b, c, d, n = lines[i].split()
+        if lines[i].startswith("Q ") or lines[i].startswith("R "):
+            kind, a, b, c, d, n = lines[i].split()
             n = int(n)
             vals = np.array([float(x) for x in lines[i + 1:i + 1 + n]])
-            o, _, _ = oracle.eri_quartets(arrays, [[int(a), int(b), int(c), int(d)]])
+            # R: erf_coulomb with omega 0.7 (the façade's Engine params)
+            op = dict(oper=oracle.ERF_COULOMB, omega=0.7) if kind == "R" else {}
+            o, _, _ = oracle.eri_quartets(arrays, [[int(a), int(b), int(c), int(d)]], **op)
             assert compare(vals, o, 1e-12, 1e-10)[0] == 0
             checked += 1
             i += n + 1
         else:
             i += 1
-    assert checked == 7  # 5 quartets, one 3-centre and one 2-centre set
+    assert checked == 8  # 5 quartets, one 3-centre, one 2-centre and one erf set
