@@ -531,10 +531,14 @@ class Engine:
                "XSXS": (0, None, 1, None)}
 
     def __init__(self, max_nprim: int, max_l: int, precision: float = np.finfo(float).eps,
-                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None, braket: str = "XXXX"):
+                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None, braket: str = "XXXX",
+                 deriv_order: int = 0):
         if braket not in self.BRAKETS:
             raise ValueError(f"braket must be one of {sorted(self.BRAKETS)}")
+        if deriv_order not in (0, 1) or (deriv_order and braket != "XXXX"):
+            raise ValueError("deriv_order 0, or 1 for 4-centre integrals")
         self.braket = braket
+        self.deriv_order = deriv_order
         if max_nprim <= 0:
             raise ValueError("max_nprim must be positive")
         if max_l > globals()["max_l"]():
@@ -569,6 +573,12 @@ class Engine:
             if s.nprim() > self.max_nprim:
                 raise ValueError("shell has more primitives than max_nprim")
         quad = [shells[k] if k is not None else Shell.unit() for k in slots]
+        if self.deriv_order == 1:
+            # 12 shell sets, d/dA_x .. d/dD_z (libint's results()[0..11])
+            from .deriv import eri_deriv1
+            vals, _, nz = eri_deriv1(self.ctx, BasisArrays.from_shells(quad), [[0, 1, 2, 3]], self.precision,
+                                     self.screening)
+            return [vals[d] for d in range(12)] if nz[0] else None
         b = Basis(self.ctx, BasisArrays.from_shells(quad))
         vals, _, nz = compute_quartets(b, [[0, 1, 2, 3]], self.precision, self.screening, self.scale,
                                        self.uniform_cart)

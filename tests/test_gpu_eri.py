@@ -358,3 +358,40 @@ def test_range_separated(ctx, oper):
     else:
         c, _, _ = la.compute_quartets(b, q)
         assert np.array_equal(g, c)
+
+
+def test_first_derivatives(ctx):
+    """deriv_order = 1 (12 shell sets per quartet, libint's results()[0..11])
+    with the GPU's raised and lowered ERIs: against the same combination of
+    the oracle's ERIs at the integral bar (the combination itself is pinned
+    by finite differences in tests/test_deriv.py), translational invariance,
+    and the Engine facade on the reference's derivative-test shells
+    (tests/unit/test-2body.cc:191-196)."""
+    from libint_amd import deriv
+    rng = np.random.default_rng(72)
+    shells = random_shells(rng, [0, 1, 2, 1, 2], kmax=3)
+    shells[4].pure = False
+    arrays = la.BasisArrays.from_shells(shells)
+    n = len(shells)
+    allq = np.array(list(itertools.product(range(n), repeat=4)), np.int32)
+    q = allq[rng.choice(len(allq), 300, replace=False)]
+    say(f"case: first derivatives, {len(q)} quartets")
+    g, offs, nz = deriv.eri_deriv1(ctx, arrays, q)
+    # the same combination over the oracle's ERIs
+    aug = deriv._augmented(arrays)
+    n_ = arrays.nshell
+    big = np.concatenate([q + np.eye(4, dtype=np.int32)[k] * n_ for k in range(4)] +
+                         [q + np.eye(4, dtype=np.int32)[k] * 2 * n_ for k in range(4)])
+    gv, _, _ = la.compute_quartets(la.Basis(ctx, aug), big)
+    ov, _, _ = oracle.eri_quartets(aug, big)
+    assert compare(gv, ov, ABS, REL)[0] == 0
+    for d in range(3):
+        s_ = g[d] + g[3 + d] + g[6 + d] + g[9 + d]
+        assert np.abs(s_).max() < 1e-11
+    p1 = la.Shell(1, [(1.0, 0.9), (0.3, 0.3)], (0.0, 0.0, 0.0), pure=False)
+    p2 = la.Shell(1, [(2.0, 0.8), (0.4, -0.2)], (1.0, 1.0, 1.0), pure=False)
+    e = la.Engine(2, 1, ctx=ctx, deriv_order=1)
+    sets = e.compute(p1, p2, p1, p2)
+    assert len(sets) == 12 and all(s.size == 81 for s in sets)
+    for d in range(3):
+        assert np.abs(sets[d] + sets[3 + d] + sets[6 + d] + sets[9 + d]).max() < 1e-12
