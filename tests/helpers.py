@@ -63,3 +63,27 @@ def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol):
         narb += int(m.sum())
         oerr = max(oerr, float(np.abs(ref[a:e] - vb).max()))
     return nfail, narb, oerr
+
+
+def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100):
+    """Closed-shell SCF as the reference's hartree-fock.cc driver does
+    (tests/hartree-fock/hartree-fock.cc:181-236): F = H + G(D) with
+    G = 2J - K from fock2e(D), D = C_occ C_occ^T, energy sum D (H + F) +
+    E_nuc, stopping when the energy change and the density change (Frobenius
+    norm) both fall below conv; core-Hamiltonian guess.  Returns the energy."""
+    import scipy.linalg
+    _, C = scipy.linalg.eigh(H, S)
+    D = C[:, :nocc] @ C[:, :nocc].T
+    e_last = 0.0
+    for it in range(maxiter):
+        F = H + fock2e(D)
+        e = float(np.sum(D * (H + F)))
+        _, C = scipy.linalg.eigh(F, S)
+        Dn = C[:, :nocc] @ C[:, :nocc].T
+        rmsd = np.linalg.norm(Dn - D)
+        ediff = e - e_last
+        D, e_last = Dn, e
+        if abs(ediff) < conv and rmsd < conv:
+            break
+    F = H + fock2e(D)
+    return float(np.sum(D * (H + F))) + enuc
