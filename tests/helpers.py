@@ -65,19 +65,40 @@ def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol):
     return nfail, narb, oerr
 
 
-def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100):
+def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0):
     """Closed-shell SCF as the reference's hartree-fock.cc driver does
     (tests/hartree-fock/hartree-fock.cc:181-236): F = H + G(D) with
     G = 2J - K from fock2e(D), D = C_occ C_occ^T, energy sum D (H + F) +
     E_nuc, stopping when the energy change and the density change (Frobenius
-    norm) both fall below conv; core-Hamiltonian guess.  Returns the energy."""
+    norm) both fall below conv; core-Hamiltonian guess.  With diis > 0 the
+    Fock matrix is Pulay-extrapolated over the last diis iterations on the
+    commutator error F D S - S D F, as hartree-fock++.cc does with
+    libint2::DIIS (tests/hartree-fock/hartree-fock++.cc:296-330); the
+    converged energy is the same.  Returns the energy."""
     import scipy.linalg
     _, C = scipy.linalg.eigh(H, S)
     D = C[:, :nocc] @ C[:, :nocc].T
     e_last = 0.0
+    fs, errs = [], []
     for it in range(maxiter):
         F = H + fock2e(D)
         e = float(np.sum(D * (H + F)))
+        if diis:
+            err = F @ D @ S - S @ D @ F
+            fs.append(F)
+            errs.append(err)
+            fs, errs = fs[-diis:], errs[-diis:]
+            m = len(fs)
+            if m > 1:
+                B = -np.ones((m + 1, m + 1))
+                B[m, m] = 0.0
+                for i in range(m):
+                    for j in range(m):
+                        B[i, j] = float(np.sum(errs[i] * errs[j]))
+                rhs = np.zeros(m + 1)
+                rhs[m] = -1.0
+                c = np.linalg.lstsq(B, rhs, rcond=None)[0]
+                F = sum(ci * fi for ci, fi in zip(c[:m], fs))
         _, C = scipy.linalg.eigh(F, S)
         Dn = C[:, :nocc] @ C[:, :nocc].T
         rmsd = np.linalg.norm(Dn - D)
