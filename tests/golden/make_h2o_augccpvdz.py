@@ -3,13 +3,16 @@ as libint's BasisSet("aug-cc-pVDZ") assembles them from lib/basis/cc-pvdz.g94
 and lib/basis/augmentation-cc-pvdz.g94 (include/libint2/basis.h.in:134-200,
 388-400; d shells pure, :568) and the water geometry of
 tests/hartree-fock/h2o_rotated.xyz, for the end-to-end Hartree-Fock check
-against the reference's -76.003354058439
-(tests/hartree-fock/hartree-fock++-validate.py:47-48).  Data only; run here
+against the reference's -76.003354058439 and its force components
+(tests/hartree-fock/hartree-fock++-validate.py:47-48, 74-103: 1-body, Pulay,
+2-body, nuclear-repulsion and total forces, read from that file's text).
+Data only; run here
 with the reference checked out:
     python tests/golden/make_h2o_augccpvdz.py /root/reference
 """
 import json
 import os
+import re
 import sys
 
 ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
@@ -46,5 +49,19 @@ basis = {el: base[el] + aug[el] for el in ("H", "O")}
 xyz = open(os.path.join(ref, "tests", "hartree-fock", "h2o_rotated.xyz")).read().split("\n")
 natom = int(xyz[0])
 atoms = [(l.split()[0], [float(x) for x in l.split()[1:4]]) for l in xyz[2:2 + natom]]
-json.dump({"basis": basis, "atoms_angstrom": atoms, "energy": -76.003354058439, "energy_tolerance": 5e-12},
+val = open(os.path.join(ref, "tests", "hartree-fock", "hartree-fock++-validate.py")).read()
+
+
+def vector(name):
+    return [float(x) for x in re.search(name + r"\s*=\s*\[([^\]]*)\]", val).group(1).split(",")]
+
+
+def number(name):
+    return float(re.search(name + r"\s*=\s*([-+0-9.e]+)", val).group(1))
+
+
+forces = {k: {"values": vector(k + "ref"), "tolerance": number(k + "tol")} for k in ("F1", "FP", "F2", "FN", "F")}
+assert vector("eref") == [-76.003354058439] and number("etol") == 5e-12
+json.dump({"basis": basis, "atoms_angstrom": atoms, "energy": -76.003354058439, "energy_tolerance": 5e-12,
+           "forces": forces},
           open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "h2o_augccpvdz.json"), "w"), indent=1)

@@ -65,7 +65,7 @@ def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol):
     return nfail, narb, oerr
 
 
-def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0):
+def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0, density=False):
     """Closed-shell SCF as the reference's hartree-fock.cc driver does
     (tests/hartree-fock/hartree-fock.cc:181-236): F = H + G(D) with
     G = 2J - K from fock2e(D), D = C_occ C_occ^T, energy sum D (H + F) +
@@ -74,7 +74,8 @@ def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0):
     Fock matrix is Pulay-extrapolated over the last diis iterations on the
     commutator error F D S - S D F, as hartree-fock++.cc does with
     libint2::DIIS (tests/hartree-fock/hartree-fock++.cc:296-330); the
-    converged energy is the same.  Returns the energy."""
+    converged energy is the same.  Returns the energy, or with density=True
+    (energy, D) with D the density the energy was evaluated on."""
     import scipy.linalg
     _, C = scipy.linalg.eigh(H, S)
     D = C[:, :nocc] @ C[:, :nocc].T
@@ -107,4 +108,5 @@ def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0):
         if abs(ediff) < conv and rmsd < conv:
             break
     F = H + fock2e(D)
-    return float(np.sum(D * (H + F))) + enuc
+    e = float(np.sum(D * (H + F))) + enuc
+    return (e, D) if density else e

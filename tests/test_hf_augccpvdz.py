@@ -24,11 +24,14 @@ ZOF = {"H": 1, "O": 8}
 def water():
     d = json.load(open(os.path.join(HERE, "golden", "h2o_augccpvdz.json")))
     atoms = [(ZOF[el], np.array(x) / BOHR_TO_ANGSTROM) for el, x in d["atoms_angstrom"]]
-    shells = []
-    for (Z, pos), (el, _) in zip(atoms, d["atoms_angstrom"]):
+    shells, shell2atom = [], []
+    for a, ((Z, pos), (el, _)) in enumerate(zip(atoms, d["atoms_angstrom"])):
         for s in d["basis"][el]:
             shells.append(la.Shell(s["l"], list(zip(s["exponents"], s["coefficients"])), tuple(pos),
                                    pure=s["l"] > 1))
+            shell2atom.append(a)
+    water.shell2atom = shell2atom
+    water.forces = d["forces"]
     enuc = sum(atoms[i][0] * atoms[j][0] / np.linalg.norm(atoms[i][1] - atoms[j][1])
                for i in range(len(atoms)) for j in range(i))
     return atoms, shells, enuc, d["energy"], d["energy_tolerance"]
@@ -67,9 +70,17 @@ def test_rhf_energy_gpu_augccpvdz(ctx):
         G, _, _ = fb.build(D, 1e-15)
         return G
 
-    e = rhf(S, H, 5, g, enuc, diis=8)
+    e, D = rhf(S, H, 5, g, enuc, diis=8, density=True)
     print(f"RHF/aug-cc-pVDZ rotated water on the GPU Fock build: {e:.12f} (reference {eref})", flush=True)
     assert abs(e - eref) < tol, e
+    # two-electron part of the energy gradient (the reference prints it as
+    # "2-body forces"): tr(G1_x D) with G1 from the GPU derivative ERIs
+    from libint_amd import deriv
+    G1 = deriv.fock_deriv1(ctx, la.BasisArrays.from_shells(shells), water.shell2atom, len(atoms), D, 1e-15)
+    F2 = np.array([np.sum(G1[x] * D) for x in range(3 * len(atoms))])
+    ref = water.forces["F2"]
+    print("  2-body forces", F2, flush=True)
+    assert np.abs(F2 - ref["values"]).max() < ref["tolerance"], F2 - ref["values"]
 
 
 def test_rhf_energy_oracle_augccpvdz():
@@ -86,3 +97,68 @@ def test_rhf_energy_oracle_augccpvdz():
 
     e = rhf(S, H, 5, g, enuc, diis=8)
     assert abs(e - eref) < tol, e
+
+
+def nuclear_repulsion_gradient(atoms):
+    """d E_nuc / d x, as hartree-fock++.cc:672-701 forms it."""
+    F = np.zeros((len(atoms), 3))
+    for i in range(len(atoms)):
+        for j in range(i):
+            r = atoms[i][1] - atoms[j][1]
+            f = -r * atoms[i][0] * atoms[j][0] / np.linalg.norm(r) ** 3
+            F[i] += f
+            F[j] -= f
+    return F.reshape(-1)
+
+
+def test_nuclear_repulsion_force():
+    """Pins the geometry (units, atom order) at the reference's 1e-10."""
+    atoms = water()[0]
+    ref = water.forces["FN"]
+    assert np.abs(nuclear_repulsion_gradient(atoms) - ref["values"]).max() < ref["tolerance"]
+
+
+def test_two_body_force_oracle(monkeypatch):
+    """The reference's "2-body forces" golden, with the oracle's ERIs in
+    deriv.fock_deriv1 (the same host combination and Fock-derivative digest
+    the GPU run uses), and by an independent route: Richardson-extrapolated
+    central differences of tr(D G(D)) in the atom positions at fixed D."""
+    import oracle
+    from libint_amd import deriv
+    atoms, shells, enuc, eref, tol = water()
+    S, H = one_body_pure(atoms, shells)
+    arrays = la.BasisArrays.from_shells(shells)
+
+    def g(D, arr=arrays):
+        G, _, _, _ = oracle.fock_2e(arr, D, precision=1e-15)
+        return G
+
+    e, D = rhf(S, H, 5, g, enuc, diis=8, density=True)
+    assert abs(e - eref) < tol
+    ref = water.forces["F2"]
+
+    class OracleBasis:
+        def __init__(self, ctx, arr):
+            self.arrays = arr
+
+    monkeypatch.setattr(deriv, "Basis", OracleBasis)
+    monkeypatch.setattr(deriv, "compute_quartets",
+                        lambda b, q, p, s: oracle.eri_quartets(b.arrays, q, precision=p, screening=s))
+    G1 = deriv.fock_deriv1(None, arrays, water.shell2atom, len(atoms), D, 1e-15)
+    F2 = np.array([np.sum(G1[x] * D) for x in range(3 * len(atoms))])
+    assert np.abs(F2 - ref["values"]).max() < ref["tolerance"], F2 - ref["values"]
+
+    def e2(a, x, h):
+        shift = np.eye(3)[x] * h
+        moved = [la.Shell(s.l, list(zip(s.alpha, s.coeff)), tuple(s.O + shift * (at == a)), pure=s.pure,
+                          renorm=False) for s, at in zip(shells, water.shell2atom)]
+        return float(np.sum(D * g(D, la.BasisArrays.from_shells(moved))))
+
+    fd = []
+    for a in range(len(atoms)):
+        for x in range(3):
+            h = 1e-3
+            d1 = (e2(a, x, h) - e2(a, x, -h)) / (2 * h)
+            d2 = (e2(a, x, h / 2) - e2(a, x, -h / 2)) / h
+            fd.append((4 * d2 - d1) / 3)
+    assert np.abs(np.array(fd) - ref["values"]).max() < 1e-7, np.array(fd) - ref["values"]
