@@ -211,6 +211,31 @@ int l2g_eri_plan_elapsed(l2g_eri_plan plan, float* ms);
 
 int l2g_eri_plan_destroy(l2g_eri_plan plan);
 
+/* ---- first geometric derivatives (deriv_order = 1) --------------------- */
+
+/* Plans the first derivatives of a batch of quartets, replacing
+ * Engine(Operator::coulomb, max_nprim, max_l, 1)::compute
+ * (engine.h:502-526, 747-749; compute2<coulomb, xx_xx, 1>, engine.impl.h:
+ * 1151-2113): per quartet 12 shell sets, d = 3 * centre + axis for the
+ * shells' centres in caller order (CartesianDerivIterator<4>,
+ * util/intpart_iter.h:89-103), each laid out like the integrals of
+ * l2g_eri_plan_create.  Derivative set d of quartet q is at
+ * out[d * total + offsets[q]].  Every shell needs l + 1 <= l2g_max_l()
+ * (L2G_E_LMAX otherwise).  The solid-harmonics ordering is the context's
+ * at execution. */
+typedef struct l2g_deriv1_plan_s* l2g_deriv1_plan;
+int l2g_eri_deriv1_plan_create(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision,
+                               int screening, l2g_deriv1_plan* plan);
+/* doubles per derivative set (the output holds 12 * total) */
+int l2g_eri_deriv1_plan_size(l2g_deriv1_plan plan, int64_t* total);
+int l2g_eri_deriv1_plan_offsets(l2g_deriv1_plan plan, int64_t* offsets);
+/* d_out: 12 * total device doubles, or NULL for the plan's own buffer */
+int l2g_eri_deriv1_plan_execute(l2g_deriv1_plan plan, double* d_out);
+/* host_out: 12 * total doubles (plan's own buffer only); nonzero[q] = 0 when
+ * every primitive of the quartet's raised and lowered ERIs was screened out */
+int l2g_eri_deriv1_plan_download(l2g_deriv1_plan plan, double* host_out, int8_t* nonzero);
+int l2g_eri_deriv1_plan_destroy(l2g_deriv1_plan plan);
+
 /* Schwarz factors K(s1,s2) = sqrt(max|(s1 s2|s1 s2)|) (norm = 0) or
  * sqrt(||(s1 s2|s1 s2)||_2) (norm = 1) for npairs shell pairs
  * (pairs[2*i], pairs[2*i+1]), on the GPU.  Replaces the per-pair loop of

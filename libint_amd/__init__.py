@@ -69,6 +69,13 @@ ABI = {
     "l2g_eri_plan_set_operator": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "l2g_eri_plan_elapsed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
+    "l2g_eri_deriv1_plan_create": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int,
+                                                  ctypes.POINTER(_vp)]),
+    "l2g_eri_deriv1_plan_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "l2g_eri_deriv1_plan_offsets": (ctypes.c_int, [_vp, _i64p]),
+    "l2g_eri_deriv1_plan_execute": (ctypes.c_int, [_vp, _vp]),
+    "l2g_eri_deriv1_plan_download": (ctypes.c_int, [_vp, _vp, _vp]),
+    "l2g_eri_deriv1_plan_destroy": (ctypes.c_int, [_vp]),
     "l2g_eri_compute": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp]),
     "l2g_schwarz": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, _f64p]),
     "l2g_fock_create": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -402,6 +409,52 @@ class EriPlan:
     def _release(self):
         if self._h:
             lib().l2g_eri_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+
+class Deriv1Plan:
+    """First geometric derivatives of a batch of quartets (deriv_order = 1):
+    12 shell sets per quartet, d = 3 * centre + axis in caller order."""
+
+    def __init__(self, basis: Basis, quartets, precision: float = np.finfo(float).eps,
+                 screening: int = SCREEN_ORIGINAL):
+        self.basis = basis
+        q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+        self.nq = q.shape[0]
+        h = _vp()
+        _check(lib().l2g_eri_deriv1_plan_create(basis.handle, self.nq, q.reshape(-1), precision, screening,
+                                                ctypes.byref(h)))
+        self._h = h
+        basis.ctx._children.add(self)
+        n = ctypes.c_int64()
+        _check(lib().l2g_eri_deriv1_plan_size(h, ctypes.byref(n)))
+        self.size = n.value
+
+    def offsets(self) -> np.ndarray:
+        o = np.zeros(self.nq, np.int64)
+        if self.nq:
+            _check(lib().l2g_eri_deriv1_plan_offsets(self._h, o))
+        return o
+
+    def execute(self, d_out: int | None = None) -> None:
+        _check(lib().l2g_eri_deriv1_plan_execute(self._h, d_out))
+
+    def download(self):
+        """(values of shape (12, size), nonzero[nq])"""
+        out = np.zeros(max(12 * self.size, 1))
+        nz = np.zeros(max(self.nq, 1), np.int8)
+        _check(lib().l2g_eri_deriv1_plan_download(self._h, out.ctypes.data, nz.ctypes.data))
+        return out[: 12 * self.size].reshape(12, self.size), nz[: self.nq]
+
+    def _release(self):
+        if self._h:
+            lib().l2g_eri_deriv1_plan_destroy(self._h)
             self._h = None
 
     def __del__(self):

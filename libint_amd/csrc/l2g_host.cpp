@@ -996,3 +996,203 @@ int l2g_precision_bounds(l2g_basis B, int64_t npairs, const int32_t* pairs, doub
 
 }  // extern "C"
 
+
+// ===========================================================================
+// First geometric derivatives (deriv_order = 1)
+// ===========================================================================
+// Engine(Operator::coulomb, max_nprim, max_l, 1)::compute returns, per
+// quartet, 12 shell sets d/dA_x .. d/dD_z for the shells' centres in caller
+// order (CartesianDerivIterator<4>, include/libint2/util/intpart_iter.h:
+// 89-103, 161-182).  The plan evaluates them from ERIs of an augmented
+// basis, shells 0..n-1 the caller's shells made Cartesian, n..2n-1 raised
+// (l+1, coefficients 2 alpha c), 2n..3n-1 lowered (l-1; an unused s copy for
+// l = 0), in one ERI plan over the 8 nq quartets with one shell raised or
+// lowered; a combination kernel forms the derivatives and applies the
+// solid-harmonic transform of the pure centres.
+struct l2g_deriv1_plan_s {
+  l2g_context_s* ctx = nullptr;
+  l2g_basis aug = nullptr;
+  l2g_eri_plan inner = nullptr;
+  int64_t nq = 0, total = 0;
+  std::vector<int64_t> offsets;
+  std::vector<int> lq;  // l of the four shells of each quartet, 8 bits each
+  DevBuf raw_off, out_off, qinfo, out;
+  ~l2g_deriv1_plan_s() {
+    if (inner) l2g_eri_plan_destroy(inner);
+    if (aug) l2g_basis_destroy(aug);
+  }
+};
+
+namespace {
+
+L2gDerivTab deriv_tab(bool gaussian) {
+  L2gDerivTab t;
+  std::memset(&t, 0, sizeof(t));
+  for (int l = 0; l <= 3; ++l)
+    for (int f = 0; f < 2 * l + 1; ++f) {
+      // output position f -> m (Standard: m = f - l; Gaussian: 0, +1, -1, +2, ...)
+      const int m = gaussian ? (f == 0 ? 0 : ((f & 1) ? (f + 1) / 2 : -f / 2)) : f - l;
+      int c = 0, k = 0;
+      for (int x = l; x >= 0; --x)
+        for (int y = l - x; y >= 0; --y, ++c) {
+          const double v = l2g_solid_harmonics_coeff(l, m, x, y, l - x - y);
+          if (v == 0.0) continue;
+          if (k == 4) throw std::logic_error("solid-harmonic row with more than 4 terms");
+          t.cidx[l][f][k] = (signed char)c;
+          t.coef[l][f][k] = v;
+          ++k;
+        }
+      t.nnz[l][f] = (signed char)k;
+    }
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int l2g_eri_deriv1_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets, double precision, int screening,
+                               l2g_deriv1_plan* out) {
+  return guarded([&] {
+    if (!B) throw L2gError(L2G_E_STATE, "null basis");
+    if (!out || nq < 0 || (nq > 0 && !quartets)) throw std::invalid_argument("l2g_eri_deriv1_plan_create: bad arguments");
+    l2g_context_s* ctx = B->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = B->nshell;
+    for (int64_t i = 0; i < 4 * nq; ++i)
+      if (quartets[i] < 0 || quartets[i] >= n) throw std::invalid_argument("shell index out of range");
+    for (int64_t i = 0; i < 4 * nq; ++i)
+      if (B->l[quartets[i]] + 1 > lmax_supported())
+        throw L2gError(L2G_E_LMAX, "first derivatives need l + 1 <= " + std::to_string(lmax_supported()) +
+                                       " (angular momentum " + std::to_string(B->l[quartets[i]]) + ")");
+    auto p = std::make_unique<l2g_deriv1_plan_s>();
+    p->ctx = ctx;
+    p->nq = nq;
+    // augmented basis
+    std::vector<int> al, ap, an;
+    std::vector<double> aa, ac, ao;
+    for (int kind = 0; kind < 3; ++kind)
+      for (int s = 0; s < n; ++s) {
+        const int l = B->l[s];
+        al.push_back(kind == 1 ? l + 1 : (kind == 2 ? std::max(l - 1, 0) : l));
+        ap.push_back(0);
+        an.push_back(B->nprim[s]);
+        for (int k = B->off[s]; k < B->off[s + 1]; ++k) {
+          aa.push_back(B->alpha[k]);
+          ac.push_back(kind == 1 ? 2.0 * B->alpha[k] * B->coeff[k] : B->coeff[k]);
+        }
+        for (int x = 0; x < 3; ++x) ao.push_back(B->origin[3 * s + x]);
+      }
+    int rc = l2g_basis_create(ctx, 3 * n, al.data(), ap.data(), an.data(), aa.data(), ac.data(), ao.data(), &p->aug);
+    if (rc) throw L2gError(rc, g_err);
+    std::vector<int32_t> big((size_t)32 * nq);
+    for (int k = 0; k < 8; ++k)
+      for (int64_t q = 0; q < nq; ++q)
+        for (int j = 0; j < 4; ++j)
+          big[4 * ((size_t)k * nq + q) + j] = quartets[4 * q + j] + (j == (k & 3) ? (k < 4 ? n : 2 * n) : 0);
+    rc = l2g_eri_plan_create(p->aug, 8 * nq, big.data(), precision, screening, 1.0, 0, &p->inner);
+    if (rc) throw L2gError(rc, g_err);
+    p->offsets.resize(nq);
+    p->lq.resize(nq);
+    std::vector<int> qinfo(nq);
+    int64_t total = 0;
+    for (int64_t q = 0; q < nq; ++q) {
+      const int32_t* s = quartets + 4 * q;
+      int info = 0, lw = 0;
+      int64_t sz = 1;
+      for (int k = 0; k < 4; ++k) {
+        const int l = B->l[s[k]];
+        const bool pure = B->pure[s[k]] && l > 0;
+        info |= l << (4 * k);
+        if (pure) info |= 1 << (16 + k);
+        lw |= l << (8 * k);
+        sz *= nfun(l, pure ? 1 : 0);
+      }
+      qinfo[q] = info;
+      p->lq[q] = lw;
+      p->offsets[q] = total;
+      total += sz;
+    }
+    p->total = total;
+    std::vector<long long> ro(p->inner->offsets.begin(), p->inner->offsets.end());
+    std::vector<long long> oo(p->offsets.begin(), p->offsets.end());
+    upload(p->raw_off, ro, ctx->stream);
+    upload(p->out_off, oo, ctx->stream);
+    upload(p->qinfo, qinfo, ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ++ctx->live;
+    *out = p.release();
+  });
+}
+
+int l2g_eri_deriv1_plan_size(l2g_deriv1_plan p, int64_t* total) {
+  return guarded([&] {
+    if (!p || !total) throw std::invalid_argument("null argument");
+    *total = p->total;
+  });
+}
+
+int l2g_eri_deriv1_plan_offsets(l2g_deriv1_plan p, int64_t* offsets) {
+  return guarded([&] {
+    if (!p || !offsets) throw std::invalid_argument("null argument");
+    std::memcpy(offsets, p->offsets.data(), p->offsets.size() * sizeof(int64_t));
+  });
+}
+
+int l2g_eri_deriv1_plan_execute(l2g_deriv1_plan p, double* d_out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    l2g_context_s* ctx = p->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    double* out = d_out;
+    if (!out) {
+      p->out.alloc(std::max<int64_t>(12 * p->total, 1) * sizeof(double));
+      out = p->out.as<double>();
+    }
+    int rc = l2g_eri_plan_execute(p->inner, nullptr);
+    if (rc) throw L2gError(rc, g_err);
+    if (p->nq == 0) return;
+    launch_deriv1_combine(p->inner->out.as<double>(), p->raw_off.as<long long>(), p->out_off.as<long long>(),
+                          p->qinfo.as<int>(), p->nq, p->total, out, deriv_tab(ctx->shg_gaussian != 0), ctx->stream);
+  });
+}
+
+int l2g_eri_deriv1_plan_download(l2g_deriv1_plan p, double* host_out, int8_t* nonzero) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("null plan");
+    l2g_context_s* ctx = p->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("kernel execution failed: ") + hipGetErrorString(e));
+    if (host_out && p->total > 0) {
+      if (!p->out.p) throw L2gError(L2G_E_STATE, "plan has not been executed into its own buffer");
+      HIPCHK(hipMemcpy(host_out, p->out.p, 12 * p->total * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (nonzero && p->nq > 0) {
+      std::vector<int8_t> nz(8 * p->nq);
+      int rc = l2g_eri_plan_download(p->inner, nullptr, nz.data());
+      if (rc) throw L2gError(rc, g_err);
+      for (int64_t q = 0; q < p->nq; ++q) {
+        int8_t v = 0;
+        for (int k = 0; k < 4; ++k) {
+          v |= nz[(size_t)k * p->nq + q];
+          if ((p->lq[q] >> (8 * k)) & 255) v |= nz[(size_t)(4 + k) * p->nq + q];
+        }
+        nonzero[q] = v ? 1 : 0;
+      }
+    }
+  });
+}
+
+int l2g_eri_deriv1_plan_destroy(l2g_deriv1_plan p) {
+  if (p) {
+    l2g_context_s* ctx = p->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete p;
+    --ctx->live;
+  }
+  return L2G_OK;
+}
+
+}  // extern "C"

@@ -321,6 +321,22 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
 // K into kg, G = 2J - K into g)
 void launch_fock_finish(double* jg, double* kg, double* g, long long N, hipStream_t s);
 
+// Solid-harmonic rows in output order (Standard or Gaussian) for l <= 3: the
+// Cartesian components and coefficients of output function f of a pure
+// shell of angular momentum l are cidx/coef[l][f][0 .. nnz[l][f]).
+struct L2gDerivTab {
+  signed char nnz[4][7];
+  signed char cidx[4][7][4];
+  double coef[4][7][4];
+};
+
+// l2g_aux.hip: first-derivative shell sets from the raised and lowered ERIs
+// of a derivative plan (one block per quartet); raw_off[k * nq + q] and
+// raw_off[(4 + k) * nq + q] locate quartet q's ERIs with centre k raised and
+// lowered; qinfo[q] packs l (4 bits per centre) and the pure flags (bit 16 + k).
+void launch_deriv1_combine(const double* raw, const long long* raw_off, const long long* out_off, const int* qinfo,
+                           long long nq, long long total, double* out, const L2gDerivTab& tab, hipStream_t s);
+
 // l2g_generic.hip: the table-driven kernel for classes without a generated
 // register-resident kernel (f shells); modes ERI store, Fock digest, Schwarz
 enum { kGenericEri = 0, kGenericFock = 1, kGenericSchwarz = 2 };

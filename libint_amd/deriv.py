@@ -6,62 +6,19 @@ for the centres of the shells in caller order (the order of
 CartesianDerivIterator<4>, include/libint2/util/intpart_iter.h:89-103,
 161-182), each row-major in caller order like the integrals themselves.
 
-Differentiating a Cartesian Gaussian with respect to its centre,
-
-    d/dA_i  x^a_i e^{-alpha r^2} = 2 alpha x^(a_i+1) e^{...} - a_i x^(a_i-1) e^{...},
-
-turns each derivative shell set into ERIs of shells with l raised by one (the
-contraction coefficients multiplied by 2 alpha, exponents unchanged) and
-lowered by one.  Those ERIs are evaluated on the GPU in one plan (the same
-kernels, parity and precision as the integrals themselves), and combined per
-Cartesian component; pure shells are transformed to solid harmonics after
-the combination, which commutes with it.  The combination itself is a few
-multiply-adds per derivative integral on the host.
+eri_deriv1 runs the device derivative plan (l2g_eri_deriv1_plan_*): the ERIs
+with one shell raised (l+1, coefficients times 2 alpha) and lowered (l-1)
+in one ERI plan on the same kernels, and a combination kernel that forms
+d/dA_i = raised - a_i lowered per Cartesian component and applies the
+solid-harmonic rows of the pure centres.  fock_deriv1 digests the
+derivatives into the derivative Fock matrices as the reference's
+Hartree-Fock driver does.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from . import (SCREEN_ORIGINAL, Basis, BasisArrays, compute_quartets, solid_harmonics_coeff)
-
-
-def _cart(l):
-    return [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
-
-
-def _cart_index(x, y, z):
-    l = x + y + z
-    return (l - x) * (l - x + 1) // 2 + (l - x - y)
-
-
-def _sph_matrix(l):
-    """Rows m = -l..l (libint's standard ordering), columns Cartesian."""
-    return np.array([[solid_harmonics_coeff(l, m, *c) for c in _cart(l)] for m in range(-l, l + 1)])
-
-
-def _augmented(arrays: BasisArrays):
-    """Cartesian copies of the shells, then the raised (l+1, coefficients
-    2 alpha c) and the lowered (l-1, same coefficients; an unused s copy for
-    l = 0) shells."""
-    n = arrays.nshell
-    off = np.concatenate([[0], np.cumsum(arrays.nprim)])
-    ls, pure, nprim, al, co, org = [], [], [], [], [], []
-    for kind in range(3):
-        for s in range(n):
-            a = arrays.alpha[off[s]:off[s + 1]]
-            c = arrays.coeff[off[s]:off[s + 1]]
-            l = int(arrays.l[s])
-            if kind == 1:
-                l, c = l + 1, 2.0 * a * c
-            elif kind == 2:
-                l = max(l - 1, 0)
-            ls.append(l)
-            pure.append(0)
-            nprim.append(len(a))
-            al.append(a)
-            co.append(c)
-            org.append(arrays.origin[3 * s:3 * s + 3])
-    return BasisArrays(ls, pure, nprim, np.concatenate(al), np.concatenate(co), np.concatenate(org))
+from . import SCREEN_ORIGINAL, Basis, BasisArrays, Deriv1Plan
 
 
 def eri_deriv1(ctx, arrays: BasisArrays, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL):
@@ -71,86 +28,15 @@ def eri_deriv1(ctx, arrays: BasisArrays, quartets, precision=np.finfo(float).eps
     shell sets d = 3 * centre + axis back to back, quartet q's set at
     values[d][offsets[q]: offsets[q] + size_q]; nonzero[q] is 0 when every
     primitive of the raised and lowered quartets was screened out."""
-    q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
-    n = arrays.nshell
-    aug = _augmented(arrays)
-    b = Basis(ctx, aug)
-    L = arrays.l
-    # raised / lowered quartets per centre
-    big = []
-    for k in range(4):
-        r = q.copy()
-        r[:, k] += n
-        big.append(r)
-    for k in range(4):
-        r = q.copy()
-        r[:, k] += 2 * n
-        big.append(r)
-    allq = np.concatenate(big) if len(q) else np.zeros((0, 4), np.int32)
-    vals, offs, nz = compute_quartets(b, allq, precision, screening)
-    nq = len(q)
-    size = np.ones(nq, np.int64)
-    sizes = arrays.sizes()
-    for k in range(4):
-        size *= sizes[q[:, k]]
-    out_off = np.concatenate([[0], np.cumsum(size)[:-1]]) if nq else np.zeros(0, np.int64)
-    total = int(size.sum())
-    out = np.zeros((12, max(total, 1)))
-    if nq == 0:
-        return out[:, :0], out_off, np.zeros(0, np.int8)
-    ls4 = L[q].astype(np.int64)
-    pure4 = (arrays.pure[q] != 0) & (ls4 > 0)
-    nzr = nz.reshape(8, nq)
-    nonzero = np.zeros(nq, np.int8)
-    for k in range(4):
-        nonzero |= nzr[k] | np.where(ls4[:, k] > 0, nzr[4 + k], 0).astype(np.int8)
-    sph = {}
-    # quartets of one class (same l and pure flag on each centre) share block
-    # shapes: each class is combined in one batch, quartet index leading
-    key = (ls4 * 2 + pure4) @ np.array([512, 64, 8, 1])
-    for kv in np.unique(key):
-        idx = np.nonzero(key == kv)[0]
-        m = len(idx)
-        ls = [int(x) for x in ls4[idx[0]]]
-        pure = [bool(x) for x in pure4[idx[0]]]
-        nc = [(l + 1) * (l + 2) // 2 for l in ls]
-        osz = int(size[idx[0]])
-        dst = out_off[idx][:, None] + np.arange(osz)
-
-        def gather(j, dims):
-            n = int(np.prod(dims))
-            return vals[offs[j][:, None] + np.arange(n)].reshape([m] + dims)
-
-        for k in range(4):
-            dims_p = list(nc)
-            dims_p[k] = (ls[k] + 2) * (ls[k] + 3) // 2
-            ip = gather(k * nq + idx, dims_p)
-            im = None
-            if ls[k] > 0:
-                dims_m = list(nc)
-                dims_m[k] = ls[k] * (ls[k] + 1) // 2
-                im = gather((4 + k) * nq + idx, dims_m)
-            comps = _cart(ls[k])
-            for d in range(3):
-                up = [_cart_index(*[c[t] + (t == d) for t in range(3)]) for c in comps]
-                val = np.take(ip, up, axis=k + 1)
-                if im is not None:
-                    fac = np.array([c[d] for c in comps], float)
-                    dn = [_cart_index(*[c[t] - (t == d) for t in range(3)]) if c[d] > 0 else 0 for c in comps]
-                    shape = [1] * 5
-                    shape[k + 1] = len(comps)
-                    val = val - fac.reshape(shape) * np.take(im, dn, axis=k + 1)
-                for kk in range(4):  # solid harmonics on the pure centres
-                    if pure[kk]:
-                        if ls[kk] not in sph:
-                            sph[ls[kk]] = _sph_matrix(ls[kk])
-                        val = np.moveaxis(np.tensordot(sph[ls[kk]], val, axes=([1], [kk + 1])), 0, kk + 1)
-                out[3 * k + d, dst] = val.reshape(m, osz)
-    return out[:, :total], out_off, nonzero
+    p = Deriv1Plan(Basis(ctx, arrays), quartets, precision, screening)
+    offs = p.offsets()
+    p.execute()
+    vals, nz = p.download()
+    return vals, offs, nz
 
 
 def fock_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, D, precision=np.finfo(float).eps,
-                screening=SCREEN_ORIGINAL):
+                screening=SCREEN_ORIGINAL, eri_deriv=None):
     """First derivatives of the two-electron Fock matrix, as the reference's
     compute_2body_fock_deriv<1> forms them (tests/hartree-fock/
     hartree-fock++.cc:1776-1990): over the permutationally unique quartets
@@ -162,7 +48,9 @@ def fock_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, D, precision=np.fi
 
     shell2atom[s] is the atom of shell s.  Returns G of shape
     (3 * natoms, nbf, nbf), coordinate 3 * atom + xyz; sum(G[x] * D) is the
-    two-electron part of the energy gradient, d tr(D G(D)) / dx at fixed D."""
+    two-electron part of the energy gradient, d tr(D G(D)) / dx at fixed D.
+    eri_deriv(arrays, quartets, precision, screening) -> (values, offsets,
+    nonzero) replaces the device derivative plan (a checker passes its own)."""
     ns = arrays.nshell
     sizes = arrays.sizes().astype(np.int64)
     bf = np.concatenate([[0], np.cumsum(sizes)])
@@ -171,7 +59,10 @@ def fock_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, D, precision=np.fi
     quartets = [(s1, s2, s3, s4) for s1 in range(ns) for s2 in range(s1 + 1) for s3 in range(s1 + 1)
                 for s4 in range((s2 if s1 == s3 else s3) + 1)]
     q = np.array(quartets, np.int32).reshape(-1, 4)
-    vals, off, nz = eri_deriv1(ctx, arrays, q, precision, screening)
+    if eri_deriv is None:
+        vals, off, nz = eri_deriv1(ctx, arrays, q, precision, screening)
+    else:
+        vals, off, nz = eri_deriv(arrays, q, precision, screening)
     atom = np.asarray(shell2atom, np.int64)
     G = np.zeros((3 * natoms, nbf, nbf))
     for i, (s1, s2, s3, s4) in enumerate(quartets):

@@ -44,6 +44,10 @@ int main(int argc, char** argv) {
   CHECK(throws<Engine::using_default_initialized>([] { Engine{}.set(Operator::coulomb); }));
   CHECK(throws<Engine::using_default_initialized>([] { Engine{}.set(BraKet::xx_xx); }));
   CHECK(!throws<std::exception>([] { Engine{}.prescale_by(1.5); }));
+  // derivative orders beyond the first, and derivatives of 3-centre or
+  // attenuated integrals, are refused before any GPU work
+  CHECK(throws<std::invalid_argument>([] { Engine(Operator::coulomb, 2, 2, 2); }));
+  CHECK(throws<std::invalid_argument>([] { Engine(Operator::erf_coulomb, 2, 2, 1, 1e-15, 0.5); }));
   if (cpu_only) {
     std::printf("cpu checks: %s\n", fails ? "FAIL" : "ok");
     return fails ? 1 : 0;
@@ -100,6 +104,19 @@ int main(int argc, char** argv) {
     const size_t n = basis[0].size() * basis[1].size() * basis[2].size() * basis[3].size();
     std::printf("R 0 1 2 3 %zu\n", n);
     for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", er.results()[0][i]);
+  }
+  // first derivatives (deriv_order = 1): 12 shell sets, printed as D
+  Engine ed(Operator::coulomb, 2, 2, 1);
+  CHECK(ed.nshellsets() == 12);
+  CHECK(throws<Engine::lmax_exceeded>([] { Engine(Operator::coulomb, 3, l2g_max_l(), 1); }));
+  ed.compute(basis[0], basis[1], basis[2], basis[3]);
+  {
+    const size_t n = basis[0].size() * basis[1].size() * basis[2].size() * basis[3].size();
+    std::printf("D 0 1 2 3 %zu\n", n);
+    for (int dset = 0; dset < 12; ++dset) {
+      CHECK(ed.results()[dset] != nullptr);
+      for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", ed.results()[dset][i]);
+    }
   }
   // python/tests/test_libint2.py:39-42: ||(s s|s)|| = 3.6563211198 for
   // s = Shell(0, [(1, 10)])

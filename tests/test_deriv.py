@@ -1,31 +1,16 @@
-"""First geometric derivatives (deriv_order = 1): the combination of raised
-and lowered ERIs in libint_amd/deriv.py, checked on the CPU with the oracle
-supplying the ERIs against Richardson-extrapolated central differences of
-the oracle's integrals in the shell centres (an independent route), and for
+"""First geometric derivatives (deriv_order = 1): the oracle's restatement
+(oracle/deriv1.py, raised and lowered ERIs combined per Cartesian component)
+against Richardson-extrapolated central differences of the oracle's
+integrals in the shell centres (an independent route), and for
 translational invariance (the 12 sets sum to zero per axis); the GPU run
-(test_gpu_eri.py) then checks the same quantities with the GPU's ERIs."""
+(test_gpu_eri.py) then checks the device derivative plan against it."""
 import numpy as np
 import pytest
 
 import libint_amd as la
 import oracle
-from libint_amd import deriv
+import oracle.deriv1
 from tests.helpers import random_shells
-
-
-class _OracleBasis:
-    def __init__(self, ctx, arrays):
-        self.arrays = arrays
-
-
-def _oracle_quartets(b, q, precision, screening):
-    return oracle.eri_quartets(b.arrays, q, precision=precision, screening=screening)
-
-
-def oracle_deriv1(monkeypatch, arrays, q):
-    monkeypatch.setattr(deriv, "Basis", _OracleBasis)
-    monkeypatch.setattr(deriv, "compute_quartets", _oracle_quartets)
-    return deriv.eri_deriv1(None, arrays, q)
 
 
 def fd_deriv1(arrays, quartet, h=2e-3):
@@ -60,8 +45,8 @@ def fd_deriv1(arrays, quartet, h=2e-3):
     return np.array(out)
 
 
-def check_against_fd(monkeypatch, arrays, quartets, tol):
-    g, offs, nz = oracle_deriv1(monkeypatch, arrays, quartets)
+def check_against_fd(arrays, quartets, tol):
+    g, offs, nz = oracle.deriv1.eri_deriv1(arrays, quartets)
     sizes = arrays.sizes()
     for i, qq in enumerate(quartets):
         n = int(np.prod(sizes[list(qq)]))
@@ -73,20 +58,20 @@ def check_against_fd(monkeypatch, arrays, quartets, tol):
             assert np.abs(mine[d] + mine[3 + d] + mine[6 + d] + mine[9 + d]).max() < 1e-12
 
 
-def test_deriv1_reference_case(monkeypatch):
+def test_deriv1_reference_case():
     """The two contracted Cartesian p shells of the reference's derivative
     test (tests/unit/test-2body.cc:191-196), every quartet over them."""
     p1 = la.Shell(1, [(1.0, 0.9), (0.3, 0.3)], (0.0, 0.0, 0.0), pure=False)
     p2 = la.Shell(1, [(2.0, 0.8), (0.4, -0.2)], (1.0, 1.0, 1.0), pure=False)
     arrays = la.BasisArrays.from_shells([p1, p2])
     q = [(a, b, c, d) for a in range(2) for b in range(2) for c in range(2) for d in range(2)]
-    check_against_fd(monkeypatch, arrays, q, 1e-8)
+    check_against_fd(arrays, q, 1e-8)
 
 
-def test_deriv1_mixed_pure(monkeypatch):
+def test_deriv1_mixed_pure():
     rng = np.random.default_rng(71)
     shells = random_shells(rng, [0, 1, 2, 2], kmax=2)
     shells[3].pure = False
     arrays = la.BasisArrays.from_shells(shells)
     q = [(2, 1, 3, 0), (0, 0, 1, 1), (3, 2, 2, 1), (1, 3, 0, 2)]
-    check_against_fd(monkeypatch, arrays, q, 1e-8)
+    check_against_fd(arrays, q, 1e-8)

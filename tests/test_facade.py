@@ -9,6 +9,7 @@ import pytest
 
 import libint_amd as la
 import oracle
+import oracle.deriv1
 from tests.helpers import compare
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -46,7 +47,17 @@ def test_facade_results_match_oracle():
     i = 0
     checked = 0
     while i < len(lines):
-        if lines[i].startswith("Q ") or lines[i].startswith("R "):
+        if lines[i].startswith("D "):
+            # first derivatives: 12 shell sets of n, against the oracle's restatement
+            _, a, b, c, d, n = lines[i].split()
+            n = int(n)
+            vals = np.array([float(x) for x in lines[i + 1:i + 1 + 12 * n]]).reshape(12, n)
+            o, _, _ = oracle.deriv1.eri_deriv1(arrays, [[int(a), int(b), int(c), int(d)]])
+            for k in range(12):
+                assert compare(vals[k], o[k], 1e-12, 1e-10)[0] == 0, k
+            checked += 1
+            i += 12 * n + 1
+        elif lines[i].startswith("Q ") or lines[i].startswith("R "):
             kind, a, b, c, d, n = lines[i].split()
             n = int(n)
             vals = np.array([float(x) for x in lines[i + 1:i + 1 + n]])
@@ -58,4 +69,4 @@ def test_facade_results_match_oracle():
             i += n + 1
         else:
             i += 1
-    assert checked == 8  # 5 quartets, one 3-centre, one 2-centre and one erf set
+    assert checked == 9  # 5 quartets, one 3-centre, one 2-centre, one erf set, one derivative set

@@ -12,6 +12,7 @@ import pytest
 
 import libint_amd as la
 import oracle
+import oracle.deriv1
 from libint_amd import molecule
 from tests.helpers import compare, random_shells
 
@@ -362,11 +363,12 @@ def test_range_separated(ctx, oper):
 
 def test_first_derivatives(ctx):
     """deriv_order = 1 (12 shell sets per quartet, libint's results()[0..11])
-    with the GPU's raised and lowered ERIs: against the same combination of
-    the oracle's ERIs at the integral bar (the combination itself is pinned
-    by finite differences in tests/test_deriv.py), translational invariance,
-    and the Engine facade on the reference's derivative-test shells
-    (tests/unit/test-2body.cc:191-196)."""
+    from the device derivative plan: its raised and lowered ERIs and its
+    combined shell sets against the oracle's (oracle/deriv1.py, itself
+    pinned by finite differences in tests/test_deriv.py) at the integral bar,
+    the Gaussian solid-harmonics ordering as a permutation of the Standard
+    one, translational invariance, and the Engine facade on the reference's
+    derivative-test shells (tests/unit/test-2body.cc:191-196)."""
     from libint_amd import deriv
     rng = np.random.default_rng(72)
     shells = random_shells(rng, [0, 1, 2, 1, 2], kmax=3)
@@ -377,14 +379,37 @@ def test_first_derivatives(ctx):
     q = allq[rng.choice(len(allq), 300, replace=False)]
     say(f"case: first derivatives, {len(q)} quartets")
     g, offs, nz = deriv.eri_deriv1(ctx, arrays, q)
-    # the same combination over the oracle's ERIs
-    aug = deriv._augmented(arrays)
+    # the raised and lowered ERIs against the oracle's at the integral bar
+    aug = oracle.deriv1.augmented(arrays)
     n_ = arrays.nshell
     big = np.concatenate([q + np.eye(4, dtype=np.int32)[k] * n_ for k in range(4)] +
                          [q + np.eye(4, dtype=np.int32)[k] * 2 * n_ for k in range(4)])
-    gv, _, _ = la.compute_quartets(la.Basis(ctx, aug), big)
+    gv, _, _ = la.compute_quartets(la.Basis(ctx, la.BasisArrays(aug.l, aug.pure, aug.nprim, aug.alpha, aug.coeff,
+                                                                aug.origin)), big)
     ov, _, _ = oracle.eri_quartets(aug, big)
     assert compare(gv, ov, ABS, REL)[0] == 0
+    # the device combination against the oracle's restatement, same bar
+    o, ooff, onz = oracle.deriv1.eri_deriv1(arrays, q)
+    assert np.array_equal(offs, ooff) and np.array_equal(nz, onz)
+    for d in range(12):
+        assert compare(g[d], o[d], ABS, REL)[0] == 0, d
+    # Gaussian ordering (0, +1, -1, +2, -2) of the pure centres
+    ctx.set_solid_harmonics_ordering(la.SHG_GAUSSIAN)
+    try:
+        gg, _, _ = deriv.eri_deriv1(ctx, arrays, q[:40])
+    finally:
+        ctx.set_solid_harmonics_ordering(la.SHG_STANDARD)
+    sizes = arrays.sizes()
+    for i in range(40):
+        dims = [int(sizes[s_]) for s_ in q[i]]
+        blk = g[:, offs[i]:offs[i] + int(np.prod(dims))].reshape([12] + dims)
+        for k in range(4):
+            sh = q[i, k]
+            if arrays.pure[sh] and arrays.l[sh] > 0:
+                l_ = int(arrays.l[sh])
+                std = [l_ + ((p_ + 1) // 2 if p_ % 2 else -(p_ // 2)) for p_ in range(2 * l_ + 1)]
+                blk = np.take(blk, std, axis=k + 1)
+        assert np.array_equal(gg[:, offs[i]:offs[i] + int(np.prod(dims))], blk.reshape(12, -1)), i
     for d in range(3):
         s_ = g[d] + g[3 + d] + g[6 + d] + g[9 + d]
         assert np.abs(s_).max() < 1e-11

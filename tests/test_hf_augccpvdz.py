@@ -118,12 +118,13 @@ def test_nuclear_repulsion_force():
     assert np.abs(nuclear_repulsion_gradient(atoms) - ref["values"]).max() < ref["tolerance"]
 
 
-def test_two_body_force_oracle(monkeypatch):
-    """The reference's "2-body forces" golden, with the oracle's ERIs in
-    deriv.fock_deriv1 (the same host combination and Fock-derivative digest
-    the GPU run uses), and by an independent route: Richardson-extrapolated
-    central differences of tr(D G(D)) in the atom positions at fixed D."""
+def test_two_body_force_oracle():
+    """The reference's "2-body forces" golden with the oracle's derivative
+    ERIs in deriv.fock_deriv1 (the Fock-derivative digest the GPU run uses),
+    and by an independent route: Richardson-extrapolated central differences
+    of tr(D G(D)) in the atom positions at fixed D."""
     import oracle
+    import oracle.deriv1
     from libint_amd import deriv
     atoms, shells, enuc, eref, tol = water()
     S, H = one_body_pure(atoms, shells)
@@ -136,15 +137,8 @@ def test_two_body_force_oracle(monkeypatch):
     e, D = rhf(S, H, 5, g, enuc, diis=8, density=True)
     assert abs(e - eref) < tol
     ref = water.forces["F2"]
-
-    class OracleBasis:
-        def __init__(self, ctx, arr):
-            self.arrays = arr
-
-    monkeypatch.setattr(deriv, "Basis", OracleBasis)
-    monkeypatch.setattr(deriv, "compute_quartets",
-                        lambda b, q, p, s: oracle.eri_quartets(b.arrays, q, precision=p, screening=s))
-    G1 = deriv.fock_deriv1(None, arrays, water.shell2atom, len(atoms), D, 1e-15)
+    G1 = deriv.fock_deriv1(None, arrays, water.shell2atom, len(atoms), D, 1e-15,
+                           eri_deriv=lambda a, q, p, s: oracle.deriv1.eri_deriv1(a, q, p, s))
     F2 = np.array([np.sum(G1[x] * D) for x in range(3 * len(atoms))])
     assert np.abs(F2 - ref["values"]).max() < ref["tolerance"], F2 - ref["values"]
 
