@@ -82,8 +82,10 @@ const BoysTable& boys_table() {
 
 struct ClassWork {
   int cls;  // index into l2g_gen_classes (l2g_gen_classes32 when fp32, l2g_gen_generic when generic)
+  int spec_cls = -1;            // the class's own table index (cls before the layout's choice)
   bool fp32 = false;
   bool generic = false;
+  bool native_generic = false;  // no register kernel for this class
   bool coop = false;  // a wave per quartet (few, deeply contracted quartets)
   int gidx = -1;      // the class's program in the generic kernel's tables
   int64_t first, count;
@@ -492,8 +494,10 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       });
       ClassWork cw;
       cw.cls = c;
+      cw.spec_cls = c;
       cw.fp32 = is32;
       cw.generic = isgen;
+      cw.native_generic = isgen;
       {
         const L2gClassInfo* ci = isgen ? nullptr : &(is32 ? l2g_gen_classes32 : l2g_gen_classes)[c];
         cw.gidx = isgen ? c : ctx->generic_of[ci->la][ci->lb][ci->lc][ci->ld];
@@ -540,19 +544,51 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
 namespace {
 
 // Launch layout of a plan: a persistent grid with enough blocks to fill
-// every CU several times over, and the lane-private scratch for it.  With
-// wave_split, a class of few quartets (fewer than a quarter of the chip's
-// lanes at one wave per SIMD) with deep primitive loops runs a wave per
-// quartet whose lanes split the loop; one lane per quartet would leave most
-// of the chip idle while a few lanes run their loops serially.  The split
-// sums each contraction in another order than libint's serial loop.
+// every CU several times over, and the lane-private scratch for it.  A class
+// of few quartets (fewer than a quarter of the chip's lanes at one wave per
+// SIMD) with deep primitive loops would, at one lane per quartet, leave most
+// of the chip idle while a few lanes run their loops serially.  With
+// wave_split such a class runs a wave per quartet whose lanes split the loop
+// (summing each contraction in another order than libint's serial loop);
+// without it, a class of total angular momentum 7 or more averaging at least
+// 64 primitive quartets per quartet runs on the generic kernel, a block per
+// quartet whose threads split the targets and keep each contraction's
+// serial order.  Lower classes have too few targets per quartet to occupy
+// a block and stay at a lane per quartet (on 400k quartets of an s-f basis
+// with K up to 8, the cut at L >= 7 measured 180 ms, at 5, 6, 8 339, 276
+// and 207 ms, and 1556 ms without the generic kernel).
+#ifndef L2G_GENERIC_MIN_DEPTH
+#define L2G_GENERIC_MIN_DEPTH 64
+#endif
+#ifndef L2G_GENERIC_MIN_L
+#define L2G_GENERIC_MIN_L 7
+#endif
+
+// total angular momentum of a class with a register kernel
+int class_l(const ClassWork& c) {
+  if (c.native_generic) return 99;
+  const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.spec_cls];
+  return ci.la + ci.lb + ci.lc + ci.ld;
+}
+
 void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
   p->block = 128;
   std::vector<int64_t> lanes, yd;
   for (auto& c : p->classes) {
-    c.coop = p->wave_split && !c.generic && c.count * 4 <= (int64_t)ctx->ncu * 256 && c.nprimq >= 16 * c.count;
-    // the generic kernel runs a block per quartet and no lane scratch
-    lanes.push_back(c.generic ? c.count * 256 : (c.coop ? 64 * c.count : c.count));
+    c.generic = c.native_generic;
+    c.cls = c.spec_cls;
+    const bool few = c.count * 4 <= (int64_t)ctx->ncu * 256;
+    c.coop = p->wave_split && !c.generic && few && c.nprimq >= 16 * c.count;
+    if (!c.generic && !c.coop && !c.fp32 && c.gidx >= 0 && few && c.nprimq >= L2G_GENERIC_MIN_DEPTH * c.count &&
+        class_l(c) >= L2G_GENERIC_MIN_L) {
+      c.generic = true;
+      c.cls = c.gidx;
+    }
+    // the generic kernel runs a block per quartet, no lane scratch, a few
+    // blocks per CU (LDS bound): up to 4 quartets per CU leave the chip
+    // underfilled, so such classes count as small and run beside each other
+    // on the side streams
+    lanes.push_back(c.generic ? c.count * 64 : (c.coop ? 64 * c.count : c.count));
     yd.push_back(c.generic ? 0 : (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].ybuf_doubles);
   }
   std::vector<ClassSlot> slots;
