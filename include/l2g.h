@@ -211,6 +211,19 @@ int l2g_eri_plan_elapsed(l2g_eri_plan plan, float* ms);
 
 int l2g_eri_plan_destroy(l2g_eri_plan plan);
 
+/* ---- one-body integrals ------------------------------------------------ */
+
+/* Overlap S, kinetic energy T and point-charge attraction
+ * V = -sum_C q_C <a|1/|r - C||b> over every pair of shells of the basis
+ * (charges[c] at xyz[3c..3c+2], bohr), as Engine(Operator::overlap /
+ * kinetic / nuclear) computes them per shell pair (engine.impl.h:181-561,
+ * charges as Engine::set_params for nuclear, engine.h:685-690) and the
+ * reference's driver assembles them (tests/hartree-fock/hartree-fock.cc:
+ * 480-535).  Each of S, T, V (nbf x nbf, row-major, the basis's function
+ * order; may be NULL) is written whole.  Shells up to l = 3. */
+int l2g_onebody(l2g_basis basis, int ncharges, const double* charges, const double* xyz, double* S, double* T,
+                double* V);
+
 /* ---- first geometric derivatives (deriv_order = 1) --------------------- */
 
 /* Plans the first derivatives of a batch of quartets, replacing

@@ -69,6 +69,7 @@ ABI = {
     "l2g_eri_plan_set_operator": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "l2g_eri_plan_elapsed": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "l2g_eri_plan_destroy": (ctypes.c_int, [_vp]),
+    "l2g_onebody": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "l2g_eri_deriv1_plan_create": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int,
                                                   ctypes.POINTER(_vp)]),
     "l2g_eri_deriv1_plan_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
@@ -416,6 +417,22 @@ class EriPlan:
             self._release()
         except Exception:
             pass
+
+
+def one_body(basis: Basis, charges):
+    """Overlap, kinetic energy and point-charge attraction matrices (S, T, V)
+    over the basis's functions; charges: (Z, (x, y, z)) pairs in bohr, as
+    the reference's compute_1body_ints gives the nuclei to
+    Engine(Operator::nuclear) (tests/hartree-fock/hartree-fock.cc:480-535)."""
+    ch = list(charges)
+    q = np.ascontiguousarray([float(z) for z, _ in ch], np.float64)
+    xyz = np.ascontiguousarray([np.asarray(x, np.float64).reshape(3) for _, x in ch], np.float64).reshape(-1)
+    n = int(basis.arrays.sizes().sum()) if basis.arrays.nshell else 0
+    S, T, V = (np.zeros((max(n, 1), max(n, 1))) for _ in range(3))
+    _check(lib().l2g_onebody(basis.handle, len(ch), q.ctypes.data if len(ch) else None,
+                             xyz.ctypes.data if len(ch) else None, S.ctypes.data, T.ctypes.data,
+                             V.ctypes.data if len(ch) else None))
+    return S[:n, :n], T[:n, :n], V[:n, :n]
 
 
 class Deriv1Plan:

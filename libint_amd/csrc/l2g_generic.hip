@@ -378,9 +378,176 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// One-body integrals: overlap, kinetic energy and point-charge attraction
+// over contracted Cartesian shells, then the solid-harmonic transform of
+// pure shells -- what Engine(Operator::overlap / kinetic / nuclear) computes
+// per shell pair (engine.impl.h:181-561) and the reference's driver loops
+// over (tests/hartree-fock/hartree-fock.cc:480-535).  McMurchie-Davidson
+// (Helgaker, Jorgensen, Olsen, "Molecular Electronic-Structure Theory",
+// 9.5, 9.7, 9.9): Hermite coefficients E^{ij}_t of the Gaussian product per
+// axis (j to lb + 2 for the kinetic operator -1/2 nabla^2 on the ket),
+// Hermite Coulomb integrals R_{tuv} per charge.  A 64-thread block per shell
+// pair (blockIdx.x = bra, blockIdx.y = ket); a thread owns up to two of the
+// at most 100 Cartesian pairs; the E and R tables sit in LDS, built by one
+// thread per axis / charge.  Coefficients are used as the basis holds them.
+__global__ void __launch_bounds__(64) l2g_onebody_kernel(l2g_impl::L2gOneBodyArgs A) {
+  const int sa = blockIdx.x, sb = blockIdx.y;
+  if (sa >= A.nshell || sb >= A.nshell) return;
+  const int la = A.l[sa], lb = A.l[sb];
+  const int nca = ncart_d(la), ncb = ncart_d(lb), npair = nca * ncb;
+  const int tid = threadIdx.x;
+  __shared__ double E[3][4][6][10];   // [axis][i <= 3][j <= 5][t <= 9]
+  __shared__ double Rw[7][7][7][7];   // [n][t][u][v], t + u + v + n <= la + lb <= 6
+  __shared__ double cart[3][100];     // contracted Cartesian S, T, V
+  int ex[2][3], ey[2][3];  // exponents of the owned bra / ket components
+  bool own[2];
+  for (int k = 0; k < 2; ++k) {
+    const int r = tid + 64 * k;
+    own[k] = r < npair;
+    const int ca = own[k] ? r / ncb : 0, cb = own[k] ? r % ncb : 0;
+    for (int side = 0; side < 2; ++side) {
+      const int l = side ? lb : la, c = side ? cb : ca;
+      int rr = 0;
+      while ((rr + 1) * (rr + 2) / 2 <= c) ++rr;
+      int* e = side ? ey[k] : ex[k];
+      e[2] = c - rr * (rr + 1) / 2;
+      e[1] = rr - e[2];
+      e[0] = l - rr;
+    }
+  }
+  double accS[2] = {0.0, 0.0}, accT[2] = {0.0, 0.0}, accV[2] = {0.0, 0.0};
+  const double* Ao = A.origin + 3 * sa;
+  const double* Bo = A.origin + 3 * sb;
+  const int L = la + lb;
+  for (int ia = A.poff[sa]; ia < A.poff[sa + 1]; ++ia)
+    for (int ib = A.poff[sb]; ib < A.poff[sb + 1]; ++ib) {
+      const double a = A.alpha[ia], b = A.alpha[ib], p = a + b, cc = A.coeff[ia] * A.coeff[ib];
+      double P[3];
+      for (int k = 0; k < 3; ++k) P[k] = (a * Ao[k] + b * Bo[k]) / p;
+      if (tid < 3) {
+        const int k = tid;
+        const double XAB = Ao[k] - Bo[k], XPA = -b / p * XAB, XPB = a / p * XAB, o2p = 0.5 / p;
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 6; ++j)
+            for (int t = 0; t < 10; ++t) E[k][i][j][t] = 0.0;
+        E[k][0][0][0] = exp(-a * b / p * XAB * XAB);
+        for (int i = 0; i <= la; ++i)
+          for (int j = 0; j <= lb + 2; ++j) {
+            if (i == 0 && j == 0) continue;
+            for (int t = 0; t <= i + j; ++t) {
+              double v;
+              if (i > 0)
+                v = XPA * E[k][i - 1][j][t] + (t > 0 ? o2p * E[k][i - 1][j][t - 1] : 0.0) +
+                    (t + 1) * E[k][i - 1][j][t + 1];
+              else
+                v = XPB * E[k][i][j - 1][t] + (t > 0 ? o2p * E[k][i][j - 1][t - 1] : 0.0) +
+                    (t + 1) * E[k][i][j - 1][t + 1];
+              E[k][i][j][t] = v;
+            }
+          }
+      }
+      __syncthreads();
+      const double s1 = sqrt(3.14159265358979323846 / p);
+      for (int k = 0; k < 2; ++k) {
+        if (!own[k]) continue;
+        double ov[3][7];  // ov[axis][j'] for j' = j - 2 .. j + 2 at index j' - j + 2
+        for (int x = 0; x < 3; ++x)
+          for (int d = -2; d <= 2; ++d) {
+            const int j = ey[k][x] + d;
+            ov[x][d + 2] = j >= 0 ? E[x][ex[k][x]][j][0] * s1 : 0.0;
+          }
+        accS[k] += cc * ov[0][2] * ov[1][2] * ov[2][2];
+        double t = 0.0;
+        for (int x = 0; x < 3; ++x) {
+          const int j = ey[k][x];
+          double tk = b * (2 * j + 1) * ov[x][2] - 2.0 * b * b * ov[x][4];
+          if (j >= 2) tk -= 0.5 * j * (j - 1) * ov[x][0];
+          t += tk * ov[(x + 1) % 3][2] * ov[(x + 2) % 3][2];
+        }
+        accT[k] += cc * t;
+      }
+      for (int c = 0; c < A.ncharge; ++c) {
+        if (tid == 0) {
+          const double PC[3] = {P[0] - A.qxyz[3 * c], P[1] - A.qxyz[3 * c + 1], P[2] - A.qxyz[3 * c + 2]};
+          const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+          double f = 1.0;
+          for (int n = 0; n <= L; ++n) {
+            Rw[n][0][0][0] = f * boys_one(T, n, A.boys);
+            f *= -2.0 * p;
+          }
+          for (int n = L - 1; n >= 0; --n)
+            for (int t = 0; t <= L - n; ++t)
+              for (int u = 0; u <= L - n - t; ++u)
+                for (int v = 0; v <= L - n - t - u; ++v) {
+                  if (t + u + v == 0) continue;
+                  double val;
+                  if (t > 0)
+                    val = PC[0] * Rw[n + 1][t - 1][u][v] + (t > 1 ? (t - 1) * Rw[n + 1][t - 2][u][v] : 0.0);
+                  else if (u > 0)
+                    val = PC[1] * Rw[n + 1][t][u - 1][v] + (u > 1 ? (u - 1) * Rw[n + 1][t][u - 2][v] : 0.0);
+                  else
+                    val = PC[2] * Rw[n + 1][t][u][v - 1] + (v > 1 ? (v - 1) * Rw[n + 1][t][u][v - 2] : 0.0);
+                  Rw[n][t][u][v] = val;
+                }
+        }
+        __syncthreads();
+        const double zf = A.q[c] * 2.0 * 3.14159265358979323846 / p;
+        for (int k = 0; k < 2; ++k) {
+          if (!own[k]) continue;
+          double acc = 0.0;
+          for (int t = 0; t <= ex[k][0] + ey[k][0]; ++t)
+            for (int u = 0; u <= ex[k][1] + ey[k][1]; ++u)
+              for (int v = 0; v <= ex[k][2] + ey[k][2]; ++v)
+                acc += E[0][ex[k][0]][ey[k][0]][t] * E[1][ex[k][1]][ey[k][1]][u] * E[2][ex[k][2]][ey[k][2]][v] *
+                       Rw[0][t][u][v];
+          accV[k] -= cc * zf * acc;
+        }
+        __syncthreads();
+      }
+      __syncthreads();
+    }
+  for (int k = 0; k < 2; ++k)
+    if (own[k]) {
+      cart[0][tid + 64 * k] = accS[k];
+      cart[1][tid + 64 * k] = accT[k];
+      cart[2][tid + 64 * k] = accV[k];
+    }
+  __syncthreads();
+  const int pa = (A.pure[sa] && la > 0) ? 1 : 0, pb = (A.pure[sb] && lb > 0) ? 1 : 0;
+  const int noa = pa ? 2 * la + 1 : nca, nob = pb ? 2 * lb + 1 : ncb;
+  double* out[3] = {A.S, A.T, A.V};
+  for (int r = tid; r < noa * nob; r += 64) {
+    const int fi = r / nob, fj = r % nob;
+    // output position -> row of c_sh (m + l): Standard m = f - l, Gaussian 0, +1, -1, +2, ...
+    const int ra = pa ? (A.gauss ? la + (fi == 0 ? 0 : ((fi & 1) ? (fi + 1) / 2 : -fi / 2)) : fi) : 0;
+    const int rb = pb ? (A.gauss ? lb + (fj == 0 ? 0 : ((fj & 1) ? (fj + 1) / 2 : -fj / 2)) : fj) : 0;
+    const long long o = (A.bf0[sa] + fi) * A.nbf + A.bf0[sb] + fj;
+    for (int m = 0; m < 3; ++m) {
+      if (!out[m]) continue;
+      double s = 0.0;
+      for (int ca = pa ? 0 : fi; ca < (pa ? nca : fi + 1); ++ca) {
+        const double wa = pa ? c_sh[la][ra][ca] : 1.0;
+        if (wa == 0.0) continue;
+        for (int cb = pb ? 0 : fj; cb < (pb ? ncb : fj + 1); ++cb) {
+          const double wb = pb ? c_sh[lb][rb][cb] : 1.0;
+          if (wb != 0.0) s += wa * wb * cart[m][ca * ncb + cb];
+        }
+      }
+      out[m][o] = s;
+    }
+  }
+}
+
 }  // namespace
 
 namespace l2g_impl {
+
+void launch_onebody(const L2gOneBodyArgs& A, hipStream_t s) {
+  if (A.nshell <= 0) return;
+  hipLaunchKernelGGL(l2g_onebody_kernel, dim3(A.nshell, A.nshell), dim3(64), 0, s, A);
+  LAUNCHCHK("l2g_onebody_kernel");
+}
 
 template <int MODE>
 void set_lds_limit(int maxlds) {

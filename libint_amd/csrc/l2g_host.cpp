@@ -1232,3 +1232,66 @@ int l2g_eri_deriv1_plan_destroy(l2g_deriv1_plan p) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// One-body integrals
+// ===========================================================================
+extern "C" {
+
+int l2g_onebody(l2g_basis B, int ncharge, const double* charges, const double* xyz, double* S, double* T, double* V) {
+  return guarded([&] {
+    if (!B) throw L2gError(L2G_E_STATE, "null basis");
+    if (ncharge < 0 || (ncharge > 0 && (!charges || !xyz))) throw std::invalid_argument("l2g_onebody: bad arguments");
+    if (V && ncharge == 0) throw std::invalid_argument("l2g_onebody: nuclear attraction needs at least one charge");
+    l2g_context_s* ctx = B->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = B->nshell;
+    std::vector<long long> bf0(n + 1, 0);
+    for (int s = 0; s < n; ++s) {
+      if (B->l[s] > 3) throw L2gError(L2G_E_LMAX, "one-body integrals are provided up to l = 3");
+      bf0[s + 1] = bf0[s] + nfun(B->l[s], (B->pure[s] && B->l[s] > 0) ? 1 : 0);
+    }
+    const long long nbf = bf0[n];
+    if (n == 0) return;
+    hipStream_t st = ctx->stream;
+    DevBuf dl, dp, doff, da, dc, dO, dbf, dq, dqx, dS, dT, dV;
+    upload(dl, B->l, st);
+    upload(dp, B->pure, st);
+    upload(doff, B->off, st);
+    upload(da, B->alpha, st);
+    upload(dc, B->coeff, st);
+    upload(dO, B->origin, st);
+    upload(dbf, bf0, st);
+    std::vector<double> q(charges, charges + ncharge), qx(xyz, xyz + 3 * ncharge);
+    upload(dq, q, st);
+    upload(dqx, qx, st);
+    const size_t bytes = (size_t)nbf * nbf * sizeof(double);
+    L2gOneBodyArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.l = dl.as<int>();
+    A.pure = dp.as<int>();
+    A.poff = doff.as<int>();
+    A.alpha = da.as<double>();
+    A.coeff = dc.as<double>();
+    A.origin = dO.as<double>();
+    A.bf0 = dbf.as<long long>();
+    A.nshell = n;
+    A.nbf = nbf;
+    A.ncharge = ncharge;
+    A.q = dq.as<double>();
+    A.qxyz = dqx.as<double>();
+    A.boys = ctx->boys.as<double>();
+    A.gauss = ctx->shg_gaussian;
+    if (S) dS.alloc(bytes), A.S = dS.as<double>();
+    if (T) dT.alloc(bytes), A.T = dT.as<double>();
+    if (V) dV.alloc(bytes), A.V = dV.as<double>();
+    launch_onebody(A, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("kernel execution failed: ") + hipGetErrorString(e));
+    if (S) HIPCHK(hipMemcpy(S, dS.p, bytes, hipMemcpyDeviceToHost));
+    if (T) HIPCHK(hipMemcpy(T, dT.p, bytes, hipMemcpyDeviceToHost));
+    if (V) HIPCHK(hipMemcpy(V, dV.p, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

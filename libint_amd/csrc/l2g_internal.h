@@ -330,6 +330,31 @@ struct L2gDerivTab {
   double coef[4][7][4];
 };
 
+// l2g_generic.hip: one-body integrals over every shell pair of a basis.
+// Per-shell arrays (poff = first primitive, size nshell + 1; bf0 = first
+// function), point charges q at qxyz; S, T, V (any may be null) are nbf x
+// nbf row-major on the device.
+struct L2gOneBodyArgs {
+  const int* l;
+  const int* pure;
+  const int* poff;
+  const double* alpha;
+  const double* coeff;
+  const double* origin;
+  const long long* bf0;
+  int nshell;
+  long long nbf;
+  int ncharge;
+  const double* q;
+  const double* qxyz;
+  const double* boys;
+  int gauss;
+  double* S;
+  double* T;
+  double* V;
+};
+void launch_onebody(const L2gOneBodyArgs& A, hipStream_t s);
+
 // l2g_aux.hip: first-derivative shell sets from the raised and lowered ERIs
 // of a derivative plan (one block per quartet); raw_off[k * nq + q] and
 // raw_off[(4 + k) * nq + q] locate quartet q's ERIs with centre k raised and

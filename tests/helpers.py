@@ -110,3 +110,27 @@ def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0, density=False
     F = H + fock2e(D)
     e = float(np.sum(D * (H + F))) + enuc
     return (e, D) if density else e
+
+
+def cart_to_shells(shells):
+    """Block-diagonal matrix from Cartesian functions (libint's order) to the
+    shells' own functions: identity for Cartesian shells, the solid-harmonic
+    coefficients (rows m = -l..l, Standard order) for pure ones.  X^T M X
+    takes a Cartesian one-body matrix M to the shells' functions."""
+    import libint_amd as la
+    from oracle import onebody
+    blocks = []
+    for s in shells:
+        nc = (s.l + 1) * (s.l + 2) // 2
+        if s.pure and s.l > 0:
+            blocks.append(np.array([[la.solid_harmonics_coeff(s.l, m, *c) for c in onebody.cart(s.l)]
+                                    for m in range(-s.l, s.l + 1)]).T)
+        else:
+            blocks.append(np.eye(nc))
+    X = np.zeros((sum(b.shape[0] for b in blocks), sum(b.shape[1] for b in blocks)))
+    r = c = 0
+    for b in blocks:
+        X[r:r + b.shape[0], c:c + b.shape[1]] = b
+        r += b.shape[0]
+        c += b.shape[1]
+    return X

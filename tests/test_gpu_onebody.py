@@ -1,0 +1,84 @@
+"""One-body integrals on the GPU (l2g_onebody): overlap, kinetic energy and
+point-charge attraction against the oracle's McMurchie-Davidson restatement
+(oracle/onebody.py) at the integral bar, on random contracted shells to l = 3
+with mixed purity; the reference's python goldens
+(python/tests/test_libint2.py:32-36); the Gaussian solid-harmonics ordering
+as a permutation of the Standard one."""
+import sys
+
+import numpy as np
+import pytest
+
+import libint_amd as la
+from oracle import onebody
+from tests.helpers import cart_to_shells, compare, random_shells
+
+pytestmark = pytest.mark.gpu
+
+ABS, REL = 1e-12, 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_one_body(shells, charges):
+    S, T, V = onebody.one_body([(s.l, s.alpha, s.coeff, s.O) for s in shells], charges)
+    X = cart_to_shells(shells)
+    return X.T @ S @ X, X.T @ T @ X, X.T @ V @ X
+
+
+def test_one_body_vs_oracle(ctx):
+    rng = np.random.default_rng(91)
+    shells = random_shells(rng, [0, 1, 2, 3, 1, 2, 3, 0], kmax=3)
+    shells[5].pure = False
+    shells[6].pure = False
+    charges = [(8.0, rng.normal(0, 1.2, 3)), (1.0, rng.normal(0, 1.2, 3)), (2.5, rng.normal(0, 1.2, 3))]
+    say(f"case: one-body S, T, V, {len(shells)} shells to l = 3, 3 charges")
+    S, T, V = la.one_body(la.Basis(ctx, la.BasisArrays.from_shells(shells)), charges)
+    So, To, Vo = oracle_one_body(shells, charges)
+    for name, g, o in (("S", S, So), ("T", T, To), ("V", V, Vo)):
+        nf = compare(g.reshape(-1), o.reshape(-1), ABS, REL)[0]
+        say(f"  {name}: max abs {np.abs(g - o).max():.2e}, fails {nf}")
+        assert nf == 0, name
+    assert np.allclose(S, S.T, atol=1e-14) and np.allclose(V, V.T, atol=1e-13)
+
+
+def test_one_body_goldens(ctx):
+    """python/tests/test_libint2.py:32-36 (positions as given, in bohr)."""
+    s = la.Shell(0, [(1.0, 10.0)])
+    p = la.Shell(1, [(1.0, 10.0)])
+    h2o = [(8.0, (0.0, -0.07579, 0.0)), (1.0, (0.86681, 0.60144, 0.0)), (1.0, (-0.86681, 0.60144, 0.0))]
+    say("case: one-body python goldens")
+    S, T, V = la.one_body(la.Basis(ctx, la.BasisArrays.from_shells([s])), h2o)
+    assert abs(np.linalg.norm(T) - 1.5) < 1e-12
+    assert abs(np.linalg.norm(S) - 1.0) < 1e-12
+    assert abs(np.linalg.norm(V) - 14.54704336519) < 1e-10
+    S4, _, _ = la.one_body(la.Basis(ctx, la.BasisArrays.from_shells([s, p, s, p])), [])
+    assert abs(S4.sum() - 16.0) < 1e-12
+
+
+def test_one_body_gaussian_ordering(ctx):
+    rng = np.random.default_rng(92)
+    shells = random_shells(rng, [2, 3, 1], kmax=2)
+    b = la.Basis(ctx, la.BasisArrays.from_shells(shells))
+    charges = [(3.0, (0.1, 0.2, 0.3))]
+    say("case: one-body Gaussian ordering")
+    S, T, V = la.one_body(b, charges)
+    ctx.set_solid_harmonics_ordering(la.SHG_GAUSSIAN)
+    try:
+        Sg, Tg, Vg = la.one_body(b, charges)
+    finally:
+        ctx.set_solid_harmonics_ordering(la.SHG_STANDARD)
+    perm = []
+    off = 0
+    for s in shells:
+        n = 2 * s.l + 1 if (s.pure and s.l > 0) else (s.l + 1) * (s.l + 2) // 2
+        if s.pure and s.l > 0:
+            perm += [off + s.l + ((q + 1) // 2 if q % 2 else -(q // 2)) for q in range(n)]
+        else:
+            perm += list(range(off, off + n))
+        off += n
+    for a, g in ((S, Sg), (T, Tg), (V, Vg)):
+        assert np.array_equal(a[np.ix_(perm, perm)], g)

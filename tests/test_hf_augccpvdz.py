@@ -5,7 +5,8 @@ pure d shells): the energy must equal the reference's pinned
 (tests/hartree-fock/hartree-fock++-validate.py:47-48), with G = 2J - K from
 the GPU Fock build.  Basis and geometry: tests/golden/h2o_augccpvdz.json
 (made by tests/golden/make_h2o_augccpvdz.py from the reference's data files);
-one-body integrals from the oracle's restatement."""
+one-body integrals from l2g_onebody on the GPU (from the oracle's
+restatement in the CPU tests)."""
 import json
 import os
 
@@ -63,8 +64,10 @@ def one_body_pure(atoms, shells):
 @pytest.mark.gpu
 def test_rhf_energy_gpu_augccpvdz(ctx):
     atoms, shells, enuc, eref, tol = water()
-    S, H = one_body_pure(atoms, shells)
-    fb = la.FockBuilder(la.Basis(ctx, la.BasisArrays.from_shells(shells)))
+    b = la.Basis(ctx, la.BasisArrays.from_shells(shells))
+    S, T, V = la.one_body(b, [(float(Z), pos) for Z, pos in atoms])  # pure d transformed on the GPU
+    H = T + V
+    fb = la.FockBuilder(b)
 
     def g(D):
         G, _, _ = fb.build(D, 1e-15)

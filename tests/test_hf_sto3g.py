@@ -2,10 +2,11 @@
 (tests/hartree-fock/hartree-fock.cc with h2o.xyz and its built-in STO-3G,
 :338-420): the energy must equal the reference's pinned
 -74.942080057696 within its own tolerance 1e-11
-(tests/hartree-fock/hartree-fock-validate.py:13-14).  One-body integrals come
-from the oracle's restatement, pinned here by the reference's python goldens
-(python/tests/test_libint2.py:32-36); the two-electron G = 2J - K from the
-oracle (CPU) and from the GPU Fock build (GPU test)."""
+(tests/hartree-fock/hartree-fock-validate.py:13-14).  On the CPU every
+integral comes from the oracle (one-body restatement pinned here by the
+reference's python goldens, python/tests/test_libint2.py:32-36); the GPU test
+takes the one-body matrices from l2g_onebody and G = 2J - K from the GPU
+Fock build."""
 import numpy as np
 import pytest
 
@@ -70,9 +71,13 @@ def test_rhf_energy_oracle():
 
 @pytest.mark.gpu
 def test_rhf_energy_gpu(ctx):
+    """Every integral on the GPU: one-body matrices from l2g_onebody, G from
+    the Fock build."""
     atoms, shells, enuc = water()
-    S, H = one_body_matrices(atoms, shells)
-    fb = la.FockBuilder(la.Basis(ctx, la.BasisArrays.from_shells(shells)))
+    b = la.Basis(ctx, la.BasisArrays.from_shells(shells))
+    S, T, V = la.one_body(b, [(float(Z), pos) for Z, pos in atoms])
+    H = T + V
+    fb = la.FockBuilder(b)
 
     def g(D):
         G, _, _ = fb.build(D, 1e-15)
