@@ -209,6 +209,82 @@ def eri_f_leg(la, molecule, flop_model, ctx, nquartets, reps=3, seed=21):
     return out
 
 
+def next_rows_leg(la, mol, B, q, ctx, nq=400000, nderiv=50000, seed=7):
+    """Throughput of the paths beside the headline one, on the same 60-atom
+    molecule: erf/erfc-attenuated Coulomb (omega 0.4) on the first nq
+    headline quartets next to plain Coulomb on them; first derivatives of
+    nderiv of them; 3-centre (X|ab) for the headline's d pairs; the one-body
+    matrices with the 60 nuclei as charges.  GPU times only (ERI plans by
+    their events, the rest by wall clock around a synchronised call)."""
+    out = {}
+    qs = q[:nq]
+    for name, oper in (("coulomb", la.COULOMB), ("erf_coulomb", la.ERF_COULOMB), ("erfc_coulomb", la.ERFC_COULOMB)):
+        plan = la.EriPlan(B, qs)
+        if oper != la.COULOMB:
+            plan.set_operator(oper, 0.4)
+        d = plan.device_output()
+        plan.execute(d)
+        ctx.synchronize()
+        ms = []
+        for _ in range(3):
+            plan.execute(d)
+            ctx.synchronize()
+            ms.append(plan.elapsed_ms())
+        t = float(np.median(ms))
+        out[name] = {"quartets": int(len(qs)), "ms": t, "quartets_per_s": len(qs) / (t * 1e-3)}
+        log(f"{name} (dd|dd) x{len(qs)}: {t:.1f} ms, {len(qs) / (t * 1e-3):.3e} quartets/s")
+        del plan
+    qd = q[:nderiv]
+    dp = la.Deriv1Plan(B, qd)
+    dp.execute()
+    ctx.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        dp.execute()
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out["deriv1"] = {"quartets": int(len(qd)), "ms": t * 1e3, "quartets_per_s": len(qd) / t,
+                     "derivative_integrals_per_s": 12 * dp.size / t}
+    log(f"first derivatives (dd|dd) x{len(qd)}: {t * 1e3:.1f} ms, {len(qd) / t:.3e} quartets/s "
+        f"({12 * dp.size / t:.3e} derivative integrals/s)")
+    del dp
+    # 3-centre (X|ab): X over the shells of the first 20 atoms, ab the headline's d pairs
+    a = mol.basis
+    n = a.nshell
+    unit = la.BasisArrays(np.append(a.l, 0), np.append(a.pure, 0), np.append(a.nprim, 1), np.append(a.alpha, 0.0),
+                          np.append(a.coeff, 1.0), np.append(a.origin, [0.0, 0.0, 0.0]))
+    bu = la.Basis(ctx, unit)
+    X = np.nonzero(mol.atom_of_shell < 20)[0]
+    pairs = np.unique(np.sort(q[:, :2], axis=1)[:, ::-1], axis=0)[:2000]
+    q3 = np.stack([np.repeat(X, len(pairs)), np.full(len(X) * len(pairs), n),
+                   np.tile(pairs[:, 0], len(X)), np.tile(pairs[:, 1], len(X))], 1).astype(np.int32)
+    plan = la.EriPlan(bu, q3)
+    d = plan.device_output()
+    plan.execute(d)
+    ctx.synchronize()
+    ms = []
+    for _ in range(3):
+        plan.execute(d)
+        ctx.synchronize()
+        ms.append(plan.elapsed_ms())
+    t = float(np.median(ms))
+    out["three_centre"] = {"triples": int(len(q3)), "ms": t, "triples_per_s": len(q3) / (t * 1e-3)}
+    log(f"3-centre (X|ab) x{len(q3)}: {t:.1f} ms, {len(q3) / (t * 1e-3):.3e} triples/s")
+    del plan, bu
+    charges = [(8.0, c) for c in mol.coords]
+    la.one_body(B, charges)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        la.one_body(B, charges)
+        ts.append(time.perf_counter() - t0)
+    out["one_body"] = {"nbf": mol.nbf, "charges": len(charges), "ms": float(np.median(ts)) * 1e3}
+    log(f"one-body S, T, V nbf {mol.nbf}, {len(charges)} charges: {np.median(ts) * 1e3:.1f} ms (with copies)")
+    return out
+
+
 def serve_leg(la, molecule, nreq, rate_hz, seed=5000):
     """Fock-build requests for molecules of 2-12 atoms (log-uniform; the two
     recipes alternate) arriving as a Poisson stream, served with continuous
@@ -353,6 +429,8 @@ def main():
         "setup_s": t_setup,
     }
     del plan
+    if not args.no_fock:
+        out["next_rows"] = next_rows_leg(la, mol, B, q, ctx)
     if not args.no_fock and args.eri_f_quartets:
         out["eri_f"] = eri_f_leg(la, molecule, flop_model, ctx, args.eri_f_quartets)
     if not args.no_fock:

@@ -682,9 +682,10 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     bool sides = false;
     for (auto& c : p->classes) sides |= c.slot.side >= 0;
     sides &= !capture;
-    // attenuated operators run every class on the generic kernel (whose
-    // core-integral step takes omega); the register-resident kernels stay
-    // Coulomb-only, so their primitive loop carries no operator branch.
+    // attenuated passes run on the generic kernel (whose core-integral step
+    // takes omega); the register-resident kernels stay Coulomb-only, so
+    // their primitive loop carries no operator branch, and run the plain
+    // Coulomb pass of erfc.
     // Passes (attenuation, subtract): Coulomb; erf = one attenuated pass
     // (omega = 0 gives exact zeros as Coulomb minus Coulomb); erfc =
     // Coulomb minus the erf pass, both per class on the class's stream
@@ -710,7 +711,7 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       A.qidx = p->qidx.as<long long>() + c.first;
       A.n = (int)c.count;
       if (!capture && ps == 0) HIPCHK(hipEventRecord(c.ev0, cs));
-      if (c.generic || (p->oper != 0 && p->omega > 0)) {
+      if (c.generic || A.omega > 0) {  // the attenuated pass of erf / erfc
         // one block per quartet, persistent over the class
         if (c.gidx < 0) throw L2gError(L2G_E_LMAX, "no generic-kernel program for this class");
         const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
