@@ -391,9 +391,31 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
 // pair (blockIdx.x = bra, blockIdx.y = ket); a thread owns up to two of the
 // at most 100 Cartesian pairs; the E and R tables sit in LDS, built by one
 // thread per axis / charge.  Coefficients are used as the basis holds them.
+// (t, u, v) with t + u + v <= 6 ordered by t + u + v: the entries of
+// total <= k are the first (k+1)(k+2)(k+3)/6
+struct L2gTuv {
+  signed char v[84][3];
+};
+constexpr L2gTuv make_tuv() {
+  L2gTuv r{};
+  int e = 0;
+  for (int s = 0; s <= 6; ++s)
+    for (int t = s; t >= 0; --t)
+      for (int u = s - t; u >= 0; --u) {
+        r.v[e][0] = (signed char)t;
+        r.v[e][1] = (signed char)u;
+        r.v[e][2] = (signed char)(s - t - u);
+        ++e;
+      }
+  return r;
+}
+__constant__ signed char c_tuv[84][3] = {};
+
 __global__ void __launch_bounds__(64) l2g_onebody_kernel(l2g_impl::L2gOneBodyArgs A) {
+  // blocks with bra >= ket; the ket > bra blocks are their transposes, as
+  // the reference's driver fills them (hartree-fock.cc:510-530)
   const int sa = blockIdx.x, sb = blockIdx.y;
-  if (sa >= A.nshell || sb >= A.nshell) return;
+  if (sa >= A.nshell || sb >= A.nshell || sb > sa) return;
   const int la = A.l[sa], lb = A.l[sb];
   const int nca = ncart_d(la), ncb = ncart_d(lb), npair = nca * ncb;
   const int tid = threadIdx.x;
@@ -468,30 +490,32 @@ __global__ void __launch_bounds__(64) l2g_onebody_kernel(l2g_impl::L2gOneBodyArg
         accT[k] += cc * t;
       }
       for (int c = 0; c < A.ncharge; ++c) {
-        if (tid == 0) {
-          const double PC[3] = {P[0] - A.qxyz[3 * c], P[1] - A.qxyz[3 * c + 1], P[2] - A.qxyz[3 * c + 2]};
+        // R^n_{000} = (-2p)^n F_n(T), one n per thread; then level by level
+        // n = L-1 .. 0, the entries 0 < t+u+v <= L-n of a level spread over
+        // the threads (each level reads only the one above)
+        const double PC[3] = {P[0] - A.qxyz[3 * c], P[1] - A.qxyz[3 * c + 1], P[2] - A.qxyz[3 * c + 2]};
+        if (tid <= L) {
           const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
           double f = 1.0;
-          for (int n = 0; n <= L; ++n) {
-            Rw[n][0][0][0] = f * boys_one(T, n, A.boys);
-            f *= -2.0 * p;
-          }
-          for (int n = L - 1; n >= 0; --n)
-            for (int t = 0; t <= L - n; ++t)
-              for (int u = 0; u <= L - n - t; ++u)
-                for (int v = 0; v <= L - n - t - u; ++v) {
-                  if (t + u + v == 0) continue;
-                  double val;
-                  if (t > 0)
-                    val = PC[0] * Rw[n + 1][t - 1][u][v] + (t > 1 ? (t - 1) * Rw[n + 1][t - 2][u][v] : 0.0);
-                  else if (u > 0)
-                    val = PC[1] * Rw[n + 1][t][u - 1][v] + (u > 1 ? (u - 1) * Rw[n + 1][t][u - 2][v] : 0.0);
-                  else
-                    val = PC[2] * Rw[n + 1][t][u][v - 1] + (v > 1 ? (v - 1) * Rw[n + 1][t][u][v - 2] : 0.0);
-                  Rw[n][t][u][v] = val;
-                }
+          for (int n = 0; n < tid; ++n) f *= -2.0 * p;
+          Rw[tid][0][0][0] = f * boys_one(T, tid, A.boys);
         }
         __syncthreads();
+        for (int n = L - 1; n >= 0; --n) {
+          const int top = L - n, cnt = (top + 1) * (top + 2) * (top + 3) / 6;  // t+u+v <= top
+          for (int e = tid + 1; e < cnt; e += 64) {
+            const int t = c_tuv[e][0], u = c_tuv[e][1], v = c_tuv[e][2];
+            double val;
+            if (t > 0)
+              val = PC[0] * Rw[n + 1][t - 1][u][v] + (t > 1 ? (t - 1) * Rw[n + 1][t - 2][u][v] : 0.0);
+            else if (u > 0)
+              val = PC[1] * Rw[n + 1][t][u - 1][v] + (u > 1 ? (u - 1) * Rw[n + 1][t][u - 2][v] : 0.0);
+            else
+              val = PC[2] * Rw[n + 1][t][u][v - 1] + (v > 1 ? (v - 1) * Rw[n + 1][t][u][v - 2] : 0.0);
+            Rw[n][t][u][v] = val;
+          }
+          __syncthreads();
+        }
         const double zf = A.q[c] * 2.0 * 3.14159265358979323846 / p;
         for (int k = 0; k < 2; ++k) {
           if (!own[k]) continue;
@@ -523,6 +547,7 @@ __global__ void __launch_bounds__(64) l2g_onebody_kernel(l2g_impl::L2gOneBodyArg
     const int ra = pa ? (A.gauss ? la + (fi == 0 ? 0 : ((fi & 1) ? (fi + 1) / 2 : -fi / 2)) : fi) : 0;
     const int rb = pb ? (A.gauss ? lb + (fj == 0 ? 0 : ((fj & 1) ? (fj + 1) / 2 : -fj / 2)) : fj) : 0;
     const long long o = (A.bf0[sa] + fi) * A.nbf + A.bf0[sb] + fj;
+    const long long ot = (A.bf0[sb] + fj) * A.nbf + A.bf0[sa] + fi;
     for (int m = 0; m < 3; ++m) {
       if (!out[m]) continue;
       double s = 0.0;
@@ -535,6 +560,7 @@ __global__ void __launch_bounds__(64) l2g_onebody_kernel(l2g_impl::L2gOneBodyArg
         }
       }
       out[m][o] = s;
+      if (sa != sb) out[m][ot] = s;  // a diagonal block writes each of its elements itself
     }
   }
 }
@@ -574,6 +600,8 @@ void launch_mode(const L2gEriArgs& A, const L2gGenericClass& C, const int4* prog
 
 void generic_init(const double* sh) {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_sh), sh, sizeof(double) * 4 * 7 * 10));
+  constexpr L2gTuv tuv = make_tuv();  // one-body kernel's Hermite index table
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tuv), tuv.v, sizeof(tuv.v)));
   // 160 KB per CU on gfx950, less 512 bytes for the static reduction array
   // (codegen/gen_generic.py plans at most that many doubles)
   const int maxlds = 160 * 1024 - 512;
