@@ -348,6 +348,8 @@ AX = 'xyz'
 # arithmetic type of the recursion, HRR and transforms of the kernel being
 # emitted ('double', or 'float' for the reduced-precision variants)
 _R = 'double'
+# attenuated (erf) variant: the core integrals s^(m+1/2) F_m(sT), s = w^2/(w^2+rho)
+_ATT = False
 
 
 def fmt(v):
@@ -556,11 +558,20 @@ def _emit_stage2(E, sfx, ib, ik, count):
     E.end()
     if count is not None:
         E(f"if ({count}ok{sfx}) ++nsurv;")
-    E(f"pfac{sfx} = ok{sfx} ? pf{sfx} : 0.0;")
     E(f"const double PQx{sfx} = Pg{sfx}.x - Qg{sfx}.x, PQy{sfx} = Pg{sfx}.y - Qg{sfx}.y, "
       f"PQz{sfx} = Pg{sfx}.z - Qg{sfx}.z;")
-    E(f"boys_fetch(gp{sfx} * gq{sfx} * oos{sfx} * (PQx{sfx} * PQx{sfx} + PQy{sfx} * PQy{sfx} + "
-      f"PQz{sfx} * PQz{sfx}), boysL, bq{sfx});")
+    if _ATT:
+        # erf(w r)/r: G_m(T) = s^(m+1/2) F_m(sT); the screening keeps the
+        # Coulomb prefactor, as the generic kernel and the reference do
+        E(f"const double rho{sfx} = gp{sfx} * gq{sfx} * oos{sfx};")
+        E(f"sw{sfx} = A.omega * A.omega / (A.omega * A.omega + rho{sfx});")
+        E(f"pfac{sfx} = ok{sfx} ? pf{sfx} * sqrt(sw{sfx}) : 0.0;")
+        E(f"boys_fetch(sw{sfx} * rho{sfx} * (PQx{sfx} * PQx{sfx} + PQy{sfx} * PQy{sfx} + "
+          f"PQz{sfx} * PQz{sfx}), boysL, bq{sfx});")
+    else:
+        E(f"pfac{sfx} = ok{sfx} ? pf{sfx} : 0.0;")
+        E(f"boys_fetch(gp{sfx} * gq{sfx} * oos{sfx} * (PQx{sfx} * PQx{sfx} + PQy{sfx} * PQy{sfx} + "
+          f"PQz{sfx} * PQz{sfx}), boysL, bq{sfx});")
 
 
 def emit_prim_loop(E, plan, graph, gi, accs):
@@ -575,6 +586,8 @@ def emit_prim_loop(E, plan, graph, gi, accs):
     E("double4 Qg = A.pp_P[pk0 + ik], QCk = A.pp_PA[pk0 + ik];")
     E("double2 xk = A.pp_aux[pk0 + ik];")
     E("double pfac;")
+    if _ATT:
+        E("double sw;")
     E("L2gBoysArg bq;")
     E.block("")
     _emit_stage2(E, "", "ib", "ik", "" if gi == 0 else None)
@@ -588,9 +601,19 @@ def emit_prim_loop(E, plan, graph, gi, accs):
     E("const double4 Qgn = A.pp_P[pk0 + ikn], QCkn = A.pp_PA[pk0 + ikn];")
     E("const double2 xkn = A.pp_aux[pk0 + ikn];")
     E("double pfacn;")
+    if _ATT:
+        E("double swn;")
     E("L2gBoysArg bqn;")
     E(f"double F[{plan.L + 1}];")
     E(f"boys_eval<{plan.L}>(bq, pfac, F);")
+    if _ATT and plan.L > 0:
+        E.block("")
+        E("double sm = sw;")
+        E.block(f"for (int m = 1; m <= {plan.L}; ++m)")
+        E("F[m] *= sm;")
+        E("sm *= sw;")
+        E.end()
+        E.end()
     if _R == 'double':
         E("const double* Fv = F;")
     else:
@@ -623,11 +646,13 @@ def emit_prim_loop(E, plan, graph, gi, accs):
     emit_vrr_group(E, plan, graph, gi, accs, mid=mid)
     E("Pg = Pgn; PAk = PAkn; xb = xbn; Qg = Qgn; QCk = QCkn; xk = xkn;")
     E("pfac = pfacn; bq = bqn; ib = ibn; ik = ikn;")
+    if _ATT:
+        E("sw = swn;")
     E.end()  # p loop
     E.end()  # npq > 0
 
 
-def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
+def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', att=False):
     la, lb, lc, ld = plan.la, plan.lb, plan.lc, plan.ld
     graph = VrrGraph()
     ne, nf = len(plan.ecomps), len(plan.fcomps)
@@ -637,9 +662,10 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
       f"{G} target group(s)")
     E(f"// group sizes (bra components x ket targets): {[(len(g), len(f)) for g, f in plan.groups]}")
     E(f"// group graph nodes: {plan.group_nodes}; estimated peak live doubles: {plan.group_peak}")
-    global _R
+    global _R, _ATT
     _R = real
-    kname = f"l2g_{mode}{'32' if real == 'float' else ''}_{plan.name}"
+    _ATT = att
+    kname = f"l2g_{'erf' if att else mode}{'32' if real == 'float' else ''}_{plan.name}"
     attr = f" __attribute__((amdgpu_waves_per_eu({waves}, {waves})))" if waves else ""
     if mode == 'fock':
         E("// Fock variant: digests each integral of the unique quartet into the")
@@ -895,6 +921,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double'):
     E.ind -= 1
     E("}")
     _R = 'double'
+    _ATT = False
     return kname, E.text(), in_regs
 
 
@@ -914,12 +941,14 @@ def class_waves(plan, budget, waves):
 
 
 def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=None,
-             modes=('eri', 'fock', 'schwarz', 'fp32'), generic_lmax=3, max_nodes=0, fp32_lmax=2):
+             modes=('eri', 'fock', 'schwarz', 'fp32', 'erf'), generic_lmax=3, max_nodes=0, fp32_lmax=2,
+             erf_lmax=2):
     """Write the kernels and the dispatch tables.  `budgets` maps a class name
     ('2222') to its own (budget, waves).  Classes with an l above 2 whose
     partitioned recursion evaluates more than `max_nodes` nodes per primitive
     quartet (0: no limit) get no kernel of their own and run on the generic
-    kernel; FP32 variants are made for classes with every l <= fp32_lmax."""
+    kernel; FP32 variants are made for classes with every l <= fp32_lmax, and
+    erf-attenuated ERI variants (mode 'erf') for every l <= erf_lmax."""
     os.makedirs(outdir, exist_ok=True)
     classes = canonical_classes(lmax)
     if only:
@@ -943,6 +972,9 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
             jobs.append((p, 'fock', 'double'))
         if 'schwarz' in modes and (p.la, p.lb) == (p.lc, p.ld):
             jobs.append((p, 'schwarz', 'double'))
+    for p in plans:
+        if 'erf' in modes and max(p.la, p.lb, p.lc, p.ld) <= erf_lmax:
+            jobs.append((p, 'erf', 'double'))
     for p in plans32:
         if 'eri' in modes:
             jobs.append((p, 'eri', 'float'))
@@ -957,7 +989,7 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
         files[j].append(i)
         load[j] += w
     tables = {('eri', 'double'): [], ('schwarz', 'double'): [], ('fock', 'double'): [],
-              ('eri', 'float'): [], ('fock', 'float'): []}
+              ('eri', 'float'): [], ('fock', 'float'): [], ('erf', 'double'): []}
     written = []
     for fi, idxs in enumerate(files):
         fn = os.path.join(outdir, f"eri_gen_{fi}.hip")
@@ -966,7 +998,8 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
         for i in sorted(idxs):
             p, mode, real = jobs[i]
             wv = cfg((p.la, p.lb, p.lc, p.ld))[1]
-            kname, code, in_regs = emit_class(p, mode=mode, waves=wv, real=real)
+            kname, code, in_regs = emit_class(p, mode='eri' if mode == 'erf' else mode, waves=wv, real=real,
+                                              att=mode == 'erf')
             parts.append(code)
             parts.append(f"void {kname}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s) {{")
             parts.append(f"  hipLaunchKernelGGL({kname}, dim3(grid), dim3(block), 0, s, A);")
@@ -980,19 +1013,21 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
         written.append(fn)
     table, stable, ftable = tables[('eri', 'double')], tables[('schwarz', 'double')], tables[('fock', 'double')]
     table32, ftable32 = tables[('eri', 'float')], tables[('fock', 'float')]
+    etable = tables[('erf', 'double')]
     lines = ['// generated by libint_amd/codegen/gen_eri.py — do not edit',
              '#include "eri_device.h"', '']
-    for p, k, _ in table + stable + ftable + table32 + ftable32:
+    for p, k, _ in table + stable + ftable + table32 + ftable32 + etable:
         lines.append(f"void {k}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s);")
     lines.append('')
     lines.append(f"extern const int l2g_gen_lmax = {lmax};")
     for name, tab in (("l2g_gen_classes", table), ("l2g_gen_schwarz", stable), ("l2g_gen_fock", ftable),
-                      ("l2g_gen_classes32", table32), ("l2g_gen_fock32", ftable32)):
+                      ("l2g_gen_classes32", table32), ("l2g_gen_fock32", ftable32),
+                      ("l2g_gen_classes_erf", etable)):
         lines.append(f"extern const L2gClassInfo {name}[] = {{")
         for p, k, in_regs in tab:
             ybuf = 0 if in_regs else len(p.ecomps) * ncart(p.lc) * ncart(p.ld) + \
                 (len(p.ecomps) * len(p.fcomps) if p.split_rows else 0)
-            if not in_regs and name in ("l2g_gen_classes", "l2g_gen_classes32"):
+            if not in_regs and name in ("l2g_gen_classes", "l2g_gen_classes32", "l2g_gen_classes_erf"):
                 ybuf += ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld)  # staged output
             lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
                          f"{sum(p.group_nodes)}, &{k}_launch}},")
@@ -1004,6 +1039,7 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
     lines.append(f"extern const int l2g_gen_nfock = {len(ftable)};")
     lines.append(f"extern const int l2g_gen_nclasses32 = {len(table32)};")
     lines.append(f"extern const int l2g_gen_nfock32 = {len(ftable32)};")
+    lines.append(f"extern const int l2g_gen_nclasses_erf = {len(etable)};")
     fn = os.path.join(outdir, "eri_table.cpp")
     new = '\n'.join(lines) + '\n'
     if not (os.path.exists(fn) and open(fn).read() == new):
@@ -1032,7 +1068,7 @@ if __name__ == '__main__':
     ap.add_argument("--only", default="", help="comma-separated class names, e.g. 2222")
     ap.add_argument("--class-cfg", default="", help="per class budget:waves, e.g. 2222=64:2,2122=80:2")
     ap.add_argument("--nfiles", type=int, default=8)
-    ap.add_argument("--modes", default="eri,fock,schwarz,fp32")
+    ap.add_argument("--modes", default="eri,fock,schwarz,fp32,erf")
     ap.add_argument("--generic-lmax", type=int, default=3, help="highest l of the generic kernel's classes")
     ap.add_argument("--max-nodes", type=int, default=0,
                     help="classes with l > 2 and more recursion nodes than this run on the generic kernel")

@@ -264,6 +264,7 @@ int l2g_init(int device, l2g_context* out) {
     };
     fill(ctx->fock_of, l2g_gen_fock, l2g_gen_nfock);
     fill(ctx->class32_of, l2g_gen_classes32, l2g_gen_nclasses32);
+    fill(ctx->classerf_of, l2g_gen_classes_erf, l2g_gen_nclasses_erf);
     fill(ctx->fock32_of, l2g_gen_fock32, l2g_gen_nfock32);
     for (auto& a : ctx->schwarz_of)
       for (auto& b : a) b = -1;
@@ -682,10 +683,11 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     bool sides = false;
     for (auto& c : p->classes) sides |= c.slot.side >= 0;
     sides &= !capture;
-    // attenuated passes run on the generic kernel (whose core-integral step
-    // takes omega); the register-resident kernels stay Coulomb-only, so
-    // their primitive loop carries no operator branch, and run the plain
-    // Coulomb pass of erfc.
+    // attenuated passes run on the class's erf-attenuated register kernel
+    // (a separate instantiation, so the Coulomb kernels' primitive loop
+    // carries no operator branch) where one is built, else on the generic
+    // kernel, whose core-integral step takes omega; the Coulomb pass of erfc
+    // runs on the Coulomb kernels.
     // Passes (attenuation, subtract): Coulomb; erf = one attenuated pass
     // (omega = 0 gives exact zeros as Coulomb minus Coulomb); erfc =
     // Coulomb minus the erf pass, both per class on the class's stream
@@ -711,7 +713,17 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       A.qidx = p->qidx.as<long long>() + c.first;
       A.n = (int)c.count;
       if (!capture && ps == 0) HIPCHK(hipEventRecord(c.ev0, cs));
-      if (c.generic || A.omega > 0) {  // the attenuated pass of erf / erfc
+      int erf_idx = -1;  // the class's erf-attenuated register kernel, for an attenuated pass
+      if (A.omega > 0 && !c.generic && !c.fp32) {
+        const L2gClassInfo& ci = l2g_gen_classes[c.cls];
+        erf_idx = ctx->classerf_of[ci.la][ci.lb][ci.lc][ci.ld];
+      }
+      if (erf_idx >= 0) {
+        const L2gClassInfo& ci = l2g_gen_classes_erf[erf_idx];
+        A.coop = c.coop ? 1 : 0;
+        ci.launch(A, c.slot.grid, p->block, cs);
+        LAUNCHCHK(std::string("erf ERI kernel of class ") + std::to_string(c.cls));
+      } else if (c.generic || A.omega > 0) {  // the attenuated pass of erf / erfc
         // one block per quartet, persistent over the class
         if (c.gidx < 0) throw L2gError(L2G_E_LMAX, "no generic-kernel program for this class");
         const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);

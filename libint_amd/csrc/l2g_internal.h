@@ -28,6 +28,9 @@ extern const L2gClassInfo l2g_gen_classes32[];
 extern const int l2g_gen_nclasses32;
 extern const L2gClassInfo l2g_gen_fock32[];
 extern const int l2g_gen_nfock32;
+// erf-attenuated ERI variants (same plans and scratch layout as l2g_gen_classes)
+extern const L2gClassInfo l2g_gen_classes_erf[];
+extern const int l2g_gen_nclasses_erf;
 extern const int l2g_gen_lmax;
 // programs of the generic kernel (codegen/gen_generic.py)
 extern const L2gGenericClass l2g_gen_generic[];
@@ -236,6 +239,7 @@ struct l2g_context_s {
   int class_of[8][8][8][8];  // index into l2g_gen_classes
   int fock_of[8][8][8][8];   // index into l2g_gen_fock (its own order)
   int class32_of[8][8][8][8];  // index into l2g_gen_classes32
+  int classerf_of[8][8][8][8]; // index into l2g_gen_classes_erf
   int fock32_of[8][8][8][8];   // index into l2g_gen_fock32
   int schwarz_of[8][8];
   int generic_of[8][8][8][8];  // index into l2g_gen_generic
