@@ -32,6 +32,8 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include <type_traits>
@@ -193,11 +195,16 @@ class Engine {
   Engine(Operator oper, size_t max_nprim, int max_l, int deriv_order = 0,
          scalar_type precision = std::numeric_limits<scalar_type>::epsilon(), Params params = Params(),
          BraKet braket = BraKet::invalid, ScreeningMethod screening_method = default_screening_method())
-      : oper_(oper), braket_(braket == BraKet::invalid ? BraKet::xx_xx : braket), screening_(screening_method) {
+      : oper_(oper),
+        braket_(braket == BraKet::invalid ? (is_onebody(oper) ? BraKet::x_x : BraKet::xx_xx) : braket),
+        screening_(screening_method) {
     assert(max_nprim > 0);
     check_operator(oper);
     set_params(params);
-    if (braket_ == BraKet::x_x) throw std::invalid_argument("libint2_amd: 1-body brakets are not provided");
+    if ((braket_ == BraKet::x_x) != is_onebody(oper))
+      throw std::invalid_argument("libint2_amd: BraKet::x_x goes with overlap, kinetic and nuclear only");
+    if (is_onebody(oper) && deriv_order != 0)
+      throw std::invalid_argument("libint2_amd: one-body integrals are provided for deriv_order 0");
     if (deriv_order < 0 || deriv_order > 1) throw std::invalid_argument("libint2_amd: deriv_order 0 or 1 is provided");
     if (deriv_order == 1 && (oper != Operator::coulomb || braket_ != BraKet::xx_xx))
       throw std::invalid_argument("libint2_amd: deriv_order 1 is provided for 4-centre Coulomb integrals");
@@ -228,20 +235,28 @@ class Engine {
   Engine& set(Operator o) {
     if (default_) throw using_default_initialized();
     check_operator(o);
+    if (is_onebody(o) != is_onebody(oper_))
+      throw std::invalid_argument("libint2_amd: set(Operator) cannot switch between one- and two-body operators");
     if (deriv_order_ == 1 && o != Operator::coulomb)
       throw std::invalid_argument("libint2_amd: deriv_order 1 is provided for Coulomb integrals");
     oper_ = o;
     return *this;
   }
   /// omega of erf_coulomb / erfc_coulomb (engine.h:685-690)
+  /// or the point charges {q, {x, y, z}} of Operator::nuclear (engine.h:685-690)
   template <typename Params>
   Engine& set_params(const Params& params) {
     if constexpr (std::is_arithmetic_v<Params>) omega_ = (double)params;
+    if constexpr (std::is_convertible_v<Params, std::vector<std::pair<double, std::array<double, 3>>>>) {
+      charges_.clear();
+      for (const auto& c : params) charges_.push_back(c);
+    }
     return *this;
   }
   Engine& set(BraKet b) {
     if (default_) throw using_default_initialized();
-    if (b == BraKet::x_x || b == BraKet::invalid) throw std::invalid_argument("libint2_amd: 1-body brakets are not provided");
+    if (b == BraKet::invalid || (b == BraKet::x_x) != is_onebody(oper_))
+      throw std::invalid_argument("libint2_amd: BraKet::x_x goes with overlap, kinetic and nuclear only");
     if (deriv_order_ == 1 && b != BraKet::xx_xx)
       throw std::invalid_argument("libint2_amd: deriv_order 1 is provided for 4-centre integrals");
     braket_ = b;
@@ -287,9 +302,11 @@ class Engine {
     return braket_ == BraKet::xs_xx ? compute_quartet(s1, Shell::unit(), s2, s3)
                                     : compute_quartet(s1, s2, s3, Shell::unit());
   }
-  /// 2-centre: (a|c) for BraKet::xs_xs
+  /// 2-centre: (a|c) for BraKet::xs_xs; <a|O|b> for the one-body operators
+  /// (BraKet::x_x): overlap, kinetic, nuclear (charges from set_params)
   const target_ptr_vec& compute(const Shell& s1, const Shell& s2) {
     if (default_) throw using_default_initialized();
+    if (braket_ == BraKet::x_x) return compute_onebody(s1, s2);
     assert(braket_ == BraKet::xs_xs && "# of arguments to compute() does not match the braket type");
     return compute_quartet(s1, Shell::unit(), s2, Shell::unit());
   }
@@ -413,9 +430,51 @@ class Engine {
     return (int)screening_;
   }
 
+  static bool is_onebody(Operator o) {
+    return o == Operator::overlap || o == Operator::kinetic || o == Operator::nuclear;
+  }
   static void check_operator(Operator o) {
-    if (o != Operator::coulomb && o != Operator::erf_coulomb && o != Operator::erfc_coulomb)
-      throw std::invalid_argument("libint2_amd: only Operator::coulomb, erf_coulomb and erfc_coulomb are provided");
+    if (o != Operator::coulomb && o != Operator::erf_coulomb && o != Operator::erfc_coulomb && !is_onebody(o))
+      throw std::invalid_argument(
+          "libint2_amd: Operator::overlap, kinetic, nuclear, coulomb, erf_coulomb and erfc_coulomb are provided");
+  }
+  // <s1|O|s2> as the (s1, s2) block of the two-shell basis's matrix, row-major
+  const target_ptr_vec& compute_onebody(const Shell& s1, const Shell& s2) {
+    const Shell* sh[2] = {&s1, &s2};
+    std::vector<int> l(2), pure(2), np(2);
+    std::vector<double> a, c, o;
+    for (int k = 0; k < 2; ++k) {
+      assert(sh[k]->ncontr() == 1 && "generally-contracted shells are not yet supported");
+      if (sh[k]->contr[0].l > lmax_) throw lmax_exceeded("1-body", lmax_, sh[k]->contr[0].l);
+      l[k] = sh[k]->contr[0].l;
+      pure[k] = sh[k]->contr[0].pure ? 1 : 0;
+      np[k] = (int)sh[k]->nprim();
+      a.insert(a.end(), sh[k]->alpha.begin(), sh[k]->alpha.end());
+      c.insert(c.end(), sh[k]->contr[0].coeff.begin(), sh[k]->contr[0].coeff.end());
+      o.insert(o.end(), sh[k]->O.begin(), sh[k]->O.end());
+    }
+    if (oper_ == Operator::nuclear && charges_.empty())
+      throw std::invalid_argument("libint2_amd: Operator::nuclear needs the charges (set_params)");
+    auto& rt = detail::Runtime::get();
+    detail::check(l2g_set_solid_harmonics_ordering(rt.ctx, rt.ordering));
+    l2g_basis b = nullptr;
+    detail::check(l2g_basis_create(rt.ctx, 2, l.data(), pure.data(), np.data(), a.data(), c.data(), o.data(), &b));
+    std::unique_ptr<l2g_basis_s, int (*)(l2g_basis)> hold(b, &l2g_basis_destroy);
+    const size_t n1 = s1.size(), n2 = s2.size(), n = n1 + n2;
+    std::vector<double> q, xyz, M(n * n);
+    for (const auto& ch : charges_) {
+      q.push_back(ch.first);
+      xyz.insert(xyz.end(), ch.second.begin(), ch.second.end());
+    }
+    double* S = oper_ == Operator::overlap ? M.data() : nullptr;
+    double* T = oper_ == Operator::kinetic ? M.data() : nullptr;
+    double* V = oper_ == Operator::nuclear ? M.data() : nullptr;
+    detail::check(l2g_onebody(b, V ? (int)q.size() : 0, V ? q.data() : nullptr, V ? xyz.data() : nullptr, S, T, V));
+    buffer_.resize(std::max<size_t>(n1 * n2, 1));
+    for (size_t i = 0; i < n1; ++i)
+      for (size_t j = 0; j < n2; ++j) buffer_[i * n2 + j] = M[i * n + n1 + j];
+    targets_[0] = buffer_.data();
+    return targets_;
   }
   int plan_operator() const {
     return oper_ == Operator::erf_coulomb ? L2G_OPER_ERF_COULOMB
@@ -424,6 +483,7 @@ class Engine {
 
   Operator oper_ = Operator::invalid;
   double omega_ = 0;
+  std::vector<std::pair<double, std::array<double, 3>>> charges_;  // Operator::nuclear
   int deriv_order_ = 0;
   BraKet braket_ = BraKet::invalid;
   ScreeningMethod screening_ = ScreeningMethod::Original;

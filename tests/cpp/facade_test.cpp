@@ -48,6 +48,10 @@ int main(int argc, char** argv) {
   // attenuated integrals, are refused before any GPU work
   CHECK(throws<std::invalid_argument>([] { Engine(Operator::coulomb, 2, 2, 2); }));
   CHECK(throws<std::invalid_argument>([] { Engine(Operator::erf_coulomb, 2, 2, 1, 1e-15, 0.5); }));
+  // one-body operators go with BraKet::x_x only, at deriv_order 0
+  CHECK(throws<std::invalid_argument>([] { Engine(Operator::overlap, 2, 2, 1); }));
+  CHECK(throws<std::invalid_argument>([] { Engine(Operator::coulomb, 2, 2, 0, 1e-15, 0, BraKet::x_x); }));
+  CHECK(throws<std::invalid_argument>([] { Engine(Operator::kinetic, 2, 2, 0, 1e-15, 0, BraKet::xx_xx); }));
   if (cpu_only) {
     std::printf("cpu checks: %s\n", fails ? "FAIL" : "ok");
     return fails ? 1 : 0;
@@ -117,6 +121,29 @@ int main(int argc, char** argv) {
       CHECK(ed.results()[dset] != nullptr);
       for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", ed.results()[dset][i]);
     }
+  }
+  // one-body operators as compute_1body_ints drives them (hartree-fock.cc:480-535),
+  // printed as O (overlap), K (kinetic), N (nuclear with two charges)
+  {
+    Engine eo(Operator::overlap, 2, 2);
+    CHECK(eo.braket() == BraKet::x_x);
+    eo.compute(basis[2], basis[1]);
+    const size_t n = basis[2].size() * basis[1].size();
+    std::printf("O 2 1 %zu\n", n);
+    for (size_t i = 0; i < n; ++i) std::printf("%.17g\n", eo.results()[0][i]);
+    Engine ek(Operator::kinetic, 2, 2);
+    ek.compute(basis[3], basis[2]);
+    const size_t nk = basis[3].size() * basis[2].size();
+    std::printf("K 3 2 %zu\n", nk);
+    for (size_t i = 0; i < nk; ++i) std::printf("%.17g\n", ek.results()[0][i]);
+    Engine en(Operator::nuclear, 2, 2);
+    CHECK(throws<std::invalid_argument>([&] { en.compute(basis[1], basis[3]); }));  // no charges yet
+    std::vector<std::pair<double, std::array<double, 3>>> q{{8.0, {{0.1, 0.0, -0.2}}}, {1.0, {{1.0, 0.5, 0.3}}}};
+    en.set_params(q);
+    en.compute(basis[1], basis[3]);
+    const size_t nn = basis[1].size() * basis[3].size();
+    std::printf("N 1 3 %zu\n", nn);
+    for (size_t i = 0; i < nn; ++i) std::printf("%.17g\n", en.results()[0][i]);
   }
   // python/tests/test_libint2.py:39-42: ||(s s|s)|| = 3.6563211198 for
   // s = Shell(0, [(1, 10)])

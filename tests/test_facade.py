@@ -10,7 +10,8 @@ import pytest
 import libint_amd as la
 import oracle
 import oracle.deriv1
-from tests.helpers import compare
+from oracle import onebody
+from tests.helpers import cart_to_shells, compare
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "cpp", "facade_test")
@@ -57,6 +58,22 @@ def test_facade_results_match_oracle():
                 assert compare(vals[k], o[k], 1e-12, 1e-10)[0] == 0, k
             checked += 1
             i += 12 * n + 1
+        elif lines[i][:2] in ("O ", "K ", "N "):
+            # one-body <a|O|b> block against the oracle's (Cartesian, then the shells' functions)
+            kind, a, b, n = lines[i].split()
+            n = int(n)
+            vals = np.array([float(x) for x in lines[i + 1:i + 1 + n]])
+            four = sh[:4]
+            charges = [(8.0, np.array([0.1, 0.0, -0.2])), (1.0, np.array([1.0, 0.5, 0.3]))]
+            S, T, V = onebody.one_body([(x.l, x.alpha, x.coeff, x.O) for x in four], charges)
+            X = cart_to_shells(four)
+            M = X.T @ {"O": S, "K": T, "N": V}[kind] @ X
+            off = np.concatenate([[0], np.cumsum([x.size() for x in four])])
+            a, b = int(a), int(b)
+            blk = M[off[a]:off[a + 1], off[b]:off[b + 1]].reshape(-1)
+            assert compare(vals, blk, 1e-12, 1e-10)[0] == 0, kind
+            checked += 1
+            i += n + 1
         elif lines[i].startswith("Q ") or lines[i].startswith("R "):
             kind, a, b, c, d, n = lines[i].split()
             n = int(n)
@@ -69,4 +86,5 @@ def test_facade_results_match_oracle():
             i += n + 1
         else:
             i += 1
-    assert checked == 9  # 5 quartets, one 3-centre, one 2-centre, one erf set, one derivative set
+    # 5 quartets, one 3-centre, one 2-centre, one erf set, one derivative set, three one-body blocks
+    assert checked == 12
