@@ -38,80 +38,93 @@ __device__ __forceinline__ int l2g_cart_index(int l, int a, int b) { return (l -
 // rows of the pure centres.  Derivative d = 3 k + i of quartet q goes to
 // out[d * total + out_off[q] + f], f row-major over the four shells' output
 // functions in caller order (the layout of Engine::results()[d] for
-// deriv_order = 1, include/libint2/engine.h:779-781).  A block per quartet;
-// its threads stride over the 12 x nout outputs.
+// deriv_order = 1, include/libint2/engine.h:779-781).  A block per quartet
+// takes the 12 derivatives one at a time: the Cartesian block (two loads
+// per element) into LDS, then one centre's solid-harmonic rows after the
+// other between two LDS buffers, then the store.  Shells up to l = 2 (the
+// raised ones up to 3): at most 6^4 Cartesian elements.
+constexpr int kDerivMaxCart = 1296;
+
 __global__ void __launch_bounds__(256) l2g_deriv1_combine_kernel(const double* __restrict__ raw,
                                                                  const long long* __restrict__ raw_off,
                                                                  const long long* __restrict__ out_off,
                                                                  const int* __restrict__ qinfo, long long nq,
                                                                  long long total, double* __restrict__ out,
                                                                  l2g_impl::L2gDerivTab tab) {
+  __shared__ double buf[2][kDerivMaxCart];
   const long long q = blockIdx.x;
   if (q >= nq) return;
   const int info = qinfo[q];
-  int l[4], pure[4], nc[4], no[4];
+  int l[4], pure[4], nc[4];
   for (int k = 0; k < 4; ++k) {
     l[k] = (info >> (4 * k)) & 15;
     pure[k] = (info >> (16 + k)) & 1;
     nc[k] = (l[k] + 1) * (l[k] + 2) / 2;
-    no[k] = pure[k] ? 2 * l[k] + 1 : nc[k];
   }
-  const int nout = no[0] * no[1] * no[2] * no[3];
+  const int ncart = nc[0] * nc[1] * nc[2] * nc[3];
   const long long obase = out_off[q];
-  for (int idx = threadIdx.x; idx < 12 * nout; idx += blockDim.x) {
-    const int d = idx / nout;
-    int f = idx - d * nout;
+  for (int d = 0; d < 12; ++d) {
     const int k = d / 3, ax = d - 3 * k;
-    int fo[4];
-    for (int j = 3; j >= 0; --j) {
-      fo[j] = f % no[j];
-      f /= no[j];
-    }
     const double* rb = raw + raw_off[(long long)k * nq + q];
     const double* lb = raw + raw_off[(long long)(4 + k) * nq + q];
     int dp[4], dm[4];
     for (int j = 0; j < 4; ++j) dp[j] = dm[j] = nc[j];
     dp[k] = (l[k] + 2) * (l[k] + 3) / 2;
     dm[k] = l[k] * (l[k] + 1) / 2;
-    int n[4];
-    for (int j = 0; j < 4; ++j) n[j] = pure[j] ? tab.nnz[l[j]][fo[j]] : 1;
-    double sum = 0.0;
-    for (int a0 = 0; a0 < n[0]; ++a0)
-      for (int a1 = 0; a1 < n[1]; ++a1)
-        for (int a2 = 0; a2 < n[2]; ++a2)
-          for (int a3 = 0; a3 < n[3]; ++a3) {
-            const int a[4] = {a0, a1, a2, a3};
-            int c[4];
-            double w = 1.0;
-            for (int j = 0; j < 4; ++j) {
-              if (pure[j]) {
-                c[j] = tab.cidx[l[j]][fo[j]][a[j]];
-                w *= tab.coef[l[j]][fo[j]][a[j]];
-              } else {
-                c[j] = fo[j];
-              }
-            }
-            // exponents of component c[k] of shell k
-            int r = 0;
-            while ((r + 1) * (r + 2) / 2 <= c[k]) ++r;
-            int e[3];
-            e[2] = c[k] - r * (r + 1) / 2;
-            e[1] = r - e[2];
-            e[0] = l[k] - r;
-            int cu[4] = {c[0], c[1], c[2], c[3]};
-            ++e[ax];
-            cu[k] = l2g_cart_index(l[k] + 1, e[0], e[1]);
-            --e[ax];
-            double v = rb[((long long)(cu[0] * dp[1] + cu[1]) * dp[2] + cu[2]) * dp[3] + cu[3]];
-            if (e[ax] > 0) {
-              const int nx = e[ax];
-              --e[ax];
-              cu[k] = l2g_cart_index(l[k] - 1, e[0], e[1]);
-              v -= nx * lb[((long long)(cu[0] * dm[1] + cu[1]) * dm[2] + cu[2]) * dm[3] + cu[3]];
-            }
-            sum += w * v;
-          }
-    out[(long long)d * total + obase + idx - d * nout] = sum;
+    // Cartesian derivative block
+    for (int idx = threadIdx.x; idx < ncart; idx += blockDim.x) {
+      int c[4], f = idx;
+      for (int j = 3; j >= 0; --j) {
+        c[j] = f % nc[j];
+        f /= nc[j];
+      }
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= c[k]) ++r;
+      int e[3];
+      e[2] = c[k] - r * (r + 1) / 2;
+      e[1] = r - e[2];
+      e[0] = l[k] - r;
+      int cu[4] = {c[0], c[1], c[2], c[3]};
+      ++e[ax];
+      cu[k] = l2g_cart_index(l[k] + 1, e[0], e[1]);
+      --e[ax];
+      double v = rb[((long long)(cu[0] * dp[1] + cu[1]) * dp[2] + cu[2]) * dp[3] + cu[3]];
+      if (e[ax] > 0) {
+        const int nx = e[ax];
+        --e[ax];
+        cu[k] = l2g_cart_index(l[k] - 1, e[0], e[1]);
+        v -= nx * lb[((long long)(cu[0] * dm[1] + cu[1]) * dm[2] + cu[2]) * dm[3] + cu[3]];
+      }
+      buf[0][idx] = v;
+    }
+    __syncthreads();
+    // solid-harmonic rows, one pure centre at a time
+    int dims[4] = {nc[0], nc[1], nc[2], nc[3]};
+    int cur = 0;
+    for (int j = 0; j < 4; ++j) {
+      if (!pure[j]) continue;
+      const int nout_j = 2 * l[j] + 1;
+      int inner = 1;
+      for (int t = j + 1; t < 4; ++t) inner *= dims[t];
+      int outer = 1;
+      for (int t = 0; t < j; ++t) outer *= dims[t];
+      const int n_new = outer * nout_j * inner;
+      for (int idx = threadIdx.x; idx < n_new; idx += blockDim.x) {
+        const int i_in = idx % inner, rest = idx / inner;
+        const int m = rest % nout_j, o = rest / nout_j;
+        double s = 0.0;
+        for (int a = 0; a < tab.nnz[l[j]][m]; ++a)
+          s += tab.coef[l[j]][m][a] * buf[cur][(o * dims[j] + tab.cidx[l[j]][m][a]) * inner + i_in];
+        buf[cur ^ 1][idx] = s;
+      }
+      dims[j] = nout_j;
+      cur ^= 1;
+      __syncthreads();
+    }
+    const int nout = dims[0] * dims[1] * dims[2] * dims[3];
+    double* dst = out + (long long)d * total + obase;
+    for (int idx = threadIdx.x; idx < nout; idx += blockDim.x) dst[idx] = buf[cur][idx];
+    __syncthreads();
   }
 }
 

@@ -1110,9 +1110,12 @@ int l2g_eri_deriv1_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets,
     const int n = B->nshell;
     for (int64_t i = 0; i < 4 * nq; ++i)
       if (quartets[i] < 0 || quartets[i] >= n) throw std::invalid_argument("shell index out of range");
+    // raised shells need l + 1 <= the kernels' limit; the combination kernel
+    // stages at most 6^4 Cartesian elements (l <= 2)
+    const int dlmax = std::min(lmax_supported() - 1, 2);
     for (int64_t i = 0; i < 4 * nq; ++i)
-      if (B->l[quartets[i]] + 1 > lmax_supported())
-        throw L2gError(L2G_E_LMAX, "first derivatives need l + 1 <= " + std::to_string(lmax_supported()) +
+      if (B->l[quartets[i]] > dlmax)
+        throw L2gError(L2G_E_LMAX, "first derivatives are provided up to l = " + std::to_string(dlmax) +
                                        " (angular momentum " + std::to_string(B->l[quartets[i]]) + ")");
     auto p = std::make_unique<l2g_deriv1_plan_s>();
     p->ctx = ctx;
