@@ -809,9 +809,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Fock kernels failed: ") + hipGetErrorString(e));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    f->last_ms = ms;
+    f->last_ms = event_ms_or(e0, e1, 0.0f);
     if (J && NN) HIPCHK(hipMemcpy(J, f->jg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (K && NN) HIPCHK(hipMemcpy(K, f->kg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (G && NN) HIPCHK(hipMemcpy(G, f->dens.p, NN * sizeof(double), hipMemcpyDeviceToHost));
@@ -864,9 +862,7 @@ int l2g_fock_class_stats(l2g_fock f, int cap, int* nclasses, int* class_code, in
         if (nquartets) nquartets[k] = f->cls_nq[c];
         if (nprim_quartets) nprim_quartets[k] = f->cls_nprim[c];
         if (ms) {
-          float t = 0;
-          (void)hipEventElapsedTime(&t, f->ev[2 * c], f->ev[2 * c + 1]);
-          ms[k] = t;
+          ms[k] = event_ms_or(f->ev[2 * c], f->ev[2 * c + 1], 0.0f);
         }
       }
       ++k;

@@ -697,8 +697,7 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
                                                  : std::vector<std::pair<double, int>>{{0.0, 0}, {0.0, 1}};
     else passes = p->omega > 0 ? std::vector<std::pair<double, int>>{{0.0, 0}, {p->omega, 1}}
                                : std::vector<std::pair<double, int>>{{0.0, 0}};
-    auto launch_all = [&] {
-    if (sides) fork_sides(*ctx);
+    auto launch_classes = [&] {
     for (auto& c : p->classes) {
     for (size_t ps = 0; ps < passes.size(); ++ps) {
       A.omega = passes[ps].first;
@@ -738,7 +737,22 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       if (!capture && ps + 1 == passes.size()) HIPCHK(hipEventRecord(c.ev1, cs));
     }
     }
-    if (sides) join_sides(*ctx);
+    };
+    // the side streams are joined back also when a launch fails
+    auto launch_all = [&] {
+      if (sides) fork_sides(*ctx);
+      try {
+        launch_classes();
+      } catch (...) {
+        if (sides) {
+          try {
+            join_sides(*ctx);
+          } catch (...) {
+          }
+        }
+        throw;
+      }
+      if (sides) join_sides(*ctx);
     };
     if (!capture) {
       launch_all();
@@ -860,9 +874,7 @@ int l2g_eri_plan_stats(l2g_eri_plan p, int cap, int* ncls, int* code, int64_t* n
       if (nquart) nquart[i] = c.count;
       if (nprimq) nprimq[i] = c.nprimq;
       if (ms) {
-        float t = 0;
-        if (p->use_graph || hipEventElapsedTime(&t, c.ev0, c.ev1) != hipSuccess) t = -1;
-        ms[i] = t;
+        ms[i] = p->use_graph ? -1.0f : event_ms_or(c.ev0, c.ev1, -1.0f);
       }
     }
   });

@@ -149,6 +149,18 @@ struct DevBuf {
   }
 };
 
+// Elapsed time between two events where it may legitimately be unavailable
+// (events not recorded yet): dflt then, and the runtime's last-error state
+// is cleared so that a later launch check does not report the failure.
+inline float event_ms_or(hipEvent_t a, hipEvent_t b, float dflt) {
+  float t = 0;
+  if (hipEventElapsedTime(&t, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return dflt;
+  }
+  return t;
+}
+
 template <class T>
 inline void upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
   d.alloc(std::max<size_t>(h.size() * sizeof(T), 16));

@@ -143,3 +143,32 @@ def test_f_fock_vs_oracle(ctx, generic_all):
     assert np.abs(G - Go).max() <= 1e-8
     assert np.abs(J - Jo).max() <= 1e-8
     assert np.abs(K - Ko).max() <= 1e-8
+
+
+def test_few_deep_quartets_route_to_generic(ctx):
+    """A class with a register kernel, total L >= 7, few quartets and deep
+    contractions runs on the generic kernel (a block per quartet, each
+    contraction in its serial order), at the integral bar; the same class
+    with shallow contractions stays on its register kernel."""
+    rng = np.random.default_rng(41)
+    deep = []
+    for k in range(4):
+        a = np.exp(rng.uniform(np.log(0.3), np.log(20.0), 6))
+        deep.append(la.Shell(2, list(zip(a, rng.uniform(0.2, 1.0, 6))), tuple(rng.normal(0, 0.8, 3)), pure=True))
+    arrays = la.BasisArrays.from_shells(deep)
+    q = np.array(list(itertools.product(range(4), repeat=4)), np.int32)[:60]
+    say(f"case: {len(q)} (dd|dd) quartets with K = 6 on every shell")
+    b = la.Basis(ctx, arrays)
+    plan = la.EriPlan(b, q)
+    st = plan.stats()
+    assert [(s["cls"], s["generic"]) for s in st] == [("2222", True)], st
+    plan.execute()
+    g, gnz = plan.download()
+    o, off, onz = oracle.eri_quartets(arrays, q)
+    nfail, narb, oerr = compare_arbitrated(arrays, q, off, g, o, ABS, REL)
+    say(f"  max abs {np.abs(g - o).max():.2e}; fails {nfail}, arbitrated {narb}")
+    assert nfail == 0
+    assert np.array_equal(gnz, onz)
+    shallow = random_shells(rng, [2, 2, 2, 2], kmax=1)
+    st2 = la.EriPlan(la.Basis(ctx, la.BasisArrays.from_shells(shallow)), q).stats()
+    assert [(s["cls"], s["generic"]) for s in st2] == [("2222", False)], st2
