@@ -782,8 +782,9 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     for (int c = 0; c < ncls; ++c) {
       const int64_t m = big.count[c];
       if (m == 0) continue;
-      // the generic kernel runs a block per item and no lane scratch
-      lanes[c] = fock_is_generic(c) ? m * 256 : m;
+      // the generic kernel runs a block per item, no lane scratch, a few
+      // blocks per CU: up to 4 items per CU count as small (side streams)
+      lanes[c] = fock_is_generic(c) ? m * 64 : m;
       yd[c] = fock_ybuf(c);
       ymax = std::max<int64_t>(ymax, yd[c]);
     }
@@ -1217,7 +1218,7 @@ void run_round(l2g_server_s* S, const std::vector<ServerRequest*>& act) {
   std::vector<int64_t> lanes(rv.count.size(), 0), yd(rv.count.size(), 0);
   for (size_t c = 0; c < rv.count.size(); ++c) {
     if (rv.count[c] == 0) continue;
-    lanes[c] = fock_is_generic((int)c) ? rv.count[c] * 256 : rv.count[c];
+    lanes[c] = fock_is_generic((int)c) ? rv.count[c] * 64 : rv.count[c];
     yd[c] = fock_ybuf((int)c);
   }
   std::vector<ClassSlot> slots;  // within the server's scratch (sized for the full grid)
