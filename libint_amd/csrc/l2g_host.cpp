@@ -1132,13 +1132,14 @@ int l2g_eri_deriv1_plan_create(l2g_basis B, int64_t nq, const int32_t* quartets,
     auto p = std::make_unique<l2g_deriv1_plan_s>();
     p->ctx = ctx;
     p->nq = nq;
-    // augmented basis
+    // augmented basis; shells above the derivative limit are never referenced
+    // by the (validated) quartets, and their raised copies keep their own l
     std::vector<int> al, ap, an;
     std::vector<double> aa, ac, ao;
     for (int kind = 0; kind < 3; ++kind)
       for (int s = 0; s < n; ++s) {
         const int l = B->l[s];
-        al.push_back(kind == 1 ? l + 1 : (kind == 2 ? std::max(l - 1, 0) : l));
+        al.push_back(kind == 1 ? (l <= dlmax ? l + 1 : l) : (kind == 2 ? std::max(l - 1, 0) : l));
         ap.push_back(0);
         an.push_back(B->nprim[s]);
         for (int k = B->off[s]; k < B->off[s + 1]; ++k) {

@@ -126,3 +126,30 @@ def test_zero_density_fock(ctx):
     G, _, _ = fb.build(D, precision=1e-12)
     Go, _, _, _ = oracle.fock_2e(mol.basis, D, precision=1e-12)
     assert np.abs(G - Go).max() < 1e-8
+
+
+def test_deriv_and_onebody_edges(ctx):
+    """Derivative plans: empty, an f shell (raised beyond the kernels' l) and
+    a bad index are handled; one-body integrals without charges leave V
+    untouched and give symmetric S and T."""
+    say("case: derivative plan and one-body edge cases")
+    rng = np.random.default_rng(36)
+    shells = random_shells(rng, [0, 1, 2, 3])
+    arrays = la.BasisArrays.from_shells(shells)
+    b = la.Basis(ctx, arrays)
+    p = la.Deriv1Plan(b, np.zeros((0, 4), np.int32))
+    p.execute()
+    ctx.synchronize()
+    v, nz = p.download()
+    assert v.size == 0 and nz.size == 0
+    with pytest.raises(la.L2gError):
+        la.Deriv1Plan(b, [[3, 0, 0, 0]])
+    with pytest.raises(la.L2gError):
+        la.Deriv1Plan(b, [[0, 0, 0, 9]])
+    # the context stays usable
+    g, _, _ = la.compute_quartets(b, [[2, 1, 2, 0]])
+    assert g.size == 5 * 3 * 5
+    S, T, V = la.one_body(b, [])
+    n = int(arrays.sizes().sum())
+    assert S.shape == (n, n) and not V.any()
+    assert np.allclose(S, S.T, atol=1e-14) and np.allclose(T, T.T, atol=1e-13)
