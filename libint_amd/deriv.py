@@ -85,3 +85,125 @@ def fock_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, D, precision=np.fi
         G[uc, r[0], r[3]] -= 0.25 * np.einsum("xabcd,bc->xad", V, D[r[1], r[2]])
         G[uc, r[1], r[2]] -= 0.25 * np.einsum("xabcd,ad->xbc", V, D[r[0], r[3]])
     return 0.5 * (G + G.transpose(0, 2, 1))
+
+
+def _cart_exps(l):
+    return [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+
+
+def _cart_index(x, y, z):
+    l = x + y + z
+    return (l - x) * (l - x + 1) // 2 + (l - x - y)
+
+
+def onebody_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, charges, one_body_fn=None):
+    """First derivatives of the overlap, kinetic and point-charge attraction
+    matrices with respect to the atom coordinates, as the reference's
+    compute_1body_ints_deriv<overlap / kinetic / nuclear>(1, obs, atoms)
+    forms them (tests/hartree-fock/hartree-fock++.cc:1028-1227): S1, T1, V1 of
+    shape (3 * natoms, nbf, nbf), coordinate 3 * atom + xyz.  charges[c] =
+    (Z, position) sits on atom c (the nuclei, in atom order).
+
+    The bra derivative d/dA_i <a|O|b> = <a+1_i|O|b> (coefficients 2 alpha c)
+    - n_i <a-1_i|O|b> comes from the one-body matrices of an augmented basis
+    (Cartesian copies, raised, lowered shells) on the GPU; the ket derivative
+    is its transpose (S, T, V are symmetric); the derivative of each charge's
+    own term with respect to the charge's position is minus the sum of the
+    bra and ket derivatives of that term (translational invariance).  Pure
+    shells are transformed at the end.  one_body_fn(arrays, charges) ->
+    (S, T, V) replaces la.one_body (a checker passes its own)."""
+    from . import Basis, one_body, solid_harmonics_coeff
+    n = arrays.nshell
+    L = arrays.l.astype(int)
+    if L.max(initial=0) > 2:
+        raise ValueError("one-body derivatives are provided up to l = 2 (raised shells up to 3)")
+    off = np.concatenate([[0], np.cumsum(arrays.nprim)])
+    ls, pure, nprim, al, co, org = [], [], [], [], [], []
+    for kind in range(3):
+        for s in range(n):
+            a = arrays.alpha[off[s]:off[s + 1]]
+            c = arrays.coeff[off[s]:off[s + 1]]
+            l = int(L[s])
+            if kind == 1:
+                l, c = l + 1, 2.0 * a * c
+            elif kind == 2:
+                l = max(l - 1, 0)
+            ls.append(l)
+            pure.append(0)
+            nprim.append(len(a))
+            al.append(a)
+            co.append(c)
+            org.append(arrays.origin[3 * s:3 * s + 3])
+    aug = BasisArrays(ls, pure, nprim, np.concatenate(al), np.concatenate(co), np.concatenate(org))
+    if one_body_fn is None:
+        b = Basis(ctx, aug)
+
+        def one_body_fn(_arrays, ch):
+            return one_body(b, ch)
+    nca = [(l + 1) * (l + 2) // 2 for l in ls]
+    aoff = np.concatenate([[0], np.cumsum(nca)])
+    ncart = int(aoff[n])  # the Cartesian copies come first
+
+    def bra_derivs(M):
+        """d/dA_i <a|.|b> over Cartesian a (rows) and b (columns), i = x, y, z."""
+        out = np.zeros((3, ncart, ncart))
+        for s in range(n):
+            l = int(L[s])
+            for ci, e in enumerate(_cart_exps(l)):
+                row = aoff[s] + ci
+                for i in range(3):
+                    up = list(e)
+                    up[i] += 1
+                    ru = aoff[n + s] + _cart_index(*up)
+                    v = M[ru, :ncart].copy()
+                    if e[i] > 0:
+                        dn = list(e)
+                        dn[i] -= 1
+                        v -= e[i] * M[aoff[2 * n + s] + _cart_index(*dn), :ncart]
+                    out[i, row] = v
+        return out
+
+    atom_of_fn = np.concatenate([[shell2atom[s]] * nca[s] for s in range(n)]) if n else np.zeros(0, int)
+    # Cartesian -> the shells' own functions
+    blocks = []
+    for s in range(n):
+        l = int(L[s])
+        if arrays.pure[s] and l > 0:
+            blocks.append(np.array([[solid_harmonics_coeff(l, m, *c) for c in _cart_exps(l)]
+                                    for m in range(-l, l + 1)]).T)
+        else:
+            blocks.append(np.eye((l + 1) * (l + 2) // 2))
+    X = np.zeros((ncart, sum(bk.shape[1] for bk in blocks)))
+    r = c = 0
+    for bk in blocks:
+        X[r:r + bk.shape[0], c:c + bk.shape[1]] = bk
+        r += bk.shape[0]
+        c += bk.shape[1]
+
+    def assemble(Dbra, extra=None):
+        """Per atom coordinate: bra part on rows of that atom plus its
+        transpose, plus the operator-centre term; then the pure transform."""
+        out = np.zeros((3 * natoms, X.shape[1], X.shape[1]))
+        for a in range(natoms):
+            mask = (atom_of_fn == a).astype(float)[:, None]
+            for i in range(3):
+                M = Dbra[i] * mask
+                M = M + M.T
+                if extra is not None:
+                    M = M + extra[3 * a + i]
+                out[3 * a + i] = X.T @ M @ X
+        return out
+
+    S, T, _ = one_body_fn(aug, [])
+    S1 = assemble(bra_derivs(S))
+    T1 = assemble(bra_derivs(T))
+    V1 = np.zeros_like(S1)
+    for cidx, (Z, pos) in enumerate(charges):
+        _, _, Vc = one_body_fn(aug, [(Z, pos)])
+        Dc = bra_derivs(Vc)
+        full = Dc + Dc.transpose(0, 2, 1)  # d/dA + d/dB of the charge's term, all rows
+        extra = np.zeros((3 * natoms, ncart, ncart))
+        for i in range(3):
+            extra[3 * cidx + i] = -full[i]
+        V1 += assemble(Dc, extra)
+    return S1, T1, V1

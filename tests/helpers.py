@@ -65,7 +65,7 @@ def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol):
     return nfail, narb, oerr
 
 
-def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0, density=False):
+def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0, density=False, orbitals=False):
     """Closed-shell SCF as the reference's hartree-fock.cc driver does
     (tests/hartree-fock/hartree-fock.cc:181-236): F = H + G(D) with
     G = 2J - K from fock2e(D), D = C_occ C_occ^T, energy sum D (H + F) +
@@ -75,7 +75,9 @@ def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0, density=False
     commutator error F D S - S D F, as hartree-fock++.cc does with
     libint2::DIIS (tests/hartree-fock/hartree-fock++.cc:296-330); the
     converged energy is the same.  Returns the energy, or with density=True
-    (energy, D) with D the density the energy was evaluated on."""
+    (energy, D) with D the density the energy was evaluated on, or with
+    orbitals=True (energy, D, W) with the energy-weighted density
+    W = C_occ diag(eps_occ) C_occ^T of the final Fock matrix."""
     import scipy.linalg
     _, C = scipy.linalg.eigh(H, S)
     D = C[:, :nocc] @ C[:, :nocc].T
@@ -109,6 +111,10 @@ def rhf(S, H, nocc, fock2e, enuc, conv=1e-12, maxiter=100, diis=0, density=False
             break
     F = H + fock2e(D)
     e = float(np.sum(D * (H + F))) + enuc
+    if orbitals:
+        eps, C = scipy.linalg.eigh(F, S)
+        W = C[:, :nocc] @ np.diag(eps[:nocc]) @ C[:, :nocc].T
+        return e, D, W
     return (e, D) if density else e
 
 

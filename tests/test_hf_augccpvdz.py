@@ -73,7 +73,7 @@ def test_rhf_energy_gpu_augccpvdz(ctx):
         G, _, _ = fb.build(D, 1e-15)
         return G
 
-    e, D = rhf(S, H, 5, g, enuc, diis=8, density=True)
+    e, D, W = rhf(S, H, 5, g, enuc, diis=8, orbitals=True)
     print(f"RHF/aug-cc-pVDZ rotated water on the GPU Fock build: {e:.12f} (reference {eref})", flush=True)
     assert abs(e - eref) < tol, e
     # two-electron part of the energy gradient (the reference prints it as
@@ -84,6 +84,11 @@ def test_rhf_energy_gpu_augccpvdz(ctx):
     ref = water.forces["F2"]
     print("  2-body forces", F2, flush=True)
     assert np.abs(F2 - ref["values"]).max() < ref["tolerance"], F2 - ref["values"]
+    # 1-body, Pulay and total forces with the one-body derivatives from the GPU
+    S1, T1, V1 = deriv.onebody_deriv1(ctx, la.BasisArrays.from_shells(shells), water.shell2atom, len(atoms),
+                                      [(float(Z), pos) for Z, pos in atoms])
+    dev = forces_check(S1, T1, V1, F2, D, W, atoms)
+    print("  max deviation from the reference's 1-body, Pulay and total forces:", dev, flush=True)
 
 
 def test_rhf_energy_oracle_augccpvdz():
@@ -159,3 +164,50 @@ def test_two_body_force_oracle():
             d2 = (e2(a, x, h / 2) - e2(a, x, -h / 2)) / h
             fd.append((4 * d2 - d1) / 3)
     assert np.abs(np.array(fd) - ref["values"]).max() < 1e-7, np.array(fd) - ref["values"]
+
+
+def _oracle_one_body_fn(aug, charges):
+    """oracle one-body matrices over a Cartesian BasisArrays (the augmented basis)."""
+    off = np.concatenate([[0], np.cumsum(aug.nprim)])
+    sh = [(int(aug.l[s]), aug.alpha[off[s]:off[s + 1]], aug.coeff[off[s]:off[s + 1]], aug.origin[3 * s:3 * s + 3])
+          for s in range(aug.nshell)]
+    return onebody.one_body(sh, [(float(Z), np.asarray(p, float)) for Z, p in charges])
+
+
+def forces_check(S1, T1, V1, F2, D, W, atoms):
+    f = water.forces
+    nx = 3 * len(atoms)
+    F1 = np.array([2 * np.sum((T1[x] + V1[x]) * D) for x in range(nx)])
+    FP = np.array([-2 * np.sum(S1[x] * W) for x in range(nx)])
+    FN = nuclear_repulsion_gradient(atoms)
+    F = F1 + FP + F2 + FN
+    out = {}
+    for name, v in (("F1", F1), ("FP", FP), ("F", F)):
+        out[name] = np.abs(v - f[name]["values"]).max()
+        assert out[name] < f[name]["tolerance"], (name, v - f[name]["values"])
+    return out
+
+
+def test_all_forces_oracle():
+    """The reference's 1-body, Pulay and total Hartree-Fock force goldens
+    (hartree-fock++-validate.py:74-101) from the oracle's one-body and ERI
+    derivatives through deriv.onebody_deriv1 / deriv.fock_deriv1."""
+    import oracle
+    import oracle.deriv1
+    from libint_amd import deriv
+    atoms, shells, enuc, eref, tol = water()
+    S, H = one_body_pure(atoms, shells)
+    arrays = la.BasisArrays.from_shells(shells)
+
+    def g(D):
+        return oracle.fock_2e(arrays, D, precision=1e-15)[0]
+
+    e, D, W = rhf(S, H, 5, g, enuc, diis=8, orbitals=True)
+    assert abs(e - eref) < tol
+    charges = [(float(Z), pos) for Z, pos in atoms]
+    S1, T1, V1 = deriv.onebody_deriv1(None, arrays, water.shell2atom, len(atoms), charges,
+                                      one_body_fn=_oracle_one_body_fn)
+    G1 = deriv.fock_deriv1(None, arrays, water.shell2atom, len(atoms), D, 1e-15,
+                           eri_deriv=lambda a, q, p, s: oracle.deriv1.eri_deriv1(a, q, p, s))
+    F2 = np.array([np.sum(G1[x] * D) for x in range(3 * len(atoms))])
+    forces_check(S1, T1, V1, F2, D, W, atoms)
