@@ -507,6 +507,9 @@ FP32_HRR_FACTOR = 16.0
 # Operator of an ERI plan (l2g.h L2G_OPER_*; libint's Operator::coulomb,
 # erf_coulomb, erfc_coulomb, engine.h:158-167)
 COULOMB, ERF_COULOMB, ERFC_COULOMB = 0, 1, 2
+# one-body operators of the Engine (Operator::overlap / kinetic / nuclear), BraKet "XX"
+OVERLAP, KINETIC, NUCLEAR = "overlap", "kinetic", "nuclear"
+ONE_BODY = (OVERLAP, KINETIC, NUCLEAR)
 
 
 class FockBuilder:
@@ -601,12 +604,19 @@ class Engine:
                "XSXS": (0, None, 1, None)}
 
     def __init__(self, max_nprim: int, max_l: int, precision: float = np.finfo(float).eps,
-                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None, braket: str = "XXXX",
-                 deriv_order: int = 0):
-        if braket not in self.BRAKETS:
-            raise ValueError(f"braket must be one of {sorted(self.BRAKETS)}")
-        if deriv_order not in (0, 1) or (deriv_order and braket != "XXXX"):
-            raise ValueError("deriv_order 0, or 1 for 4-centre integrals")
+                 screening: int = SCREEN_ORIGINAL, ctx: Context | None = None, braket: str | None = None,
+                 deriv_order: int = 0, oper=None):
+        oper = COULOMB if oper is None else oper
+        if oper not in (COULOMB, ERF_COULOMB, ERFC_COULOMB) + ONE_BODY:
+            raise ValueError("oper must be COULOMB, ERF_COULOMB, ERFC_COULOMB, OVERLAP, KINETIC or NUCLEAR")
+        braket = braket or ("XX" if oper in ONE_BODY else "XXXX")
+        if (braket == "XX") != (oper in ONE_BODY) or (braket != "XX" and braket not in self.BRAKETS):
+            raise ValueError(f"braket must be XX for one-body operators, else one of {sorted(self.BRAKETS)}")
+        if deriv_order not in (0, 1) or (deriv_order and (braket != "XXXX" or oper != COULOMB)):
+            raise ValueError("deriv_order 0, or 1 for 4-centre Coulomb integrals")
+        self.oper = oper
+        self.omega = 0.0
+        self.charges = []
         self.braket = braket
         self.deriv_order = deriv_order
         if max_nprim <= 0:
@@ -625,6 +635,15 @@ class Engine:
         self.precision = max(0.0, float(prec))
         return self
 
+    def set_params(self, params) -> "Engine":
+        """omega for ERF_COULOMB / ERFC_COULOMB; the point charges
+        [(q, (x, y, z)), ...] for NUCLEAR (engine.h:685-690)."""
+        if self.oper == NUCLEAR:
+            self.charges = [(float(q), np.asarray(x, float).reshape(3)) for q, x in params]
+        else:
+            self.omega = float(params)
+        return self
+
     def prescale_by(self, s: float) -> "Engine":
         self.scale = float(s)
         return self
@@ -634,6 +653,19 @@ class Engine:
         XSXX and compute(a, b, c) = (ab|c) for XXXS (3-centre); compute(a, c)
         = (a|c) for XSXS (2-centre), the missing centres being Shell.unit()
         as Engine::compute does (engine.impl.h:140-178)."""
+        if self.oper in ONE_BODY:
+            if len(shells) != 2:
+                raise ValueError("one-body operators take 2 shells")
+            if self.oper == NUCLEAR and not self.charges:
+                raise ValueError("NUCLEAR needs the charges (set_params)")
+            for s_ in shells:
+                if s_.l > self.max_l_:
+                    raise LmaxExceeded(-2, "the angular momentum limit is exceeded")
+            S, T, V = one_body(Basis(self.ctx, BasisArrays.from_shells(shells)),
+                               self.charges if self.oper == NUCLEAR else [])
+            M = {OVERLAP: S, KINETIC: T, NUCLEAR: V}[self.oper]
+            n1 = shells[0].size()
+            return np.ascontiguousarray(M[:n1, n1:]).reshape(-1)
         slots = self.BRAKETS[self.braket]
         if len(shells) != sum(k is not None for k in slots):
             raise ValueError(f"{self.braket} takes {sum(k is not None for k in slots)} shells")
@@ -650,8 +682,11 @@ class Engine:
                                      self.screening)
             return [vals[d] for d in range(12)] if nz[0] else None
         b = Basis(self.ctx, BasisArrays.from_shells(quad))
-        vals, _, nz = compute_quartets(b, [[0, 1, 2, 3]], self.precision, self.screening, self.scale,
-                                       self.uniform_cart)
+        p = EriPlan(b, [[0, 1, 2, 3]], self.precision, self.screening, self.scale, self.uniform_cart)
+        if self.oper != COULOMB:
+            p.set_operator(self.oper, self.omega)
+        p.execute()
+        vals, nz = p.download()
         return vals if nz[0] else None
 
     def compute_batch(self, basis: Basis, quartets):

@@ -82,3 +82,29 @@ def test_one_body_gaussian_ordering(ctx):
         off += n
     for a, g in ((S, Sg), (T, Tg), (V, Vg)):
         assert np.array_equal(a[np.ix_(perm, perm)], g)
+
+
+def test_engine_one_body_and_erf(ctx):
+    """The Engine path of the reference's python binding: one-body operators
+    with BraKet XX and the charges through set_params
+    (python/tests/test_libint2.py:32-36 goldens), and erf_coulomb with omega
+    through set_params, against the plan path."""
+    say("case: Engine one-body operators and erf_coulomb")
+    s = la.Shell(0, [(1.0, 10.0)])
+    h2o = [(8.0, (0.0, -0.07579, 0.0)), (1.0, (0.86681, 0.60144, 0.0)), (1.0, (-0.86681, 0.60144, 0.0))]
+    assert abs(np.linalg.norm(la.Engine(1, 0, ctx=ctx, oper=la.OVERLAP).compute(s, s)) - 1.0) < 1e-12
+    assert abs(np.linalg.norm(la.Engine(1, 0, ctx=ctx, oper=la.KINETIC).compute(s, s)) - 1.5) < 1e-12
+    en = la.Engine(1, 0, ctx=ctx, oper=la.NUCLEAR)
+    with pytest.raises(ValueError):
+        en.compute(s, s)
+    assert abs(np.linalg.norm(en.set_params(h2o).compute(s, s)) - 14.54704336519) < 1e-10
+    rng = np.random.default_rng(93)
+    sh = random_shells(rng, [2, 1, 0, 2], kmax=2)
+    v = la.Engine(2, 2, ctx=ctx, oper=la.ERF_COULOMB).set_params(0.4).compute(*sh)
+    plan = la.EriPlan(la.Basis(ctx, la.BasisArrays.from_shells(sh)), [[0, 1, 2, 3]])
+    plan.set_operator(la.ERF_COULOMB, 0.4)
+    plan.execute()
+    w, _ = plan.download()
+    assert np.array_equal(v, w)
+    with pytest.raises(ValueError):
+        la.Engine(1, 0, ctx=ctx, oper=la.OVERLAP, deriv_order=1)
