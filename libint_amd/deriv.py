@@ -115,6 +115,8 @@ def onebody_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, charges, one_bo
     from . import Basis, one_body, solid_harmonics_coeff
     n = arrays.nshell
     L = arrays.l.astype(int)
+    if len(charges) != natoms:
+        raise ValueError("onebody_deriv1 takes the nuclei as the charges, one per atom in atom order")
     if L.max(initial=0) > 2:
         raise ValueError("one-body derivatives are provided up to l = 2 (raised shells up to 3)")
     off = np.concatenate([[0], np.cumsum(arrays.nprim)])
