@@ -56,8 +56,8 @@ def fock_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, D, precision=np.fi
     bf = np.concatenate([[0], np.cumsum(sizes)])
     nbf = int(bf[-1])
     D = np.asarray(D, float)
-    quartets = [(s1, s2, s3, s4) for s1 in range(ns) for s2 in range(s1 + 1) for s3 in range(s1 + 1)
-                for s4 in range((s2 if s1 == s3 else s3) + 1)]
+    quartets = [(a, b, c, d) for a in range(ns) for b in range(a + 1) for c in range(a + 1)
+                for d in range((b if a == c else c) + 1)]
     q = np.array(quartets, np.int32).reshape(-1, 4)
     if eri_deriv is None:
         vals, off, nz = eri_deriv1(ctx, arrays, q, precision, screening)
@@ -65,25 +65,37 @@ def fock_deriv1(ctx, arrays: BasisArrays, shell2atom, natoms, D, precision=np.fi
         vals, off, nz = eri_deriv(arrays, q, precision, screening)
     atom = np.asarray(shell2atom, np.int64)
     G = np.zeros((3 * natoms, nbf, nbf))
-    for i, (s1, s2, s3, s4) in enumerate(quartets):
-        if not nz[i]:
-            continue
-        n = [int(sizes[s]) for s in (s1, s2, s3, s4)]
-        r = [slice(bf[s], bf[s] + sizes[s]) for s in (s1, s2, s3, s4)]
-        deg = (1.0 if s1 == s2 else 2.0) * (1.0 if s3 == s4 else 2.0) * \
-            (1.0 if (s1 == s3 and s2 == s4) else 2.0)
-        blk = vals[:, off[i]:off[i] + int(np.prod(n))].reshape([12] + n)
-        coord = np.array([3 * atom[(s1, s2, s3, s4)[c // 3]] + c % 3 for c in range(12)])
-        uc, inv = np.unique(coord, return_inverse=True)
-        V = np.zeros([len(uc)] + n)
-        np.add.at(V, inv, blk)
-        V *= deg
-        G[uc, r[0], r[1]] += np.einsum("xabcd,cd->xab", V, D[r[2], r[3]])
-        G[uc, r[2], r[3]] += np.einsum("xabcd,ab->xcd", V, D[r[0], r[1]])
-        G[uc, r[0], r[2]] -= 0.25 * np.einsum("xabcd,bd->xac", V, D[r[1], r[3]])
-        G[uc, r[1], r[3]] -= 0.25 * np.einsum("xabcd,ac->xbd", V, D[r[0], r[2]])
-        G[uc, r[0], r[3]] -= 0.25 * np.einsum("xabcd,bc->xad", V, D[r[1], r[2]])
-        G[uc, r[1], r[2]] -= 0.25 * np.einsum("xabcd,ad->xbc", V, D[r[0], r[3]])
+    if len(q) == 0:
+        return G
+    s1, s2, s3, s4 = (q[:, k].astype(np.int64) for k in range(4))
+    deg = np.where(s1 == s2, 1.0, 2.0) * np.where(s3 == s4, 1.0, 2.0) * \
+        np.where((s1 == s3) & (s2 == s4), 1.0, 2.0)
+    dims = sizes[q]
+    key = ((dims[:, 0] * 16 + dims[:, 1]) * 16 + dims[:, 2]) * 16 + dims[:, 3]
+    keep = nz != 0
+    # quartets of equal block shape in one batch: the six scatter terms as
+    # batched contractions, added into G with np.add.at
+    for kv in np.unique(key[keep]):
+        idx = np.nonzero((key == kv) & keep)[0]
+        n = [int(x) for x in dims[idx[0]]]
+        m = len(idx)
+        blk = vals[:, off[idx][:, None] + np.arange(int(np.prod(n)))]  # (12, m, size)
+        V = blk.transpose(1, 0, 2).reshape([m, 12] + n) * deg[idx][:, None, None, None, None, None]
+        r = [bf[sk[idx]][:, None] + np.arange(nk) for sk, nk in zip((s1, s2, s3, s4), n)]
+        coord = 3 * atom[q[idx][:, np.arange(12) // 3]] + np.arange(12) % 3  # (m, 12)
+
+        def Dblk(i, j):
+            return D[r[i][:, :, None], r[j][:, None, :]]  # (m, n_i, n_j)
+
+        def add(t, i, j, scale):
+            np.add.at(G, (coord[:, :, None, None], r[i][:, None, :, None], r[j][:, None, None, :]), scale * t)
+
+        add(np.einsum("mxabcd,mcd->mxab", V, Dblk(2, 3)), 0, 1, 1.0)
+        add(np.einsum("mxabcd,mab->mxcd", V, Dblk(0, 1)), 2, 3, 1.0)
+        add(np.einsum("mxabcd,mbd->mxac", V, Dblk(1, 3)), 0, 2, -0.25)
+        add(np.einsum("mxabcd,mac->mxbd", V, Dblk(0, 2)), 1, 3, -0.25)
+        add(np.einsum("mxabcd,mbc->mxad", V, Dblk(1, 2)), 0, 3, -0.25)
+        add(np.einsum("mxabcd,mad->mxbc", V, Dblk(0, 3)), 1, 2, -0.25)
     return 0.5 * (G + G.transpose(0, 2, 1))
 
 
