@@ -42,32 +42,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def schwarz_pairs(la, B, shells):
-    """K(s1,s2) = sqrt(max|(s1 s2|s1 s2)|) for s1 >= s2 in `shells`, no
-    primitive screening (tests/hartree-fock/hartree-fock++.cc:1231-1298),
-    computed on the GPU by the Schwarz kernels."""
-    i, j = np.tril_indices(shells.size)
-    p1, p2 = shells[i], shells[j]
-    K = la.schwarz(B, np.stack([p1, p2], 1), precision=0.0)
-    return p1, p2, K
-
-
-def screened_quartets(p1, p2, K, thresh):
-    """Unique quartets over the pair list (pair index a >= b) with
-    K_a K_b >= thresh."""
-    order = np.lexsort((p2, p1))  # loop order s1, then s2
-    p1, p2, K = p1[order], p2[order], K[order]
-    n = K.size
-    chunks = []
-    for a in range(n):
-        if K[a] == 0:
-            continue
-        b = np.nonzero(K[:a + 1] * K[a] >= thresh)[0]
-        if b.size:
-            chunks.append(np.stack([np.full(b.size, p1[a]), np.full(b.size, p2[a]), p1[b], p2[b]], 1))
-    return np.concatenate(chunks).astype(np.int32)
-
-
 def cpu_baseline(B, quartets, seed=5, target_s=10.0):
     import oracle
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
@@ -359,10 +333,7 @@ def main():
     mol = molecule.make_molecule(2)
     ctx = la.Context(0)
     B = la.Basis(ctx, mol.basis)
-    dsh = np.nonzero(mol.basis.l == 2)[0]
-    p1, p2, K = schwarz_pairs(la, B, dsh)
-    q = screened_quartets(p1, p2, K, 1e-12)
-    n_unique = K.size * (K.size + 1) // 2
+    q, n_unique = molecule.headline_quartets(B, 1e-12)
     if args.max_quartets and len(q) > args.max_quartets:
         q = q[: args.max_quartets]
     log(f"workload: {len(q)} screened (dd|dd) quartets of {n_unique} unique")

@@ -18,6 +18,7 @@ The objects mirror libint's C++ API for this path:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import subprocess
@@ -112,6 +113,36 @@ class ServerConfig(ctypes.Structure):
                 ("max_engine_precision", ctypes.c_double)]
 
 _lib = None
+
+# Every Python object that owns a native handle (servers, plans, Fock
+# builders, bases, contexts).  At interpreter exit _release_all() frees them
+# in dependency order while the library and the HIP runtime are still whole;
+# finalizers that run after it find nothing left to free.
+_live = weakref.WeakSet()
+_exiting = False
+
+
+def _release_all() -> None:
+    """Frees every live native handle: servers (their threads use the device)
+    first, then plans and Fock builders, then bases, then contexts.  Runs
+    once, from atexit, before the interpreter tears its modules down."""
+    global _exiting
+    if _exiting:
+        return
+    objs = list(_live)
+    rank = {Server: 0, Basis: 2, Context: 3}
+    for o in sorted(objs, key=lambda o: rank.get(type(o), 1)):
+        try:
+            if isinstance(o, (Server, Context)):
+                o.close()
+            else:
+                o._release()
+        except Exception:
+            pass
+    _exiting = True
+
+
+atexit.register(_release_all)
 
 
 class L2gError(RuntimeError):
@@ -263,6 +294,7 @@ class Context:
         self._h = h
         # live bases, plans and Fock builders: released before the context
         self._children = weakref.WeakSet()
+        _live.add(self)
 
     @property
     def handle(self):
@@ -309,6 +341,7 @@ class Basis:
                                       ctypes.byref(h)))
         self._h = h
         ctx._children.add(self)
+        _live.add(self)
 
     @property
     def handle(self):
@@ -346,6 +379,7 @@ class EriPlan:
                                              int(uniform_cart), ctypes.byref(h)))
         self._h = h
         basis.ctx._children.add(self)
+        _live.add(self)
         n = ctypes.c_int64()
         _check(lib().l2g_eri_plan_size(h, ctypes.byref(n)))
         self.size = n.value
@@ -449,6 +483,7 @@ class Deriv1Plan:
                                                 ctypes.byref(h)))
         self._h = h
         basis.ctx._children.add(self)
+        _live.add(self)
         n = ctypes.c_int64()
         _check(lib().l2g_eri_deriv1_plan_size(h, ctypes.byref(n)))
         self.size = n.value
@@ -524,6 +559,7 @@ class FockBuilder:
         _check(lib().l2g_fock_create(basis.handle, pair_threshold, max_engine_precision, screening, ctypes.byref(h)))
         self._h = h
         basis.ctx._children.add(self)
+        _live.add(self)
         n = ctypes.c_int()
         _check(lib().l2g_fock_nbf(h, ctypes.byref(n)))
         self.nbf = n.value
@@ -591,6 +627,18 @@ def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, scre
     return vals, offs, nz
 
 
+_default_ctx = None
+
+
+def default_context() -> "Context":
+    """The process's shared context on device 0, opened on first use (the
+    Engine's default when no context is given); released at exit."""
+    global _default_ctx
+    if _default_ctx is None or not _default_ctx._h:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
 class Engine:
     """Coulomb ``libint2::Engine`` for this path (engine.h:502-526).
 
@@ -625,7 +673,7 @@ class Engine:
             raise LmaxExceeded(-2, f"max_l {max_l} exceeds the compiled limit {globals()['max_l']()}")
         self.max_nprim = max_nprim
         self.max_l_ = max_l
-        self.ctx = ctx or Context(0)
+        self.ctx = ctx or default_context()
         self.set_precision(precision)
         self.screening = screening
         self.scale = 1.0
@@ -665,7 +713,8 @@ class Engine:
                                self.charges if self.oper == NUCLEAR else [])
             M = {OVERLAP: S, KINETIC: T, NUCLEAR: V}[self.oper]
             n1 = shells[0].size()
-            return np.ascontiguousarray(M[:n1, n1:]).reshape(-1)
+            # prescale_by applies to one-body integrals too (engine.impl.h:1050)
+            return self.scale * np.ascontiguousarray(M[:n1, n1:]).reshape(-1)
         slots = self.BRAKETS[self.braket]
         if len(shells) != sum(k is not None for k in slots):
             raise ValueError(f"{self.braket} takes {sum(k is not None for k in slots)} shells")
@@ -676,6 +725,8 @@ class Engine:
                 raise ValueError("shell has more primitives than max_nprim")
         quad = [shells[k] if k is not None else Shell.unit() for k in slots]
         if self.deriv_order == 1:
+            if self.scale != 1.0 or self.uniform_cart:
+                raise ValueError("deriv_order 1 supports neither prescale_by nor uniform Cartesian normalization")
             # 12 shell sets, d/dA_x .. d/dD_z (libint's results()[0..11])
             from .deriv import eri_deriv1
             vals, _, nz = eri_deriv1(self.ctx, BasisArrays.from_shells(quad), [[0, 1, 2, 3]], self.precision,
@@ -706,6 +757,7 @@ class Server:
         _check(lib().l2g_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self._nbf = {}
+        _live.add(self)
 
     def submit(self, arrays: BasisArrays, D, precision: float = 1e-12, fp32_threshold: float = 0.0) -> int:
         a = arrays

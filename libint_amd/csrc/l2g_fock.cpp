@@ -873,7 +873,7 @@ int l2g_fock_class_stats(l2g_fock f, int cap, int* nclasses, int* class_code, in
 }
 
 int l2g_fock_destroy(l2g_fock f) {
-  if (f) {
+  if (f && !g_exiting) {
     l2g_context_s* ctx = f->setup.B->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
@@ -991,6 +991,16 @@ struct l2g_server_s {
 };
 
 namespace {
+
+// live servers, so that the exit handler can stop their threads
+std::mutex g_servers_mu;
+std::vector<l2g_server_s*> g_servers;
+
+void server_registry(l2g_server_s* S, bool add) {
+  std::lock_guard<std::mutex> lk(g_servers_mu);
+  if (add) g_servers.push_back(S);
+  else g_servers.erase(std::remove(g_servers.begin(), g_servers.end(), S), g_servers.end());
+}
 
 // marks a request failed and wakes waiters; caller holds S->mu
 void fail_locked(l2g_server_s* S, ServerRequest* r, int code, const std::string& msg) {
@@ -1345,6 +1355,7 @@ int l2g_server_create(const l2g_server_config* config, l2g_server* out) {
     l2g_server_s* p = S.get();
     for (int k = 0; k < 4; ++k) p->admit_thr.emplace_back(admit_loop, p);
     p->exec_thr = std::thread(exec_loop, p);
+    server_registry(p, true);
     *out = S.release();
   });
 }
@@ -1454,8 +1465,29 @@ int l2g_server_stats(l2g_server S, int64_t* rounds, int64_t* max_requests_per_ro
   });
 }
 
+// At process exit: the threads of every server not destroyed by its owner
+// finish the requests in flight and stop, before the HIP runtime goes.
+extern "C++" void l2g_impl::stop_servers_at_exit() {
+  std::vector<l2g_server_s*> live;
+  {
+    std::lock_guard<std::mutex> lk(g_servers_mu);
+    live.swap(g_servers);
+  }
+  for (l2g_server_s* S : live) {
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stop = true;
+      S->cv.notify_all();
+    }
+    for (auto& t : S->admit_thr)
+      if (t.joinable()) t.join();
+    if (S->exec_thr.joinable()) S->exec_thr.join();
+  }
+}
+
 int l2g_server_destroy(l2g_server S) {
-  if (!S) return L2G_OK;
+  if (!S || g_exiting) return L2G_OK;
+  server_registry(S, false);
   {
     std::lock_guard<std::mutex> lk(S->mu);
     S->stop = true;

@@ -225,12 +225,23 @@ int l2g_boys_table(double* out, int* nint, int* mmax) {
   });
 }
 
+extern "C++" void l2g_impl::register_exit_handler() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::atexit([] {
+      l2g_impl::stop_servers_at_exit();
+      l2g_impl::g_exiting = true;
+    });
+  });
+}
+
 int l2g_init(int device, l2g_context* out) {
   return guarded([&] {
     if (!out) throw std::invalid_argument("l2g_init: null output");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw L2gError(L2G_E_NODEVICE, "no HIP device available");
     if (device < 0 || device >= n) throw std::invalid_argument("l2g_init: device index out of range");
+    register_exit_handler();  // after the HIP runtime has initialised, so it runs before its teardown
     auto ctx = std::make_unique<l2g_context_s>();
     ctx->device = device;
     HIPCHK(hipSetDevice(device));
@@ -303,7 +314,7 @@ int l2g_init(int device, l2g_context* out) {
 
 int l2g_finalize(l2g_context ctx) {
   return guarded([&] {
-    if (!ctx) return;
+    if (!ctx || g_exiting) return;
     if (ctx->live > 0)
       throw L2gError(L2G_E_STATE, std::to_string(ctx->live.load()) +
                                       " basis/plan/Fock object(s) of this context are still alive; destroy them first");
@@ -369,7 +380,7 @@ int l2g_basis_create(l2g_context ctx, int nshell, const int* l, const int* pure,
 }
 
 int l2g_basis_destroy(l2g_basis b) {
-  delete b;
+  if (!g_exiting) delete b;
   return L2G_OK;
 }
 
@@ -893,7 +904,7 @@ int l2g_eri_compute(l2g_basis B, int64_t nq, const int32_t* quartets, double pre
 }
 
 int l2g_eri_plan_destroy(l2g_eri_plan p) {
-  if (p) {
+  if (p && !g_exiting) {
     l2g_context_s* ctx = p->basis->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
@@ -1250,7 +1261,7 @@ int l2g_eri_deriv1_plan_download(l2g_deriv1_plan p, double* host_out, int8_t* no
 }
 
 int l2g_eri_deriv1_plan_destroy(l2g_deriv1_plan p) {
-  if (p) {
+  if (p && !g_exiting) {
     l2g_context_s* ctx = p->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);

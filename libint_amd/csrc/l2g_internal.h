@@ -47,6 +47,17 @@ namespace l2g_impl {
 
 inline thread_local std::string g_err;
 
+// Process teardown.  The first l2g_init registers an atexit handler; atexit
+// handlers run in reverse order of registration, so it runs before the HIP
+// runtime (loaded, and registered, earlier) tears itself down.  The handler
+// stops the threads of every live request server and sets g_exiting; from
+// then on the destroy entry points and the device-buffer destructors touch
+// nothing on the device: the process is ending and its device resources go
+// with it.
+inline std::atomic<bool> g_exiting{false};
+void register_exit_handler();  // l2g_host.cpp
+void stop_servers_at_exit();   // l2g_fock.cpp
+
 struct L2gError : std::runtime_error {
   int code;
   L2gError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -132,7 +143,7 @@ struct DevBuf {
     return *this;
   }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !g_exiting) (void)hipFree(p);
   }
   void alloc(size_t b) {
     if (p && bytes >= b) return;

@@ -120,3 +120,83 @@ def quartet_count(nshell: int) -> int:
 
 def class_shells(basis: BasisArrays, l: int) -> np.ndarray:
     return np.nonzero(basis.l == l)[0]
+
+
+def screened_quartets(p1: np.ndarray, p2: np.ndarray, K: np.ndarray, thresh: float) -> np.ndarray:
+    """Permutationally-unique quartets over a shell-pair list (pair index
+    a >= b in the (s1, s2) loop order) whose Schwarz bound K_a K_b is at least
+    thresh, as an (n, 4) int32 array."""
+    order = np.lexsort((p2, p1))  # loop order s1, then s2
+    p1, p2, K = p1[order], p2[order], K[order]
+    chunks = []
+    for a in range(K.size):
+        if K[a] == 0:
+            continue
+        b = np.nonzero(K[:a + 1] * K[a] >= thresh)[0]
+        if b.size:
+            chunks.append(np.stack([np.full(b.size, p1[a]), np.full(b.size, p2[a]), p1[b], p2[b]], 1))
+    if not chunks:
+        return np.zeros((0, 4), np.int32)
+    return np.concatenate(chunks).astype(np.int32)
+
+
+def headline_quartets(basis, thresh: float = 1e-12):
+    """The (dd|dd) work list of the 60-atom batched workload: every
+    permutationally-unique quartet of d shells whose Schwarz bound
+    K12 K34 >= thresh, with K(s1, s2) = sqrt(max|(s1 s2|s1 s2)|) computed on
+    the GPU without primitive screening (compute_schwarz_ints,
+    tests/hartree-fock/hartree-fock++.cc:1231-1298).  `basis` is a device
+    Basis.  Returns (quartets, number of unique (dd|dd) quartets before
+    screening)."""
+    from . import schwarz
+    dsh = class_shells(basis.arrays, 2)
+    i, j = np.tril_indices(dsh.size)
+    p1, p2 = dsh[i], dsh[j]
+    K = schwarz(basis, np.stack([p1, p2], 1), precision=0.0)
+    return screened_quartets(p1, p2, K, thresh), K.size * (K.size + 1) // 2
+
+
+def stratified_quartets(arrays: BasisArrays, pairs: np.ndarray, K: np.ndarray, per_class: int,
+                        rng: np.random.Generator, thresh: float = 1e-12, shuffle_order: bool = True,
+                        per_class_of=None) -> np.ndarray:
+    """Seeded sample of quartets (ab|cd) whose Schwarz bound K_ab K_cd is at
+    least thresh, stratified by class: for every combination of a bra pair
+    type and a ket pair type (pair type = the two l of the pair), up to
+    per_class quartets (per_class_of(la, lb, lc, ld) overrides it) drawn
+    uniformly among the significant ones.  With shuffle_order each quartet
+    comes in a random caller order (bra/ket and within-pair swaps), so that
+    the reordering and un-permuting are exercised too.  pairs: (n, 2) shell
+    pairs, K their Schwarz factors."""
+    pairs = np.asarray(pairs)
+    lp = np.sort(arrays.l[pairs], axis=1)[:, ::-1]
+    types = sorted({tuple(t) for t in lp.tolist()})
+    members = {t: np.nonzero((lp[:, 0] == t[0]) & (lp[:, 1] == t[1]))[0] for t in types}
+    out = []
+    for ti, t1 in enumerate(types):
+        for t2 in types[ti:]:
+            want = per_class if per_class_of is None else per_class_of(t1[0], t1[1], t2[0], t2[1])
+            if want <= 0:
+                continue
+            A, Bm = members[t1], members[t2]
+            got = []
+            n = 0
+            for _ in range(50):
+                a = A[rng.integers(0, A.size, 4 * want)]
+                b = Bm[rng.integers(0, Bm.size, 4 * want)]
+                ok = K[a] * K[b] >= thresh
+                got.append(np.stack([a[ok], b[ok]], 1))
+                n += int(ok.sum())
+                if n >= want:
+                    break
+            ab = np.concatenate(got)[:want]
+            if ab.size:
+                out.append(np.concatenate([pairs[ab[:, 0]], pairs[ab[:, 1]]], 1))
+    q = np.concatenate(out).astype(np.int32) if out else np.zeros((0, 4), np.int32)
+    if shuffle_order and len(q):
+        sb = rng.integers(0, 2, len(q)).astype(bool)
+        sk = rng.integers(0, 2, len(q)).astype(bool)
+        q[sb, 0], q[sb, 1] = q[sb, 1].copy(), q[sb, 0].copy()
+        q[sk, 2], q[sk, 3] = q[sk, 3].copy(), q[sk, 2].copy()
+        bk = rng.integers(0, 2, len(q)).astype(bool)
+        q[bk] = q[bk][:, [2, 3, 0, 1]]
+    return q

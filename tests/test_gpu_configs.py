@@ -1,0 +1,263 @@
+"""Parity on the five benchmark workloads themselves (the molecules of
+libint_amd.molecule at their stated sizes), against the oracle's restatement
+of Engine::compute2 / compute_2body_fock.
+
+* 60-atom batched workload: every one of the (dd|dd) quartets the benchmark
+  times (the full screened list, one plan, as bench.py runs it), and a
+  stratified sample over every class up to (dd|dd) in random caller order.
+* 120-atom f recipe: a stratified sample over every class up to (ff|ff), FP64
+  at the per-integral bar, and the mixed FP32/FP64 plan with its error per
+  class printed and held to the FP32 bound.
+* 200-atom J/K recipe: the Fock build at the recipe's atom density and the
+  largest size the oracle digests in about a minute, FP64 and mixed, <= 1e-8.
+* serving: Fock-build requests of f-recipe molecules of 10 and more atoms,
+  each equal alone and in the mix, and against the oracle.
+
+Per-integral bar: 1e-12 absolute / 1e-10 relative, failing only if both are
+exceeded (tests/eri/test.cc:77-88); where the oracle itself misses the exact
+value (HRR cancellation), the long-double brute force settles it
+(tests.helpers.compare_arbitrated)."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import libint_amd as la
+import oracle
+from libint_amd import molecule
+from tests.helpers import compare_arbitrated
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = float(np.finfo(np.float32).eps)
+THR = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def density(nbf, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.normal(0.0, 0.1, (nbf, max(1, nbf // 5)))
+    return C @ C.T
+
+
+def all_pairs(b):
+    n = b.arrays.nshell
+    i, j = np.tril_indices(n)
+    pairs = np.stack([i, j], 1).astype(np.int32)
+    return pairs, la.schwarz(b, pairs, precision=0.0)
+
+
+def check_batch(arrays, q, g, offs, label):
+    """Per-integral bar against the oracle, with brute-force arbitration;
+    prints the per-class maxima."""
+    t0 = time.perf_counter()
+    o, ooff, onz = oracle.eri_quartets(arrays, q)
+    t_or = time.perf_counter() - t0
+    assert np.array_equal(offs, ooff)
+    sizes = arrays.sizes()
+    n = sizes[q[:, 0]] * sizes[q[:, 1]] * sizes[q[:, 2]] * sizes[q[:, 3]]
+    cls = np.array(["".join(map(str, r)) for r in np.sort(arrays.l[q][:, :2], 1)[:, ::-1].tolist()])
+    cls = np.char.add(cls, np.array(["".join(map(str, r)) for r in np.sort(arrays.l[q][:, 2:], 1)[:, ::-1].tolist()]))
+    d = np.abs(g - o)
+    for c in sorted(set(cls.tolist())):
+        m = np.repeat(cls == c, n)
+        say(f"  {label} ({c[:2]}|{c[2:]}) x{int((cls == c).sum())}: max abs {d[m].max():.2e}")
+    nfail, narb, oerr = compare_arbitrated(arrays, q, offs, g, o, 1e-12, 1e-10)
+    say(f"  {label}: {len(q)} quartets, {g.size} integrals, max abs {d.max():.2e}; oracle {t_or:.1f} s; "
+        f"fails {nfail}, {narb} values arbitrated by brute force (oracle's own error up to {oerr:.1e})")
+    return nfail
+
+
+@pytest.mark.timeout(900)
+def test_config2_headline_dddd_full(ctx):
+    """All (dd|dd) quartets of the benchmark's work list, evaluated by one
+    plan exactly as the benchmark step runs it, against the oracle."""
+    mol = molecule.make_molecule(2)
+    b = la.Basis(ctx, mol.basis)
+    q, nuniq = molecule.headline_quartets(b)
+    say(f"case: 60-atom workload, headline (dd|dd) list, {len(q)} of {nuniq} unique quartets")
+    plan = la.EriPlan(b, q)
+    offs = plan.offsets()
+    plan.execute()
+    g, nz = plan.download()
+    st = plan.stats()
+    assert len(st) == 1 and st[0]["cls"] == "2222"
+    nfail, worst, narb = 0, 0.0, 0
+    chunk = 150000
+    t0 = time.perf_counter()
+    for a in range(0, len(q), chunk):
+        qc = q[a:a + chunk]
+        lo = offs[a]
+        hi = offs[a + chunk] if a + chunk < len(q) else g.size
+        o, ooff, onz = oracle.eri_quartets(mol.basis, qc)
+        assert np.array_equal(ooff + lo, offs[a:a + chunk])
+        assert np.array_equal(onz, nz[a:a + chunk])
+        gc = g[lo:hi]
+        worst = max(worst, float(np.abs(gc - o).max()))
+        f, k, _ = compare_arbitrated(mol.basis, qc, ooff, gc, o, 1e-12, 1e-10)
+        nfail += f
+        narb += k
+    say(f"  {len(q)} quartets, {g.size} integrals vs oracle in {time.perf_counter() - t0:.1f} s: "
+        f"max abs {worst:.2e}, fails {nfail}, {narb} values arbitrated by brute force")
+    assert nfail == 0
+
+
+def test_config2_all_classes_sample(ctx):
+    """At least 1000 quartets of every class up to (dd|dd) of the 60-atom
+    molecule (K_ab K_cd >= 1e-12), random caller orders."""
+    mol = molecule.make_molecule(2)
+    b = la.Basis(ctx, mol.basis)
+    pairs, K = all_pairs(b)
+    rng = np.random.default_rng(202)
+    q = molecule.stratified_quartets(mol.basis, pairs, K, 1200, rng)
+    ncls = len({tuple(np.sort(r[:2]).tolist() + np.sort(r[2:]).tolist()) for r in mol.basis.l[q].tolist()})
+    say(f"case: 60-atom workload, stratified sample, {len(q)} quartets over {ncls} pair-type combinations")
+    assert len(q) >= 20000
+    plan = la.EriPlan(b, q)
+    offs = plan.offsets()
+    plan.execute()
+    g, _ = plan.download()
+    assert check_batch(mol.basis, q, g, offs, "fp64") == 0
+
+
+def _f_per_class(la_, lb, lc, ld):
+    L = la_ + lb + lc + ld
+    return 400 if L <= 6 else (200 if L <= 8 else (80 if L <= 10 else 30))
+
+
+def test_config3_f_sample_fp64_and_mixed(ctx):
+    """120-atom f recipe (K up to 8): every class up to (ff|ff), FP64 at the
+    per-integral bar; the mixed plan (FP32 where K_ab K_cd < 1e-6 and
+    S_ab S_cd < 16e-6) with its per-class errors against the FP64 oracle."""
+    mol = molecule.make_molecule(3)
+    B = mol.basis
+    b = la.Basis(ctx, B)
+    pairs, K = all_pairs(b)
+    rng = np.random.default_rng(303)
+    q = molecule.stratified_quartets(B, pairs, K, 0, rng, per_class_of=_f_per_class)
+    say(f"case: 120-atom f recipe, stratified sample of {len(q)} quartets, classes to (ff|ff)")
+    assert len(q) >= 10000
+    plan = la.EriPlan(b, q)
+    offs = plan.offsets()
+    plan.execute()
+    g, _ = plan.download()
+    o, ooff, _ = oracle.eri_quartets(B, q)
+    assert np.array_equal(offs, ooff)
+    nfail, narb, oerr = compare_arbitrated(B, q, offs, g, o, 1e-12, 1e-10)
+    say(f"  fp64: max abs {np.abs(g - o).max():.2e}, fails {nfail}, {narb} values arbitrated "
+        f"(oracle's own error up to {oerr:.1e})")
+    assert nfail == 0
+    # mixed precision: per-class table
+    mplan = la.EriPlan(b, q, fp32_threshold=THR)
+    mplan.execute()
+    gm, _ = mplan.download()
+    n32 = sum(s["nquartets"] for s in mplan.stats() if s["fp32"])
+    Kv, Sv = la.precision_bounds(b, pairs)
+    Km = np.zeros((B.nshell, B.nshell))
+    Sm = np.zeros_like(Km)
+    Km[pairs[:, 0], pairs[:, 1]] = Km[pairs[:, 1], pairs[:, 0]] = Kv
+    Sm[pairs[:, 0], pairs[:, 1]] = Sm[pairs[:, 1], pairs[:, 0]] = Sv
+    KK = Km[q[:, 0], q[:, 1]] * Km[q[:, 2], q[:, 3]]
+    SS = Sm[q[:, 0], q[:, 1]] * Sm[q[:, 2], q[:, 3]]
+    sel = (KK < THR) & (SS < la.FP32_HRR_FACTOR * THR) & (B.l[q].max(axis=1) <= la.fp32_max_l())
+    assert n32 == int(sel.sum())
+    sizes = B.sizes()
+    n = sizes[q[:, 0]] * sizes[q[:, 1]] * sizes[q[:, 2]] * sizes[q[:, 3]]
+    ends = offs + n
+    cls = np.array(["".join(map(str, np.sort(r[:2])[::-1])) + "".join(map(str, np.sort(r[2:])[::-1]))
+                    for r in B.l[q]])
+    say(f"  mixed: {n32} of {len(q)} quartets in FP32")
+    for c in sorted(set(cls.tolist())):
+        for part, msk in (("fp64", (cls == c) & ~sel), ("fp32", (cls == c) & sel)):
+            if not msk.any():
+                continue
+            gg = np.concatenate([gm[a:e] for a, e in zip(offs[msk], ends[msk])])
+            oo = np.concatenate([o[a:e] for a, e in zip(offs[msk], ends[msk])])
+            big = np.abs(oo) > 1e-14
+            rel = (np.abs(gg - oo)[big] / np.abs(oo[big])).max(initial=0)
+            say(f"  mixed ({c[:2]}|{c[2:]}) {part} x{int(msk.sum())}: max abs {np.abs(gg - oo).max():.2e}, "
+                f"max rel (|ref|>1e-14) {rel:.2e}")
+    # FP64 quartets of the mixed plan are the FP64 plan's values
+    m64 = np.repeat(~sel, n)
+    assert np.array_equal(gm[m64], g[m64])
+    if sel.any():
+        e32 = np.array([np.abs(gm[a:e] - o[a:e]).max() for a, e in zip(offs[sel], ends[sel])])
+        assert e32.max() <= 1e-5 * THR
+        assert (e32 <= 64 * EPS32 * SS[sel] + 1e-16).all()
+
+
+CONFIG4_ATOMS = 10
+
+
+@pytest.fixture(scope="module")
+def fock_f_reference():
+    """The 200-atom J/K recipe (s4p3d2f1, K up to 8, exponents [0.03, 2e4],
+    D = C C^T with C ~ N(0, 0.1) of N_bf x N_bf/5) at its atom density and
+    10 atoms, the J/K build the benchmark's f-recipe Fock leg times (same
+    molecule and density), with the oracle's compute_2body_fock of it."""
+    mol = molecule.make_molecule(4, natoms=CONFIG4_ATOMS, same_density=True)
+    D = density(mol.nbf, 12)
+    t0 = time.perf_counter()
+    ref = oracle.fock_2e(mol.basis, D, 1e-12)
+    return mol, D, ref, time.perf_counter() - t0
+
+
+@pytest.mark.timeout(900)
+def test_config4_fock_recipe_vs_oracle(ctx, fock_f_reference):
+    """FP64 and mixed G, J, K of the f-recipe J/K build within 1e-8 of the
+    oracle (about a minute of oracle time on the host's cores)."""
+    mol, D, (Go, Jo, Ko, nqo), t_or = fock_f_reference
+    b = la.Basis(ctx, mol.basis)
+    fb = la.FockBuilder(b)
+    say(f"case: 200-atom J/K recipe at {CONFIG4_ATOMS} atoms "
+        f"({molecule.box_edge(4, CONFIG4_ATOMS):.2f} A cube), nbf {mol.nbf}")
+    G, J, K = fb.build(D, 1e-12)
+    st64 = fb.stats()
+    fb.set_fp32_threshold(THR)
+    Gm, Jm, Km = fb.build(D, 1e-12)
+    stm = fb.stats()
+    say(f"  quartets gpu {st64['nquartets']} oracle {nqo}; FP32 in mixed {stm['nquartets_fp32']}; "
+        f"gpu kernels {st64['kernel_ms']:.1f} / {stm['kernel_ms']:.1f} ms, oracle {t_or:.1f} s")
+    say(f"  fp64: max|G-Go| {np.abs(G - Go).max():.2e} |J-Jo| {np.abs(J - Jo).max():.2e} "
+        f"|K-Ko| {np.abs(K - Ko).max():.2e}")
+    say(f"  mixed: max|G-Go| {np.abs(Gm - Go).max():.2e} |J-Jo| {np.abs(Jm - Jo).max():.2e} "
+        f"|K-Ko| {np.abs(Km - Ko).max():.2e}")
+    for X, Xo in ((G, Go), (J, Jo), (K, Ko), (Gm, Go), (Jm, Jo), (Km, Ko)):
+        assert np.abs(X - Xo).max() <= 1e-8
+    assert np.allclose(G, G.T, atol=1e-14)
+
+
+@pytest.mark.timeout(900)
+def test_config5_serving_f_recipe(fock_f_reference):
+    """Fock-build requests of f-recipe molecules (10, 12 and 14 atoms at the
+    200-atom recipe's density), submitted together: each result equals the
+    same request alone, and the 10-atom one equals the oracle."""
+    mol0, D0, (Go, Jo, Ko, _), _ = fock_f_reference
+    reqs = [(mol0.basis, D0)]
+    for k, n in enumerate((12, 14)):
+        mol = molecule.make_molecule(4, natoms=n, seed=5000 + k, same_density=True)
+        reqs.append((mol.basis, density(mol.nbf, 5100 + k)))
+    say(f"case: serving, f-recipe requests of 10, 12, 14 atoms, nbf {[D.shape[0] for _, D in reqs]}")
+    srv = la.Server(0, round_quartets=1 << 18)
+    try:
+        alone = [srv.result(srv.submit(B, D, 1e-12)) for B, D in reqs]
+        ids = [srv.submit(B, D, 1e-12) for B, D in reqs]
+        mix = [srv.result(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    say(f"  most requests in one round {st['max_requests_per_round']}; latency alone "
+        f"{[round(a[3][1]) for a in alone]} ms, in the mix {[round(m[3][1]) for m in mix]} ms")
+    assert st["max_requests_per_round"] >= 2
+    for a, m in zip(alone, mix):
+        for x, y in zip(a[:3], m[:3]):
+            assert np.abs(x - y).max() <= 1e-12
+    dG = np.abs(mix[0][0] - Go).max()
+    say(f"  10-atom request vs oracle: max|G-Go| {dG:.2e}")
+    assert dG <= 1e-8
+    assert np.abs(mix[0][1] - Jo).max() <= 1e-8 and np.abs(mix[0][2] - Ko).max() <= 1e-8

@@ -152,7 +152,13 @@ def test_generator_groups_cover_targets_once(budget):
 def test_flop_model_matches_canonical_counts():
     # HRR and cart->sph counts of the (dd|dd) canonical model
     assert flop_model.quartet_flops(2, 2, 2, 2) == 11256 + 10736
-    assert flop_model.vrr_flops(2, 2, 2, 2) > 961
+    # node-by-node and class-by-class counts of the VRR agree
+    for cls in [(0, 0, 0, 0), (1, 0, 1, 0), (1, 1, 1, 1), (2, 1, 2, 0), (2, 2, 2, 2), (3, 1, 2, 2), (3, 3, 3, 3)]:
+        assert flop_model.vrr_flops(*cls) == flop_model.class_vrr_flops(*cls)
+    assert flop_model.vrr_flops(2, 2, 2, 2) == 18261 + 961
+    assert flop_model.prim_flops(2, 2, 2, 2) == 204 + 18261 + 961
+    assert flop_model.prim_flops(2, 2, 2, 2, model="survey") == 24116
+    assert flop_model.prim_flops(2, 1, 2, 2, model="survey") == flop_model.prim_flops(2, 1, 2, 2)
     assert flop_model.prim_flops(0, 0, 0, 0) == 60 + 16 + 1  # prefactors + Boys + 1 contraction add
 
 
