@@ -379,6 +379,40 @@ int l2g_server_stats(l2g_server server, int64_t* rounds, int64_t* max_requests_p
  * frees the server (results not yet collected are dropped). */
 int l2g_server_destroy(l2g_server server);
 
+/* ---- single quartets: the per-call path of Engine::compute ------------- */
+
+/* A resident evaluator of one shell quartet per call, for callers that loop
+ * over quartets as libint's do (Engine::compute, engine.h:747-749;
+ * compute2<coulomb, xx_xx, 0>, :787-791; the driver loop of
+ * tests/hartree-fock/hartree-fock++.cc:1628-1743).  Creation allocates the
+ * device buffers and pinned host staging for shells of up to max_nprim
+ * primitives and l <= max_l; a call then makes one host-to-device copy, one
+ * kernel launch per operator pass and one device-to-host copy, and waits
+ * for them.  Not thread-safe (one per thread, as libint's Engine). */
+typedef struct l2g_quartet_engine_s* l2g_quartet_engine;
+int l2g_quartet_engine_create(l2g_context ctx, int max_nprim, int max_l, l2g_quartet_engine* engine);
+int l2g_quartet_engine_destroy(l2g_quartet_engine engine);
+
+/* Integrals of the quartet (s0 s1|s2 s3) in caller order, libint's layout
+ * (n0 n1 n2 n3 doubles into out).  Shells as l2g_basis_create with nshell 4
+ * (l[4], pure[4], nprim[4], alpha and coeff concatenated, origin[12]).
+ * ln_scr = NULL: the shell pairs are built here with `screening`, Original
+ * or Conservative (as compute2 without ShellPair arguments,
+ * engine.impl.h:1212-1216).  Otherwise the caller's precomputed shell pairs
+ * (compute2 with spbra/spket, engine.impl.h:1249-1285; ShellPair::init,
+ * shell.h:1137-1328): ln_scr holds every primitive pair's ln_scr of the bra
+ * pair (s0, s1), n0 n1 values p0-major, then of the ket pair (s2, s3), n2 n3
+ * values; -INFINITY marks a primitive pair that is not in the ShellPair's
+ * list; nonsph (optional) the same for nonsph_screen_fac.  Any screening
+ * method is then allowed.  oper/omega as l2g_eri_plan_set_operator;
+ * scale = prescale_by, uniform_cart as l2g_eri_plan_create.  *nonzero
+ * (optional) receives 0 when every primitive quartet was screened out
+ * (results()[0] == nullptr, engine.impl.h:1781-1784). */
+int l2g_quartet_compute(l2g_quartet_engine engine, const int* l, const int* pure, const int* nprim,
+                        const double* alpha, const double* coeff, const double* origin, const double* ln_scr,
+                        const double* nonsph, double precision, int screening, double scale, int uniform_cart,
+                        int oper, double omega, double* out, int8_t* nonzero);
+
 /* One-shot convenience: plan, execute, download, destroy.  offsets and
  * nonzero may be NULL. */
 int l2g_eri_compute(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision, int screening,

@@ -103,6 +103,11 @@ ABI = {
     "l2g_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64)]),
     "l2g_server_destroy": (ctypes.c_int, [_vp]),
+    "l2g_quartet_engine_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "l2g_quartet_engine_destroy": (ctypes.c_int, [_vp]),
+    "l2g_quartet_compute": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           _f64p, _vp]),
 }
 
 
@@ -617,6 +622,50 @@ class FockBuilder:
             pass
 
 
+class QuartetEngine:
+    """Resident single-quartet evaluator (l2g_quartet_engine_* of
+    include/l2g.h): device buffers and pinned staging are allocated once, and
+    each compute() makes one copy in, one launch per operator pass and one
+    copy out.  The per-call path of Engine.compute."""
+
+    def __init__(self, ctx: Context, max_nprim: int, max_l: int):
+        h = _vp()
+        _check(lib().l2g_quartet_engine_create(ctx.handle, max_nprim, max_l, ctypes.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        ctx._children.add(self)
+        _live.add(self)
+
+    def compute(self, shells, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL, scale=1.0,
+                uniform_cart=False, oper=0, omega=0.0, ln_scr=None, nonsph=None):
+        """Integrals of (s0 s1|s2 s3) in libint's layout and the nonzero
+        flag.  ln_scr / nonsph: the precomputed shell pairs' primitive-pair
+        screening data (bra n0 n1 values, then ket n2 n3; -inf drops a
+        primitive pair), as compute2 with ShellPair arguments."""
+        a = BasisArrays.from_shells(shells)
+        n = int(np.prod(a.sizes()))
+        out = np.zeros(max(n, 1))
+        nz = np.zeros(1, np.int8)
+        ls = None if ln_scr is None else np.ascontiguousarray(ln_scr, np.float64)
+        ns = None if nonsph is None else np.ascontiguousarray(nonsph, np.float64)
+        _check(lib().l2g_quartet_compute(self._h, a.l, a.pure, a.nprim, a.alpha, a.coeff, a.origin,
+                                         None if ls is None else ls.ctypes.data,
+                                         None if ns is None else ns.ctypes.data, precision, screening, scale,
+                                         int(uniform_cart), oper, omega, out, nz.ctypes.data))
+        return out[:n], bool(nz[0])
+
+    def _release(self):
+        if self._h:
+            lib().l2g_quartet_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+
 def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,
                      scale=1.0, uniform_cart=False, fp32_threshold=0.0):
     """Evaluate a batch; returns (values, offsets, nonzero)."""
@@ -625,6 +674,25 @@ def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, scre
     p.execute()
     vals, nz = p.download()
     return vals, offs, nz
+
+
+def uniform_cart_scales(shell: "Shell") -> np.ndarray:
+    """Per-function factors that take a shell from libint's standard to its
+    uniform Cartesian normalization: sqrt((2l-1)!! / ((2i-1)!! (2j-1)!!
+    (2k-1)!!)) for Cartesian component (i, j, k), 1 for a solid-harmonic shell
+    (include/libint2/cartesian.h:96-133)."""
+    if shell.pure:
+        return np.ones(2 * shell.l + 1)
+
+    def dfact(k):
+        r = 1.0
+        for i in range(2 * k - 1, 1, -2):
+            r *= i
+        return r
+
+    l = shell.l
+    comps = [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+    return np.array([np.sqrt(dfact(l) / (dfact(x) * dfact(y) * dfact(z))) for x, y, z in comps])
 
 
 _default_ctx = None
@@ -678,6 +746,7 @@ class Engine:
         self.screening = screening
         self.scale = 1.0
         self.uniform_cart = False
+        self._qe = None  # the resident single-quartet evaluator, made on first use
 
     def set_precision(self, prec: float) -> "Engine":
         self.precision = max(0.0, float(prec))
@@ -694,6 +763,12 @@ class Engine:
 
     def prescale_by(self, s: float) -> "Engine":
         self.scale = float(s)
+        return self
+
+    def set_cartesian_normalization(self, uniform: bool) -> "Engine":
+        """CartesianShellNormalization::uniform (True) or ::standard (False),
+        as Engine::set(CartesianShellNormalization) (engine.h:842-845)."""
+        self.uniform_cart = bool(uniform)
         return self
 
     def compute(self, *shells: Shell):
@@ -713,8 +788,13 @@ class Engine:
                                self.charges if self.oper == NUCLEAR else [])
             M = {OVERLAP: S, KINETIC: T, NUCLEAR: V}[self.oper]
             n1 = shells[0].size()
-            # prescale_by applies to one-body integrals too (engine.impl.h:1050)
-            return self.scale * np.ascontiguousarray(M[:n1, n1:]).reshape(-1)
+            # prescale_by applies to one-body integrals too (engine.impl.h:1050);
+            # uniform normalization rescales each Cartesian shell's functions
+            # (engine.impl.h:552-558)
+            blk = self.scale * np.array(M[:n1, n1:])
+            if self.uniform_cart:
+                blk *= np.outer(uniform_cart_scales(shells[0]), uniform_cart_scales(shells[1]))
+            return np.ascontiguousarray(blk).reshape(-1)
         slots = self.BRAKETS[self.braket]
         if len(shells) != sum(k is not None for k in slots):
             raise ValueError(f"{self.braket} takes {sum(k is not None for k in slots)} shells")
@@ -732,13 +812,11 @@ class Engine:
             vals, _, nz = eri_deriv1(self.ctx, BasisArrays.from_shells(quad), [[0, 1, 2, 3]], self.precision,
                                      self.screening)
             return [vals[d] for d in range(12)] if nz[0] else None
-        b = Basis(self.ctx, BasisArrays.from_shells(quad))
-        p = EriPlan(b, [[0, 1, 2, 3]], self.precision, self.screening, self.scale, self.uniform_cart)
-        if self.oper != COULOMB:
-            p.set_operator(self.oper, self.omega)
-        p.execute()
-        vals, nz = p.download()
-        return vals if nz[0] else None
+        if self._qe is None:
+            self._qe = QuartetEngine(self.ctx, self.max_nprim, self.max_l_)
+        vals, nz = self._qe.compute(quad, self.precision, self.screening, self.scale, self.uniform_cart,
+                                    self.oper, self.omega)
+        return vals if nz else None
 
     def compute_batch(self, basis: Basis, quartets):
         return compute_quartets(basis, quartets, self.precision, self.screening, self.scale, self.uniform_cart)

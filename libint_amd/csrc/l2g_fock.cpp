@@ -177,7 +177,7 @@ void fock_setup(FockSetup& f, l2g_basis_s* B, double pair_threshold, double max_
     l2g_basis PB = nullptr;
     int rc = l2g_basis_create(B->ctx, (int)pl.size(), pl.data(), pp.data(), pn.data(), pa.data(), pc.data(), po.data(), &PB);
     if (rc) throw L2gError(rc, g_err);
-    std::unique_ptr<l2g_basis_s> hold(PB);
+    std::unique_ptr<l2g_basis_s, void (*)(l2g_basis_s*)> hold(PB, &basis_release);
     std::vector<int32_t> ppairs;
     for (int64_t i = 0; i < np; ++i) {
       const int a = pairs[2 * i], b = pairs[2 * i + 1];
@@ -545,6 +545,7 @@ void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, b
   if (slots)
     for (size_t c = 0; c < items.count.size(); ++c) sides |= items.count[c] > 0 && (*slots)[c].side >= 0;
   if (sides) fork_sides(*ctx);
+  try {
   for (size_t c = 0; c < items.count.size(); ++c) {
     const int64_t m = items.count[c], o = items.first[c];
     if (m == 0) continue;
@@ -576,6 +577,16 @@ void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, b
                 ", block " + std::to_string(block) + ", items " + std::to_string(m) + ")");
     }
     if (ev) HIPCHK(hipEventRecord(ev[2 * c + 1], cs));
+  }
+  } catch (...) {
+    // the side streams are joined back also when a launch fails
+    if (sides) {
+      try {
+        join_sides(*ctx);
+      } catch (...) {
+      }
+    }
+    throw;
   }
   if (sides) join_sides(*ctx);
 }
@@ -672,6 +683,7 @@ int l2g_fock_create(l2g_basis B, double pair_threshold, double max_engine_precis
     HIPCHK(hipMemcpyAsync(f->mats.p, &md, sizeof(md), hipMemcpyHostToDevice, B->ctx->stream));
     HIPCHK(hipStreamSynchronize(B->ctx->stream));
     ++B->ctx->live;
+    basis_retain(B);
     *out = f.release();
   });
 }
@@ -874,11 +886,13 @@ int l2g_fock_class_stats(l2g_fock f, int cap, int* nclasses, int* class_code, in
 
 int l2g_fock_destroy(l2g_fock f) {
   if (f && !g_exiting) {
-    l2g_context_s* ctx = f->setup.B->ctx;
+    l2g_basis_s* B = f->setup.B;
+    l2g_context_s* ctx = B->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete f;
     --ctx->live;
+    basis_release(B);  // the builder's hold on its basis
   }
   return L2G_OK;
 }
@@ -1290,7 +1304,14 @@ void exec_loop(l2g_server_s* S) {
     }
     const int rc = guarded([&] { run_round(S, act); });
     if (rc) {
-      // a launch failed: every request of the round fails with it
+      // a launch failed: every request of the round fails with it.  Kernels
+      // of the round queued before the failure may still be adding into the
+      // requests' accumulators: wait for the exec streams (the side streams
+      // were joined into the main one) before their arena ranges go back to
+      // the free lists, where the next admission zeroes and reuses them on
+      // another stream
+      (void)hipStreamSynchronize(S->exec->stream);
+      for (int k = 0; k < l2g_context_s::kSide; ++k) (void)hipStreamSynchronize(S->exec->side[k]);
       std::lock_guard<std::mutex> lk(S->mu);
       for (ServerRequest* r : act) {
         auto it = std::find(S->active.begin(), S->active.end(), r);

@@ -119,11 +119,13 @@ struct l2g_eri_plan_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double* graph_out = nullptr;
+  int graph_gauss = -1;  // solid-harmonic ordering the graph's launches were captured with
   hipEvent_t t0 = nullptr, t1 = nullptr;  // whole execution
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr, graph = nullptr, graph_out = nullptr;
+    graph_gauss = -1;
   }
   ~l2g_eri_plan_s() {
     drop_graph();
@@ -380,7 +382,7 @@ int l2g_basis_create(l2g_context ctx, int nshell, const int* l, const int* pure,
 }
 
 int l2g_basis_destroy(l2g_basis b) {
-  if (!g_exiting) delete b;
+  if (!g_exiting) basis_release(b);
   return L2G_OK;
 }
 
@@ -547,6 +549,7 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
     plan_layout(plan.get(), ctx);
     HIPCHK(hipStreamSynchronize(st));
     ++ctx->live;
+    basis_retain(B);
     *out = plan.release();
   });
 }
@@ -676,7 +679,10 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       HIPCHK(hipEventCreate(&p->t0));
       HIPCHK(hipEventCreate(&p->t1));
     }
-    const bool capture = p->use_graph && !(p->gexec && p->graph_out == out);
+    // a replay reuses the captured kernel arguments: recapture for another
+    // output buffer or after the solid-harmonic ordering changed
+    const bool capture =
+        p->use_graph && !(p->gexec && p->graph_out == out && p->graph_gauss == (int)ctx->shg_gaussian);
     if (p->use_graph && !capture) {  // replay
       HIPCHK(hipEventRecord(p->t0, ctx->stream));
       HIPCHK(hipGraphLaunch(p->gexec, ctx->stream));
@@ -781,6 +787,7 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
     HIPCHK(hipStreamEndCapture(ctx->stream, &p->graph));
     HIPCHK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
     p->graph_out = out;
+    p->graph_gauss = (int)ctx->shg_gaussian;
     HIPCHK(hipEventRecord(p->t0, ctx->stream));
     HIPCHK(hipGraphLaunch(p->gexec, ctx->stream));
     HIPCHK(hipEventRecord(p->t1, ctx->stream));
@@ -905,11 +912,13 @@ int l2g_eri_compute(l2g_basis B, int64_t nq, const int32_t* quartets, double pre
 
 int l2g_eri_plan_destroy(l2g_eri_plan p) {
   if (p && !g_exiting) {
-    l2g_context_s* ctx = p->basis->ctx;
+    l2g_basis_s* B = p->basis;
+    l2g_context_s* ctx = B->ctx;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete p;
     --ctx->live;
+    basis_release(B);  // the plan's hold on its basis
   }
   return L2G_OK;
 }

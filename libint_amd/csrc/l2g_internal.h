@@ -288,6 +288,11 @@ inline void join_sides(const l2g_context_s& ctx) {
 
 struct l2g_basis_s {
   l2g_context_s* ctx;
+  // owners: the caller's handle plus every plan and Fock builder made on the
+  // basis; the basis is freed when the last of them lets go, so a plan may
+  // outlive l2g_basis_destroy of its basis (as happens when a garbage
+  // collector finalizes both in either order)
+  std::atomic<int> refs{1};
   int nshell;
   std::vector<int> l, pure, nprim, off;
   std::vector<double> alpha, coeff, lnc, origin;
@@ -328,6 +333,13 @@ struct l2g_basis_s {
   // canonical bra orientation of (i, j): l_i >= l_j, ties by second_is_centre
   bool swap_pair(int i, int j) { return l[i] < l[j] || (l[i] == l[j] && second_is_centre(i, j)); }
 };
+
+namespace l2g_impl {
+inline void basis_retain(l2g_basis_s* b) { ++b->refs; }
+inline void basis_release(l2g_basis_s* b) {
+  if (b && --b->refs == 0) delete b;
+}
+}  // namespace l2g_impl
 
 // FP32 selection of a mixed batch: a quartet runs in FP32 when its Schwarz
 // bound K_ab K_cd is below the caller's threshold and its pre-HRR bound
@@ -468,6 +480,17 @@ struct PairBuilder {
   // the ordered pair (i, j), p1-major, supplied per pair by the caller
   std::unordered_map<uint64_t, const std::vector<double>*> pfac;
 
+  // Precomputed shell pairs (Engine::compute2 with ShellPair arguments,
+  // engine.impl.h:1249-1285): the caller's ln_scr and nonsph_screen_fac of
+  // every primitive pair of the ordered pair (i, j), p1-major; a primitive
+  // pair whose ln_scr is -infinity (or NaN) is not in the caller's list.
+  // When any pair is given, every pair must be.
+  struct Given {
+    const double* ln_scr;
+    const double* nonsph;  // may be null (zeros)
+  };
+  std::unordered_map<uint64_t, Given> given;
+
   PairBuilder(const l2g_basis_s& b, double lp, int m) : B(b), ln_prec(lp), method(m) {}
 
   int get(int i, int j) {
@@ -476,7 +499,7 @@ struct PairBuilder {
     if (it != id.end()) return it->second;
     const std::vector<double>* fac = nullptr;
     bool fac_t = false;
-    if (method == L2G_SCREEN_SCHWARZ || method == L2G_SCREEN_SCHWARZINF) {
+    if (given.empty() && (method == L2G_SCREEN_SCHWARZ || method == L2G_SCREEN_SCHWARZINF)) {
       auto f = pfac.find(key);
       if (f == pfac.end()) {
         f = pfac.find((uint64_t)(uint32_t)j << 32 | (uint32_t)i);
@@ -484,6 +507,17 @@ struct PairBuilder {
       }
       if (f == pfac.end()) throw L2gError(L2G_E_STATE, "no primitive Schwarz factors for a shell pair");
       fac = f->second;
+    }
+    const Given* gv = nullptr;
+    bool gv_t = false;
+    if (!given.empty()) {
+      auto g = given.find(key);
+      if (g == given.end()) {
+        g = given.find((uint64_t)(uint32_t)j << 32 | (uint32_t)i);
+        gv_t = true;
+      }
+      if (g == given.end()) throw std::invalid_argument("no precomputed primitive-pair data for a shell pair");
+      gv = &g->second;
     }
     const int pid = (int)first.size();
     id.emplace(key, pid);
@@ -506,16 +540,21 @@ struct PairBuilder {
         const double rho = a1 * a2 * oog;
         const double mr = -rho * AB2;
         double ln_scr = mr + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
-        if (fac) {
+        double nonsph = 0;
+        if (gv) {
+          const size_t k = gv_t ? (size_t)p2 * n1 + p1 : (size_t)p1 * n2 + p2;
+          ln_scr = gv->ln_scr[k];
+          if (!(ln_scr > -std::numeric_limits<double>::infinity())) continue;
+          nonsph = gv->nonsph ? gv->nonsph[k] : 0.0;
+        } else if (fac) {
           const double f = fac_t ? (*fac)[(size_t)p2 * n1 + p1] : (*fac)[(size_t)p1 * n2 + p2];
           ln_scr = std::log((double)(n1 * n2) * f) + B.lnc[B.off[i] + p1] + B.lnc[B.off[j] + p2];
           if (ln_scr < ln_prec) continue;
         }
-        if (method == L2G_SCREEN_ORIGINAL && ln_scr < ln_prec) continue;
+        if (!gv && method == L2G_SCREEN_ORIGINAL && ln_scr < ln_prec) continue;
         double Pp[3];
         for (int k = 0; k < 3; ++k) Pp[k] = (AB2 == 0.0) ? A[k] : (a1 * A[k] + a2 * Bo[k]) * oog;
-        double nonsph = 0;
-        if (method == L2G_SCREEN_CONSERVATIVE) {
+        if (!gv && method == L2G_SCREEN_CONSERVATIVE) {
           double mpa = 0, mpb = 0;
           for (int k = 0; k < 3; ++k) {
             mpa = std::max(mpa, std::fabs(Pp[k] - A[k]));

@@ -47,6 +47,23 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_native_build(outdir: str) -> str:
+    """Compiles the oracle for the ISA of the host this runs on
+    (g++ -O3 -march=native) into outdir and loads that library instead of
+    the portable build (x86-64-v3, made where the repository is built and
+    loaded wherever it runs).  For timing the CPU baseline on the machine
+    that measures the GPU.  Returns the compile flags used."""
+    global _lib, _LIBPATH
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "liboracle_native.so")
+    flags = ["-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c++17"]
+    subprocess.run(["g++", *flags, "-shared", "-o", path, os.path.join(_HERE, "eri_oracle.cpp")], check=True)
+    _lib = None
+    _LIBPATH = path
+    lib()
+    return " ".join(flags)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -67,6 +84,8 @@ def lib():
             ctypes.c_int64, _i32p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
             ctypes.c_void_p, _i64p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_eri_quartets.restype = ctypes.c_int64
+        L.oracle_eri_quartets_ld.argtypes = L.oracle_eri_quartets.argtypes
+        L.oracle_eri_quartets_ld.restype = ctypes.c_int64
         L.oracle_eri_brute.argtypes = basis_args + [ctypes.c_int64, _i32p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_eri_brute.restype = ctypes.c_int64
         L.oracle_fock_2e.argtypes = basis_args + [
@@ -159,6 +178,31 @@ def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGI
         nz = np.zeros(nq, np.int8)
         L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart),
                               out.ctypes.data, offs, nz.ctypes.data, nthreads)
+    finally:
+        L.oracle_set_operator(COULOMB, 0.0)
+    return out[:total], offs, nz
+
+
+def eri_quartets_ld(basis, quartets, precision=np.finfo(float).eps, screening=ORIGINAL,
+                    uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0):
+    """eri_quartets evaluated in long double throughout (Boys function by
+    its series, pair data from the shells), rounded to double: the arbiter
+    for values where the double restatement loses digits to HRR
+    cancellation.  Returns (flat values, offsets, nonzero)."""
+    ob = OBasis.of(basis)
+    q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+    nq = q.shape[0]
+    offs = np.zeros(nq, np.int64)
+    L = lib()
+    if L.oracle_set_operator(int(oper), float(omega)) != 0:
+        raise ValueError("bad operator or omega")
+    try:
+        total = L.oracle_eri_quartets_ld(*ob.args(), nq, q.reshape(-1), precision, screening,
+                                         int(uniform_cart), None, offs, None, nthreads)
+        out = np.zeros(max(total, 1))
+        nz = np.zeros(nq, np.int8)
+        L.oracle_eri_quartets_ld(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart),
+                                 out.ctypes.data, offs, nz.ctypes.data, nthreads)
     finally:
         L.oracle_set_operator(COULOMB, 0.0)
     return out[:total], offs, nz

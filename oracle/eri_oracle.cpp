@@ -239,13 +239,23 @@ static std::vector<int> shg_m_list(int l) {
   return ms;
 }
 // dense (2l+1) x ncart matrix
-static std::vector<double> sh_matrix(int l) {
+static std::vector<double> make_sh_matrix(int l) {
   auto ms = shg_m_list(l);
   auto cc = cart_comps(l);
   std::vector<double> M((2 * l + 1) * cc.size());
   for (size_t p = 0; p < ms.size(); ++p)
     for (size_t c = 0; c < cc.size(); ++c) M[p * cc.size() + c] = sh_coeff(l, ms[p], cc[c][0], cc[c][1], cc[c][2]);
   return M;
+}
+// the matrix for the current ordering, built once per (ordering, l)
+static const std::vector<double>& sh_matrix(int l) {
+  static std::mutex mtx;
+  static std::map<int, std::vector<double>> cache;
+  const int key = g_shg_ordering * 64 + l;
+  std::lock_guard<std::mutex> lk(mtx);
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, make_sh_matrix(l)).first;
+  return it->second;
 }
 
 // ----------------------------------------------------------------------------
@@ -538,30 +548,34 @@ static const VrrProgram& vrr_program(int la, int lb, int lc, int ld) {
 // (a, b+1i) = (a+1i, b) + AB_i (a, b)   (hrr.h:290-330).  Generic, level by
 // level in b; the build direction is the first axis with b_i > 0.
 // ----------------------------------------------------------------------------
-static void hrr_transfer(int l1, int l2, const double AB[3], const double* X, int rest, double* Y) {
-  // level-indexed storage: T[bl][a-level a1 in l1 .. l1+l2-bl][ncart(a1)][ncart(bl)][rest]
-  // implemented with maps from (a triple, b triple) to vector
-  if (l2 == 0) {
-    std::memcpy(Y, X, sizeof(double) * ncart(l1) * rest);
-    return;
-  }
-  // offsets of e blocks within X
-  std::vector<int> eoff(l1 + l2 + 2, 0);
-  for (int l = l1; l <= l1 + l2; ++l) eoff[l + 1 - l1] = eoff[l - l1] + ncart(l);
-  // current level: b of level bl, a in levels l1 .. l1+l2-bl
-  // store as cur[(aoff(a level) + ia) * ncart(bl) + ib][rest]
-  std::vector<double> cur(X, X + (size_t)eoff[l2 + 1] * rest), nxt;
+// The transfer runs from a program built once per (l1, l2): per level bl of
+// b, the size of the level's array and its ops (destination, the two
+// sources as row offsets into the previous level, the axis).
+struct HrrOp {
+  int d, s0, s1, dir;
+};
+struct HrrProgram {
+  int l1, l2;
+  int nin;                           // rows of X: the e blocks l1 .. l1+l2
+  std::vector<int> nrows;            // rows of level bl = 1 .. l2
+  std::vector<std::vector<HrrOp>> ops;
+};
+static HrrProgram make_hrr_program(int l1, int l2) {
+  HrrProgram P;
+  P.l1 = l1, P.l2 = l2;
+  P.nin = 0;
+  for (int l = l1; l <= l1 + l2; ++l) P.nin += ncart(l);
   for (int bl = 1; bl <= l2; ++bl) {
     const int amax = l1 + l2 - bl;
-    std::vector<int> aoff(amax - l1 + 2, 0);
+    // rows of the level: (a of level al in l1 .. amax, index ia) x b index
+    std::vector<int> aoff(amax - l1 + 2, 0), aoffp(amax + 1 - l1 + 2, 0);
     for (int l = l1; l <= amax; ++l) aoff[l + 1 - l1] = aoff[l - l1] + ncart(l);
-    const int nb = ncart(bl), nbm = ncart(bl - 1);
-    std::vector<int> aoffp(amax + 1 - l1 + 2, 0);
     for (int l = l1; l <= amax + 1; ++l) aoffp[l + 1 - l1] = aoffp[l - l1] + ncart(l);
-    nxt.assign((size_t)aoff[amax - l1 + 1] * nb * rest, 0.0);
-    auto bcomps = cart_comps(bl);
+    const int nb = ncart(bl), nbm = ncart(bl - 1);
+    const auto bcomps = cart_comps(bl);
+    std::vector<HrrOp> lev;
     for (int al = l1; al <= amax; ++al) {
-      auto acomps = cart_comps(al);
+      const auto acomps = cart_comps(al);
       for (size_t ia = 0; ia < acomps.size(); ++ia)
         for (int ib = 0; ib < nb; ++ib) {
           const auto& b = bcomps[ib];
@@ -572,16 +586,55 @@ static void hrr_transfer(int l1, int l2, const double AB[3], const double* X, in
           std::array<int, 3> ap = acomps[ia];
           ap[i] += 1;
           const int iap = cart_index(ap[0], ap[1], ap[2]);
-          const double* s0 = &cur[((size_t)(aoffp[al + 1 - l1] + iap) * nbm + ibm) * rest];
-          const double* s1 = &cur[((size_t)(aoffp[al - l1] + ia) * nbm + ibm) * rest];
-          double* d = &nxt[((size_t)(aoff[al - l1] + ia) * nb + ib) * rest];
-          const double ab = AB[i];
-          for (int r = 0; r < rest; ++r) d[r] = s0[r] + ab * s1[r];
+          HrrOp op;
+          op.d = (aoff[al - l1] + (int)ia) * nb + ib;
+          op.s0 = (aoffp[al + 1 - l1] + iap) * nbm + ibm;
+          op.s1 = (aoffp[al - l1] + (int)ia) * nbm + ibm;
+          op.dir = i;
+          lev.push_back(op);
         }
     }
-    cur.swap(nxt);
+    P.nrows.push_back(aoff[amax - l1 + 1] * nb);
+    P.ops.push_back(std::move(lev));
   }
-  std::memcpy(Y, cur.data(), sizeof(double) * ncart(l1) * ncart(l2) * rest);
+  return P;
+}
+static const HrrProgram& hrr_program(int l1, int l2) {
+  static std::mutex mtx;
+  static std::map<int, std::unique_ptr<HrrProgram>> cache;
+  const int key = l1 * 64 + l2;
+  std::lock_guard<std::mutex> lk(mtx);
+  auto it = cache.find(key);
+  if (it != cache.end()) return *it->second;
+  auto p = std::make_unique<HrrProgram>(make_hrr_program(l1, l2));
+  auto& r = *p;
+  cache[key] = std::move(p);
+  return r;
+}
+// cur and nxt: caller-owned work arrays (reused across calls)
+template <class Real>
+static void hrr_transfer(int l1, int l2, const Real AB[3], const Real* X, int rest, Real* Y, std::vector<Real>& cur,
+                         std::vector<Real>& nxt) {
+  if (l2 == 0) {
+    std::memcpy(Y, X, sizeof(Real) * ncart(l1) * rest);
+    return;
+  }
+  const HrrProgram& P = hrr_program(l1, l2);
+  const Real* src = X;
+  for (int bl = 1; bl <= l2; ++bl) {
+    std::vector<Real>& dstv = (bl & 1) ? cur : nxt;
+    dstv.resize((size_t)P.nrows[bl - 1] * rest);
+    Real* dst = dstv.data();
+    for (const HrrOp& op : P.ops[bl - 1]) {
+      const Real* s0 = src + (size_t)op.s0 * rest;
+      const Real* s1 = src + (size_t)op.s1 * rest;
+      Real* d = dst + (size_t)op.d * rest;
+      const Real ab = AB[op.dir];
+      for (int r = 0; r < rest; ++r) d[r] = s0[r] + ab * s1[r];
+    }
+    src = dst;
+  }
+  std::memcpy(Y, src, sizeof(Real) * ncart(l1) * ncart(l2) * rest);
 }
 
 // Core integrals of the operator (Engine's core_eval_pack, engine.impl.h:
@@ -610,10 +663,71 @@ static void core_ints(const Boys& boys, double* Gm, double rho, double T, int mm
   }
 }
 
+// The same core integrals in long double for the extended-precision
+// restatement (eri_quartets_ld), from FmEval_Reference2's series.
+static void core_ints_ld(long double* Gm, long double rho, long double T, int mmax) {
+  if (g_oper == 0 || g_oper == 2)
+    boys_reference(T, mmax, Gm);
+  else
+    for (int m = 0; m <= mmax; ++m) Gm[m] = 0;
+  if (g_omega > 0) {
+    const long double o2 = (long double)g_omega * g_omega, s = o2 / (o2 + rho);
+    long double Fs[64];
+    boys_reference(T * s, mmax, Fs);
+    long double f = sqrtl(s);
+    for (int m = 0; m <= mmax; ++m, f *= s) Gm[m] += (g_oper == 2 ? -f : f) * Fs[m];
+  }
+}
+static inline void core_ints_t(double* Gm, double rho, double T, int mmax) { core_ints(cheb7(), Gm, rho, T, mmax); }
+static inline void core_ints_t(long double* Gm, long double rho, long double T, int mmax) {
+  core_ints_ld(Gm, rho, T, mmax);
+}
+
+// Primitive-pair quantities.  In double they are the ShellPair's stored
+// values, exactly as the reference reads them (engine.impl.h:1330-1360); in
+// long double they are recomputed from the shells in long double, so that
+// the extended-precision restatement carries no double rounding of its
+// inputs beyond the exponents, coefficients and centres themselves.
+template <class Real>
+static inline Real oo_gamma(double a1, double a2, double stored) {
+  if constexpr (sizeof(Real) == sizeof(double)) return stored;
+  return Real(1) / ((Real)a1 + (Real)a2);
+}
+template <class Real>
+static inline void pair_center(const struct Shell& s1, const struct Shell& s2, int p1, int p2, const double* stored,
+                               Real* P);
+template <class Real>
+static inline Real pair_K(const struct Shell& s1, const struct Shell& s2, int p1, int p2, double stored);
+
 // ----------------------------------------------------------------------------
 // Engine restatement
 // ----------------------------------------------------------------------------
-struct Engine {
+template <class Real>
+static inline void pair_center(const Shell& s1, const Shell& s2, int p1, int p2, const double* stored, Real* P) {
+  if constexpr (sizeof(Real) == sizeof(double)) {
+    for (int i = 0; i < 3; ++i) P[i] = stored[i];
+  } else {
+    const Real a1 = s1.alpha[p1], a2 = s2.alpha[p2];
+    bool same = true;
+    for (int i = 0; i < 3; ++i) same = same && s1.O[i] == s2.O[i];
+    for (int i = 0; i < 3; ++i) P[i] = same ? (Real)s1.O[i] : (a1 * (Real)s1.O[i] + a2 * (Real)s2.O[i]) / (a1 + a2);
+  }
+}
+template <class Real>
+static inline Real pair_K(const Shell& s1, const Shell& s2, int p1, int p2, double stored) {
+  if constexpr (sizeof(Real) == sizeof(double)) {
+    return stored;
+  } else {
+    const Real a1 = s1.alpha[p1], a2 = s2.alpha[p2], g = a1 + a2;
+    Real AB2 = 0;
+    for (int i = 0; i < 3; ++i) AB2 += ((Real)s1.O[i] - (Real)s2.O[i]) * ((Real)s1.O[i] - (Real)s2.O[i]);
+    // sqrt(2) pi^(5/4) exp(-a1 a2 / g |AB|^2) / g (shell.h:1244)
+    return 5.914967172795612877839928522935L * expl(-a1 * a2 / g * AB2) / g;
+  }
+}
+
+template <class Real>
+struct EngineT {
   double precision = std::numeric_limits<double>::epsilon();
   double ln_precision = std::log(std::numeric_limits<double>::epsilon());
   int screening = Original;
@@ -621,9 +735,10 @@ struct Engine {
   bool uniform_cart = false;
   // scratch
   ShellPair spbra, spket;
-  std::vector<double> prim[16];
-  std::vector<double> buf, acc, work1, work2;
-  std::vector<double> result;  // caller-order result
+  std::vector<Real> prim[16];
+  std::vector<Real> buf, acc, work1, work2;
+  std::vector<Real> roe, gm, cart, kt, cur, nx, h1, h2;  // reused: no allocation once warm
+  std::vector<Real> result;  // caller-order result
   void set_precision(double p) {
     if (p <= 0) {
       precision = 0;
@@ -639,7 +754,8 @@ struct Engine {
                 const ShellPair* tspbra = nullptr, const ShellPair* tspket = nullptr);
 };
 
-bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, const Shell& tk2,
+template <class Real>
+bool EngineT<Real>::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, const Shell& tk2,
                       const ShellPair* tspbra, const ShellPair* tspket) {
   // canonical reorder (engine.impl.h:1181-1208)
   const bool swap_tbra = tb1.l < tb2.l;
@@ -671,10 +787,15 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
     spket.init(k1, k2, ln_precision, screening);
     spk = &spket;
   }
-  double AB[3], CD[3];
+  Real AB[3], CD[3];
   for (int i = 0; i < 3; ++i) {
-    AB[i] = spb_swapped ? -spb->AB[i] : spb->AB[i];
-    CD[i] = spk_swapped ? -spk->AB[i] : spk->AB[i];
+    if constexpr (sizeof(Real) == sizeof(double)) {
+      AB[i] = spb_swapped ? -spb->AB[i] : spb->AB[i];
+      CD[i] = spk_swapped ? -spk->AB[i] : spk->AB[i];
+    } else {
+      AB[i] = (Real)b1.O[i] - (Real)b2.O[i];
+      CD[i] = (Real)k1.O[i] - (Real)k2.O[i];
+    }
   }
   const auto& A = b1.O;
   const auto& C = k1.O;
@@ -686,10 +807,9 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
   // per-primitive-quartet prefactors, SoA
   enum { PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, OO2Z, OO2E, OO2ZE, ROZ };
   for (auto& v : prim) v.resize(maxp);
-  std::vector<double> roe(maxp);
-  std::vector<double> gm((size_t)(L + 1) * maxp);
-  const auto& boys = cheb7();
-  double Fm[64];
+  roe.resize(maxp);
+  gm.resize((size_t)(L + 1) * maxp);
+  Real Fm[64];
   int p = 0;
   for (int ib = 0; ib < npb; ++ib)
     for (int ik = 0; ik < npk; ++ik) {
@@ -698,31 +818,32 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
       if (!(bp.ln_scr + kp.ln_scr > ln_precision)) continue;
       const int pb1 = spb_swapped ? bp.p2 : bp.p1, pb2 = spb_swapped ? bp.p1 : bp.p2;
       const int pk1 = spk_swapped ? kp.p2 : kp.p1, pk2 = spk_swapped ? kp.p1 : kp.p2;
-      const double a0 = b1.alpha[pb1], a1 = b2.alpha[pb2], a2 = k1.alpha[pk1], a3 = k2.alpha[pk2];
-      const double c0 = b1.coeff[pb1], c1 = b2.coeff[pb2], c2 = k1.coeff[pk1], c3 = k2.coeff[pk2];
-      const double gp = a0 + a1, oogp = bp.one_over_gamma;
-      const double gq = a2 + a3, oogq = kp.one_over_gamma;
-      const double* P = bp.P;
-      const double* Q = kp.P;
-      const double PQx = P[0] - Q[0], PQy = P[1] - Q[1], PQz = P[2] - Q[2];
-      const double PQ2 = PQx * PQx + PQy * PQy + PQz * PQz;
-      const double K12 = bp.K * kp.K;
-      const double gpq = gp + gq;
-      const double sqrt_gpq = std::sqrt(gpq);
-      const double oogpq = 1.0 / gpq;
-      double pfac = K12 * sqrt_gpq * oogpq;
-      pfac *= c0 * c1 * c2 * c3 * scale;
+      const Real a0 = b1.alpha[pb1], a1 = b2.alpha[pb2], a2 = k1.alpha[pk1], a3 = k2.alpha[pk2];
+      const Real c0 = b1.coeff[pb1], c1 = b2.coeff[pb2], c2 = k1.coeff[pk1], c3 = k2.coeff[pk2];
+      const Real gp = a0 + a1, oogp = oo_gamma<Real>(a0, a1, bp.one_over_gamma);
+      const Real gq = a2 + a3, oogq = oo_gamma<Real>(a2, a3, kp.one_over_gamma);
+      Real P[3], Q[3];
+      pair_center<Real>(b1, b2, pb1, pb2, bp.P, P);
+      pair_center<Real>(k1, k2, pk1, pk2, kp.P, Q);
+      const Real PQx = P[0] - Q[0], PQy = P[1] - Q[1], PQz = P[2] - Q[2];
+      const Real PQ2 = PQx * PQx + PQy * PQy + PQz * PQz;
+      const Real K12 = pair_K<Real>(b1, b2, pb1, pb2, bp.K) * pair_K<Real>(k1, k2, pk1, pk2, kp.K);
+      const Real gpq = gp + gq;
+      const Real sqrt_gpq = sqrt(gpq);
+      const Real oogpq = Real(1) / gpq;
+      Real pfac = K12 * sqrt_gpq * oogpq;
+      pfac *= c0 * c1 * c2 * c3 * Real(scale);
       if (screening & (Original | Conservative)) {
-        double mag = std::abs(pfac);
+        double mag = std::abs((double)pfac);
         if (screening == Conservative) mag *= std::max(1., bp.nonsph * kp.nonsph) * npbraket;
         if (mag < precision) continue;
       }
-      const double rho = gp * gq * oogpq;
-      const double T = PQ2 * rho;
-      core_ints(boys, Fm, rho, T, L);
+      const Real rho = gp * gq * oogpq;
+      const Real T = PQ2 * rho;
+      core_ints_t(Fm, rho, T, L);
       for (int m = 0; m <= L; ++m) gm[(size_t)m * maxp + p] = Fm[m] * pfac;
-      const double gpo = oogpq * gp, gqo = oogpq * gq;
-      double W[3];
+      const Real gpo = oogpq * gp, gqo = oogpq * gq;
+      Real W[3];
       for (int i = 0; i < 3; ++i) W[i] = gpo * P[i] + gqo * Q[i];
       for (int i = 0; i < 3; ++i) {
         prim[PAx + i][p] = P[i] - A[i];
@@ -742,52 +863,52 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
 
   const int n1 = b1.size(), n2 = b2.size(), n3 = k1.size(), n4 = k2.size();
   const int c1n = ncart(la), c2n = ncart(lb), c3n = ncart(lc), c4n = ncart(ld);
-  std::vector<double> cart((size_t)c1n * c2n * c3n * c4n);
+  cart.resize((size_t)c1n * c2n * c3n * c4n);
   if (L == 0) {
-    double s = 0;
+    Real s = 0;
     for (int q = 0; q < np; ++q) s += gm[q];
     cart[0] = s;
   } else {
     const VrrProgram& prog = vrr_program(la, lb, lc, ld);
     buf.resize((size_t)prog.nslots * np);
     for (const auto& op : prog.ops) {
-      double* d = &buf[(size_t)op.dst * np];
+      Real* d = &buf[(size_t)op.dst * np];
       if (op.kind == 0) {
-        const double* g = &gm[(size_t)op.m * maxp];
+        const Real* g = &gm[(size_t)op.m * maxp];
         for (int q = 0; q < np; ++q) d[q] = g[q];
         continue;
       }
-      const double* x0 = &buf[(size_t)op.s0 * np];
-      const double* x1 = &buf[(size_t)op.s1 * np];
+      const Real* x0 = &buf[(size_t)op.s0 * np];
+      const Real* x1 = &buf[(size_t)op.s1 * np];
       if (op.kind == 1) {
-        const double* pa = prim[PAx + op.dir].data();
-        const double* wp = prim[WPx + op.dir].data();
+        const Real* pa = prim[PAx + op.dir].data();
+        const Real* wp = prim[WPx + op.dir].data();
         if (op.s2 >= 0) {
-          const double* x2 = &buf[(size_t)op.s2 * np];
-          const double* x3 = &buf[(size_t)op.s3 * np];
-          const double* z = prim[OO2Z].data();
-          const double* r = prim[ROZ].data();
-          const double c = op.c23;
+          const Real* x2 = &buf[(size_t)op.s2 * np];
+          const Real* x3 = &buf[(size_t)op.s3 * np];
+          const Real* z = prim[OO2Z].data();
+          const Real* r = prim[ROZ].data();
+          const Real c = op.c23;
           for (int q = 0; q < np; ++q)
             d[q] = pa[q] * x0[q] + wp[q] * x1[q] + c * z[q] * (x2[q] - r[q] * x3[q]);
         } else {
           for (int q = 0; q < np; ++q) d[q] = pa[q] * x0[q] + wp[q] * x1[q];
         }
       } else {
-        const double* qc = prim[QCx + op.dir].data();
-        const double* wq = prim[WQx + op.dir].data();
+        const Real* qc = prim[QCx + op.dir].data();
+        const Real* wq = prim[WQx + op.dir].data();
         for (int q = 0; q < np; ++q) d[q] = qc[q] * x0[q] + wq[q] * x1[q];
         if (op.s2 >= 0) {
-          const double* x2 = &buf[(size_t)op.s2 * np];
-          const double* x3 = &buf[(size_t)op.s3 * np];
-          const double* e = prim[OO2E].data();
-          const double c = op.c23;
+          const Real* x2 = &buf[(size_t)op.s2 * np];
+          const Real* x3 = &buf[(size_t)op.s3 * np];
+          const Real* e = prim[OO2E].data();
+          const Real c = op.c23;
           for (int q = 0; q < np; ++q) d[q] += c * e[q] * (x2[q] - roe[q] * x3[q]);
         }
         if (op.s4 >= 0) {
-          const double* x4 = &buf[(size_t)op.s4 * np];
-          const double* ze = prim[OO2ZE].data();
-          const double c = op.c4;
+          const Real* x4 = &buf[(size_t)op.s4 * np];
+          const Real* ze = prim[OO2ZE].data();
+          const Real c = op.c4;
           for (int q = 0; q < np; ++q) d[q] += c * ze[q] * x4[q];
         }
       }
@@ -796,45 +917,45 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
     const int ne = prog.ne, nf = prog.nf;
     acc.assign((size_t)ne * nf, 0.0);
     for (int t = 0; t < ne * nf; ++t) {
-      const double* x = &buf[(size_t)prog.target_slot[t] * np];
-      double s = 0;
+      const Real* x = &buf[(size_t)prog.target_slot[t] * np];
+      Real s = 0;
       for (int q = 0; q < np; ++q) s += x[q];
       acc[t] = s;
     }
     // HRR on the bra: X[e][f] -> [a b][f]
     work1.resize((size_t)c1n * c2n * nf);
-    hrr_transfer(la, lb, AB, acc.data(), nf, work1.data());
+    hrr_transfer(la, lb, AB, acc.data(), nf, work1.data(), h1, h2);
     // HRR on the ket: for each (ab), [f] -> [c d]
     // transpose so that f is leading: T[f][ab]
     const int nab = c1n * c2n;
     work2.resize((size_t)nf * nab);
     for (int ab = 0; ab < nab; ++ab)
       for (int f = 0; f < nf; ++f) work2[(size_t)f * nab + ab] = work1[(size_t)ab * nf + f];
-    std::vector<double> kt((size_t)c3n * c4n * nab);
-    hrr_transfer(lc, ld, CD, work2.data(), nab, kt.data());
+    kt.resize((size_t)c3n * c4n * nab);
+    hrr_transfer(lc, ld, CD, work2.data(), nab, kt.data(), h1, h2);
     for (int cd = 0; cd < c3n * c4n; ++cd)
       for (int ab = 0; ab < nab; ++ab) cart[(size_t)ab * c3n * c4n + cd] = kt[(size_t)cd * nab + ab];
   }
   // cart -> sph per index (solidharmonics transform_first/inner/last)
-  std::vector<double> cur = cart;
+  cur.assign(cart.begin(), cart.end());
   int dims[4] = {c1n, c2n, c3n, c4n};
   const Shell* sh[4] = {&b1, &b2, &k1, &k2};
   for (int k = 0; k < 4; ++k) {
     if (!sh[k]->pure) continue;
     const int l = sh[k]->l;
-    auto M = sh_matrix(l);
+    const auto& M = sh_matrix(l);
     const int nc = ncart(l), ns = 2 * l + 1;
     int pre = 1, post = 1;
     for (int j = 0; j < k; ++j) pre *= dims[j];
     for (int j = k + 1; j < 4; ++j) post *= dims[j];
-    std::vector<double> nx((size_t)pre * ns * post, 0.0);
+    nx.assign((size_t)pre * ns * post, 0.0);
     for (int a = 0; a < pre; ++a)
       for (int s = 0; s < ns; ++s)
         for (int c = 0; c < nc; ++c) {
-          const double v = M[s * nc + c];
-          if (v == 0.0) continue;
-          const double* src = &cur[((size_t)a * nc + c) * post];
-          double* dst = &nx[((size_t)a * ns + s) * post];
+          const Real v = M[s * nc + c];
+          if (v == 0) continue;
+          const Real* src = &cur[((size_t)a * nc + c) * post];
+          Real* dst = &nx[((size_t)a * ns + s) * post];
           for (int r = 0; r < post; ++r) dst[r] += v * src[r];
         }
     cur.swap(nx);
@@ -887,6 +1008,8 @@ bool Engine::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, cons
   }
   return true;
 }
+
+using Engine = EngineT<double>;
 
 // ----------------------------------------------------------------------------
 // brute force: McMurchie-Davidson over primitive Cartesian Gaussians in
@@ -1330,6 +1453,49 @@ int64_t oracle_eri_quartets(int nshell, const int* l, const int* pure, const int
         std::memcpy(out + offsets[q], e.result.data(), sizeof(double) * sz);
       else
         std::memset(out + offsets[q], 0, sizeof(double) * sz);
+    }
+  }
+  return total;
+}
+
+// The same restatement evaluated in long double throughout (Boys function
+// from its series, pair data recomputed from the shells): an arbiter for
+// values where the double restatement itself loses digits to cancellation
+// (HRR across a distant diffuse-tight pair), far cheaper than the
+// McMurchie-Davidson brute force for deep contractions.  Same arguments and
+// layout as oracle_eri_quartets; results rounded to double.
+int64_t oracle_eri_quartets_ld(int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                               const double* coeff, const double* origin, int64_t nq, const int32_t* quartets,
+                               double precision, int screening, int uniform_cart, double* out, int64_t* offsets,
+                               int8_t* nonzero, int nthreads) {
+  Basis B = make_basis(nshell, l, pure, nprim, alpha, coeff, origin);
+  int64_t total = 0;
+  for (int64_t q = 0; q < nq; ++q) {
+    offsets[q] = total;
+    int64_t sz = 1;
+    for (int k = 0; k < 4; ++k) sz *= B.shells[quartets[4 * q + k]].size();
+    total += sz;
+  }
+  if (!out) return total;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    EngineT<long double> e;
+    e.set_precision(precision);
+    e.screening = screening;
+    e.uniform_cart = uniform_cart != 0;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nq; ++q) {
+      const Shell& s1 = B.shells[quartets[4 * q + 0]];
+      const Shell& s2 = B.shells[quartets[4 * q + 1]];
+      const Shell& s3 = B.shells[quartets[4 * q + 2]];
+      const Shell& s4 = B.shells[quartets[4 * q + 3]];
+      const bool nz = e.compute2(s1, s2, s3, s4);
+      const int64_t sz = (int64_t)s1.size() * s2.size() * s3.size() * s4.size();
+      if (nonzero) nonzero[q] = nz ? 1 : 0;
+      for (int64_t k = 0; k < sz; ++k) out[offsets[q] + k] = nz ? (double)e.result[k] : 0.0;
     }
   }
   return total;

@@ -29,17 +29,19 @@ def compare(got, ref, abs_tol, rel_tol):
     return int(fail.sum()), float(d.max(initial=0.0)), float(np.where(np.abs(ref) > abs_tol, rel, 0).max(initial=0.0))
 
 
-def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol):
-    """compare(), with the oracle's own misses settled by an independent
-    method.  The oracle follows the reference's algorithm (OS VRR + HGP HRR,
-    building on the first shell of each pair); where HRR cancels — a distant
-    pair of a diffuse and a tight shell — its FP64 values themselves carry
-    ~1e-12 absolute error, and the GPU path, which builds on the tighter
-    shell, can be closer to the true value than the oracle is.  Every value
-    that fails against the oracle is checked instead against the
-    long-double McMurchie-Davidson brute force of the oracle
-    (oracle_eri_brute), and passes if it meets the same bar there.
-    Returns (nfail, n_values_arbitrated, max |oracle - brute| over them)."""
+def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol, oper=0, omega=0.0,
+                       uniform_cart=False):
+    """compare(), with the oracle's own misses settled by the same
+    restatement in long double.  The oracle follows the reference's
+    algorithm (OS VRR + HGP HRR, building on the first shell of each pair);
+    where HRR cancels — a distant pair of a diffuse and a tight shell — its
+    FP64 values themselves carry ~1e-12 absolute error, and the GPU path,
+    which builds on the tighter shell, can be closer to the true value than
+    the oracle is.  Every quartet with a value that fails against the oracle
+    is evaluated again by oracle.eri_quartets_ld (long double throughout,
+    pinned against the McMurchie-Davidson brute force in test_oracle), and
+    its failing values pass if they meet the same bar there.
+    Returns (nfail, n_values_arbitrated, max |oracle - arbiter| over them)."""
     import oracle
     got = np.asarray(got, float)
     ref = np.asarray(ref, float)
@@ -51,11 +53,11 @@ def compare_arbitrated(arrays, quartets, offsets, got, ref, abs_tol, rel_tol):
     offsets = np.asarray(offsets, np.int64)
     ends = np.append(offsets[1:], got.size)
     bad = np.unique(np.searchsorted(offsets, np.nonzero(fail)[0], side="right") - 1)
-    brute = oracle.eri_brute(arrays, q[bad])
+    exact, _, _ = oracle.eri_quartets_ld(arrays, q[bad], oper=oper, omega=omega, uniform_cart=uniform_cart)
     nfail, narb, oerr, pos = 0, 0, 0.0, 0
     for k in bad:
         a, e = offsets[k], ends[k]
-        vb = brute[pos:pos + (e - a)]
+        vb = exact[pos:pos + (e - a)]
         pos += e - a
         m = fail[a:e]
         db = np.abs(got[a:e] - vb)

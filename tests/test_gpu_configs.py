@@ -15,8 +15,9 @@ of Engine::compute2 / compute_2body_fock.
 
 Per-integral bar: 1e-12 absolute / 1e-10 relative, failing only if both are
 exceeded (tests/eri/test.cc:77-88); where the oracle itself misses the exact
-value (HRR cancellation), the long-double brute force settles it
-(tests.helpers.compare_arbitrated)."""
+value (HRR cancellation), the same restatement in long double settles it
+(tests.helpers.compare_arbitrated, the oracle's restatement in long
+double)."""
 import sys
 import time
 
@@ -53,7 +54,7 @@ def all_pairs(b):
 
 
 def check_batch(arrays, q, g, offs, label):
-    """Per-integral bar against the oracle, with brute-force arbitration;
+    """Per-integral bar against the oracle, with long-double arbitration;
     prints the per-class maxima."""
     t0 = time.perf_counter()
     o, ooff, onz = oracle.eri_quartets(arrays, q)
@@ -69,7 +70,7 @@ def check_batch(arrays, q, g, offs, label):
         say(f"  {label} ({c[:2]}|{c[2:]}) x{int((cls == c).sum())}: max abs {d[m].max():.2e}")
     nfail, narb, oerr = compare_arbitrated(arrays, q, offs, g, o, 1e-12, 1e-10)
     say(f"  {label}: {len(q)} quartets, {g.size} integrals, max abs {d.max():.2e}; oracle {t_or:.1f} s; "
-        f"fails {nfail}, {narb} values arbitrated by brute force (oracle's own error up to {oerr:.1e})")
+        f"fails {nfail}, {narb} values arbitrated in long double (oracle's own error up to {oerr:.1e})")
     return nfail
 
 
@@ -87,7 +88,7 @@ def test_config2_headline_dddd_full(ctx):
     g, nz = plan.download()
     st = plan.stats()
     assert len(st) == 1 and st[0]["cls"] == "2222"
-    nfail, worst, narb = 0, 0.0, 0
+    nfail, worst, narb, nzbad = 0, 0.0, 0, 0
     chunk = 150000
     t0 = time.perf_counter()
     for a in range(0, len(q), chunk):
@@ -96,15 +97,36 @@ def test_config2_headline_dddd_full(ctx):
         hi = offs[a + chunk] if a + chunk < len(q) else g.size
         o, ooff, onz = oracle.eri_quartets(mol.basis, qc)
         assert np.array_equal(ooff + lo, offs[a:a + chunk])
-        assert np.array_equal(onz, nz[a:a + chunk])
+        bad = np.nonzero(onz != nz[a:a + chunk])[0]
+        nzbad += bad.size
+        for k in bad[:3]:
+            say(f"  nonzero flag differs: quartet {qc[k].tolist()} gpu {int(nz[a + k])} oracle {int(onz[k])}")
         gc = g[lo:hi]
         worst = max(worst, float(np.abs(gc - o).max()))
-        f, k, _ = compare_arbitrated(mol.basis, qc, ooff, gc, o, 1e-12, 1e-10)
+        f, k, oerr = compare_arbitrated(mol.basis, qc, ooff, gc, o, 1e-12, 1e-10)
+        if f:
+            # the failing quartets, against the long-double arbiter
+            d = np.abs(gc - o)
+            ends = np.append(ooff[1:], gc.size)
+            bq = np.unique(np.searchsorted(ooff, np.nonzero((d > 1e-12) & (d > 1e-10 * np.abs(o)))[0],
+                                           side="right") - 1)
+            ex, _, _ = oracle.eri_quartets_ld(mol.basis, qc[bq[:5]])
+            pos = 0
+            for kq in bq[:5]:
+                n_ = ends[kq] - ooff[kq]
+                e_ = ex[pos:pos + n_]
+                pos += n_
+                gg = gc[ooff[kq]:ends[kq]]
+                j = int(np.argmax(np.abs(gg - e_)))
+                say(f"  FAIL quartet {qc[kq].tolist()} K {mol.basis.nprim[qc[kq]].tolist()}: worst value {j}: "
+                    f"gpu {gg[j]:.16e} oracle {o[ooff[kq] + j]:.16e} long double {e_[j]:.16e}")
         nfail += f
         narb += k
+        say(f"  quartets {a}..{a + len(qc)}: fails so far {nfail}, arbitrated {narb}, nonzero mismatches {nzbad}")
     say(f"  {len(q)} quartets, {g.size} integrals vs oracle in {time.perf_counter() - t0:.1f} s: "
-        f"max abs {worst:.2e}, fails {nfail}, {narb} values arbitrated by brute force")
-    assert nfail == 0
+        f"max abs {worst:.2e}, fails {nfail}, {narb} values arbitrated in long double, "
+        f"{nzbad} nonzero-flag mismatches")
+    assert nfail == 0 and nzbad == 0
 
 
 def test_config2_all_classes_sample(ctx):

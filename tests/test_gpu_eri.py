@@ -236,6 +236,24 @@ def test_graph_replay_matches_eager(ctx):
         P.execute()
         ctx.synchronize()
         assert np.array_equal(P.download()[0], ref)
+        # switching the solid-harmonic ordering re-captures: the replay gives
+        # the eager Gaussian-order result, and switching back the Standard one
+        E = la.EriPlan(b, q, fp32_threshold=thr)
+        ctx.set_solid_harmonics_ordering(la.SHG_GAUSSIAN)
+        try:
+            E.execute()
+            gref, _ = E.download()
+            P.execute()
+            ctx.synchronize()
+            assert np.array_equal(P.download()[0], gref)
+        finally:
+            ctx.set_solid_harmonics_ordering(la.SHG_STANDARD)
+        if any(s.pure and s.l >= 2 for s in shells):
+            assert not np.array_equal(gref, ref)
+        P.execute()
+        ctx.synchronize()
+        assert np.array_equal(P.download()[0], ref)
+        E._release()
         # graph off: eager again, with per-class timings
         P.set_graph(False)
         Z.execute(P.device_output())

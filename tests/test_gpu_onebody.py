@@ -108,3 +108,26 @@ def test_engine_one_body_and_erf(ctx):
     assert np.array_equal(v, w)
     with pytest.raises(ValueError):
         la.Engine(1, 0, ctx=ctx, oper=la.OVERLAP, deriv_order=1)
+
+
+def test_engine_one_body_prescaled_uniform(ctx):
+    """prescale_by and uniform Cartesian normalization on the one-body path:
+    the Engine's block is the oracle's times the scale and the per-function
+    uniform factors (engine.impl.h:1050, 552-558); deriv_order 1 refuses both,
+    as the C++ façade does."""
+    say("case: Engine one-body prescaled, uniform Cartesian normalization")
+    rng = np.random.default_rng(94)
+    sh = random_shells(rng, [2, 1, 3], kmax=2, pure=False)
+    S, T, V = oracle_one_body(sh, [(3.0, np.array([0.2, -0.1, 0.4]))])
+    off = np.concatenate([[0], np.cumsum([s.size() for s in sh])])
+    for oper, M in ((la.OVERLAP, S), (la.KINETIC, T), (la.NUCLEAR, V)):
+        e = la.Engine(2, 3, ctx=ctx, oper=oper).prescale_by(0.6).set_cartesian_normalization(True)
+        if oper == la.NUCLEAR:
+            e.set_params([(3.0, (0.2, -0.1, 0.4))])
+        for a, b in ((0, 1), (2, 0), (1, 2)):
+            ref = 0.6 * M[off[a]:off[a + 1], off[b]:off[b + 1]] * np.outer(la.uniform_cart_scales(sh[a]),
+                                                                           la.uniform_cart_scales(sh[b]))
+            assert compare(e.compute(sh[a], sh[b]), ref.reshape(-1), ABS, REL)[0] == 0
+    ed = la.Engine(2, 2, ctx=ctx, deriv_order=1).prescale_by(2.0)
+    with pytest.raises(ValueError):
+        ed.compute(*random_shells(rng, [1, 1, 0, 0], kmax=1))

@@ -201,6 +201,46 @@ def test_oracle_vs_brute_force_f():
     assert nfail == 0
 
 
+def test_oracle_ld_vs_brute_force():
+    """The long-double restatement (the arbiter of the GPU parity tests)
+    against the McMurchie-Davidson brute force, classes to (ff|ff) with K = 2,
+    at 1e-15 absolute: both carry ~19 significant digits."""
+    rng = np.random.default_rng(11)
+    b = rand_shells(rng, [3, 3, 2, 1], kmax=2)
+    q = [[0, 1, 0, 1], [1, 0, 3, 2], [0, 0, 1, 1], [2, 3, 1, 0], [3, 2, 0, 0]]
+    vb = oracle.eri_brute(b, q)
+    vl, _, _ = oracle.eri_quartets_ld(b, q, precision=0.0)
+    assert np.abs(vl - vb).max() <= 1e-15
+
+
+def test_oracle_ld_resolves_hrr_cancellation():
+    """A distant pair of a diffuse and a tight d shell built on the diffuse
+    one (the caller's order for equal l, as the reference builds it): the
+    HRR cancels, the double restatement is off the brute force by more than
+    the integrals themselves, the long-double one is not."""
+    b = _Arr([(2, [(0.06, 1.0)], (0, 0, 0), True), (2, [(3000.0, 1.0)], (4.0, 1.0, -2.0), True),
+              (2, [(0.08, 1.0)], (0.5, 0, 0), True), (2, [(900.0, 1.0)], (3.0, 2.5, -1.0), True)])
+    q = [[0, 1, 2, 3]]
+    vb = oracle.eri_brute(b, q)
+    vd, _, _ = oracle.eri_quartets(b, q, precision=0.0)
+    vl, _, _ = oracle.eri_quartets_ld(b, q, precision=0.0)
+    assert np.abs(vd - vb).max() > np.abs(vb).max()  # the double restatement's own error
+    assert np.abs(vl - vb).max() <= 1e-15
+
+
+def test_oracle_ld_matches_double():
+    """On well-conditioned quartets (random shells near each other, K <= 3)
+    the two precisions agree to 1e-13."""
+    rng = np.random.default_rng(12)
+    b = rand_shells(rng, [0, 1, 2, 3, 2, 1], kmax=3)
+    q = np.array(list(itertools.product(range(6), repeat=4)), np.int32)
+    q = q[rng.choice(len(q), 200, replace=False)]
+    vd, od, nd = oracle.eri_quartets(b, q)
+    vl, ol, nl = oracle.eri_quartets_ld(b, q)
+    assert np.array_equal(od, ol) and np.array_equal(nd, nl)
+    assert np.abs(vd - vl).max() <= 1e-13
+
+
 def test_c_api_case_vs_brute_force():
     """The primitive (pp|pp) case of tests/unit/test-c-api.cc:40-96."""
     b = _Arr([(1, [(1.1, 1.0)], (0, 1, 2), False), (1, [(2.3, 1.0)], (1, 2, 0), False),
