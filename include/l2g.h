@@ -292,6 +292,15 @@ int l2g_fock_schwarz(l2g_fock fock, double* K);
  * J_ab = sum (ab|cd) D_cd, K_ab = sum (ac|bd) D_cd, G = 2J - K. */
 int l2g_fock_build(l2g_fock fock, const double* D, double precision, double* J, double* K, double* G);
 
+/* A build produces and digests its screened quartet list in windows of at
+ * most `items` quartets (0: the default, 2^26), so that the list of a large
+ * molecule never has to be held in device memory at once.  No counterpart
+ * in the reference, whose CPU loop visits the quartets one at a time. */
+int l2g_fock_set_window(l2g_fock fock, int64_t items);
+
+/* Windows the last build's quartet list came in. */
+int l2g_fock_stats_windows(l2g_fock fock, int64_t* windows);
+
 /* Counters of the last build: quartets evaluated, primitive quartets (after
  * pair screening) and the GPU time of the digest kernels in ms. */
 int l2g_fock_stats(l2g_fock fock, int64_t* nquartets, int64_t* nprim_quartets, float* kernel_ms);

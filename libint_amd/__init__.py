@@ -103,6 +103,8 @@ ABI = {
     "l2g_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64)]),
     "l2g_server_destroy": (ctypes.c_int, [_vp]),
+    "l2g_fock_set_window": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "l2g_fock_stats_windows": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_quartet_engine_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "l2g_quartet_engine_destroy": (ctypes.c_int, [_vp]),
     "l2g_quartet_compute": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp, ctypes.c_double,
@@ -588,11 +590,16 @@ class FockBuilder:
         """Quartets with Schwarz bound below `threshold` run in FP32."""
         _check(lib().l2g_fock_set_fp32_threshold(self._h, threshold))
 
+    def set_window(self, items: int) -> None:
+        """Quartets per window of the build's list (0: the default)."""
+        _check(lib().l2g_fock_set_window(self._h, int(items)))
+
     def stats(self):
-        a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float(), ctypes.c_int64()
+        a, b, c, d, w = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float(), ctypes.c_int64(), ctypes.c_int64()
         _check(lib().l2g_fock_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         _check(lib().l2g_fock_stats_mixed(self._h, ctypes.byref(d)))
-        return dict(nquartets=a.value, nprim=b.value, kernel_ms=c.value, nquartets_fp32=d.value)
+        _check(lib().l2g_fock_stats_windows(self._h, ctypes.byref(w)))
+        return dict(nquartets=a.value, nprim=b.value, kernel_ms=c.value, nquartets_fp32=d.value, windows=w.value)
 
     def class_stats(self):
         """Per kernel class of the last build: class, fp32, quartets,

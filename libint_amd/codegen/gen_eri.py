@@ -350,6 +350,20 @@ AX = 'xyz'
 _R = 'double'
 # attenuated (erf) variant: the core integrals s^(m+1/2) F_m(sT), s = w^2/(w^2+rho)
 _ATT = False
+# contraction on the matrix cores: each target's primitive sum accumulates
+# through v_mfma_f64_4x4x4_4b_f64 with a one-hot B operand (see emit_class)
+_MFMA = False
+# Lane layout of v_mfma_f64_4x4x4_4b_f64 on gfx950 (4 blocks g of 4x4x4,
+# probed on the device): A(g, i, k) sits in lane 4g + i + 16k, B(g, k, j) in
+# lane 4g + j + 16k and D(g, i, j) in lane 16i + 4g + j.  With B(g, k, j) =
+# [k == j] (MFMA_SELECTOR), D(g, i, j) += A(g, i, j): the value of lane
+# 16j + 4g + i lands in the accumulator of lane 16i + 4g + j, a transpose of
+# the lane bits [5:4] <-> [1:0].  Each product but one is an exact zero, so
+# D = C + v is rounded once, as a plain add is; after the primitive loop one
+# shuffle from lane MFMA_SOURCE returns every accumulator to its own lane (a
+# wave sum needs none).
+MFMA_SELECTOR = "((lane >> 4) == (lane & 3) ? 1.0 : 0.0)"
+MFMA_SOURCE = "(((lane & 3) << 4) | (lane & 12) | (lane >> 4))"
 
 
 def fmt(v):
@@ -444,7 +458,11 @@ def emit_vrr_group(E, plan, graph, grp_idx, acc_names, mid=None):
                     j += 1
         E(f"const {_R} {v} = {expr};")
         if key in tpos:
-            E(f"{acc_names[tpos[key]]} += {v};")
+            a = acc_names[tpos[key]]
+            if _MFMA:
+                E(f"{a} = __builtin_amdgcn_mfma_f64_4x4x4f64({v}, msel, {a}, 0, 0, 0);")
+            else:
+                E(f"{a} += {v};")
     if mid is not None and half >= len(order):
         mid()
     return len(order)
@@ -539,21 +557,22 @@ def pure_variants(l1, l2):
 # ---------------------------------------------------------------------------
 
 
-def _emit_stage2(E, sfx, ib, ik, count):
+def _emit_stage2(E, sfx, gb, gk, count, act=None):
     """Screening, pfac, T and the Boys-table fetch of one primitive quartet
-    (pair data in Pg{sfx}.. Qg{sfx}..).  Screened primitives get pfac = 0:
-    the recursion then produces exact zeros, which keeps the loop free of
-    divergent branches."""
+    (pair data in Pg{sfx}.. Qg{sfx}.., records gb / gk of the pair arrays).
+    Screened primitives get pfac = 0: the recursion then produces exact
+    zeros, which keeps the loop free of divergent branches; so do primitive
+    quartets past the lane's own range (act false) in the wave-uniform loop."""
     E(f"const double gp{sfx} = Pg{sfx}.w, gq{sfx} = Qg{sfx}.w;")
     E(f"const double rs{sfx} = rsqrt(gp{sfx} + gq{sfx});")
     E(f"const double oos{sfx} = rs{sfx} * rs{sfx};")
     E(f"const double pf{sfx} = PAk{sfx}.w * QCk{sfx}.w * rs{sfx} * A.scale;")
-    E(f"bool ok{sfx} = xb{sfx}.x + xk{sfx}.x > lnprec;")
+    E(f"bool ok{sfx} = {act + ' && ' if act else ''}xb{sfx}.x + xk{sfx}.x > lnprec;")
     E.block("if (A.method == L2G_SCREEN_ORIGINAL)")
     E(f"ok{sfx} = ok{sfx} && !(fabs(pf{sfx}) < qprec);")
     E.end()
     E.block("else if (A.method == L2G_SCREEN_CONSERVATIVE)")
-    E(f"const double nsb = A.pp_nonsph[pb0 + {ib}], nsk = A.pp_nonsph[pk0 + {ik}];")
+    E(f"const double nsb = A.pp_nonsph[{gb}], nsk = A.pp_nonsph[{gk}];")
     E(f"ok{sfx} = ok{sfx} && !(fabs(pf{sfx}) * fmax(1.0, nsb * nsk) * npbk < qprec);")
     E.end()
     if count is not None:
@@ -579,27 +598,51 @@ def emit_prim_loop(E, plan, graph, gi, accs):
     pair data of the next primitive are loaded at the top of an iteration and
     its Boys-table row halfway through the recursion, so both memory
     latencies overlap the FP64 work of the current primitive."""
-    E.block("if (pend > pbeg)")
-    E("int ib = pbeg / npk, ik = pbeg - ib * npk;")
-    E("double4 Pg = A.pp_P[pb0 + ib], PAk = A.pp_PA[pb0 + ib];")
-    E("double2 xb = A.pp_aux[pb0 + ib];")
-    E("double4 Qg = A.pp_P[pk0 + ik], QCk = A.pp_PA[pk0 + ik];")
-    E("double2 xk = A.pp_aux[pk0 + ik];")
+    if _MFMA:
+        # wave-uniform trip count (tmax, the wave's longest range): a lane
+        # past its own range, or without primitive quartets, reads record 0
+        # of the pair arrays (present whenever any lane of the wave has work)
+        # and runs with pfac = 0, so every MFMA sees all 64 lanes active
+        E.block("if (tmax > 0)")
+        E("int ib = pbeg / npk1, ik = pbeg - ib * npk1;")
+        E("const int gb = trip > 0 ? pb0 + ib : 0, gk = trip > 0 ? pk0 + ik : 0;")
+        ga, ka = "gb", "gk"
+    else:
+        E.block("if (pend > pbeg)")
+        E("int ib = pbeg / npk, ik = pbeg - ib * npk;")
+        ga, ka = "pb0 + ib", "pk0 + ik"
+    E(f"double4 Pg = A.pp_P[{ga}], PAk = A.pp_PA[{ga}];")
+    E(f"double2 xb = A.pp_aux[{ga}];")
+    E(f"double4 Qg = A.pp_P[{ka}], QCk = A.pp_PA[{ka}];")
+    E(f"double2 xk = A.pp_aux[{ka}];")
     E("double pfac;")
     if _ATT:
         E("double sw;")
     E("L2gBoysArg bq;")
     E.block("")
-    _emit_stage2(E, "", "ib", "ik", "" if gi == 0 else None)
+    if _MFMA:
+        _emit_stage2(E, "", ga, ka, "" if gi == 0 else None, act="trip > 0")
+    else:
+        _emit_stage2(E, "", ga, ka, "" if gi == 0 else None)
     E.end()
-    E.block("for (int p = pbeg; p < pend; ++p)")
-    E("int ikn = ik + 1, ibn = ib;")
-    E("if (ikn == npk) { ikn = 0; ++ibn; }")
-    E("if (ibn == npb) ibn = 0;  // past the end: any valid pair, result unused")
-    E("const double4 Pgn = A.pp_P[pb0 + ibn], PAkn = A.pp_PA[pb0 + ibn];")
-    E("const double2 xbn = A.pp_aux[pb0 + ibn];")
-    E("const double4 Qgn = A.pp_P[pk0 + ikn], QCkn = A.pp_PA[pk0 + ikn];")
-    E("const double2 xkn = A.pp_aux[pk0 + ikn];")
+    if _MFMA:
+        E.block("for (int t = 0; t < tmax; ++t)")
+        E("int ikn = ik + 1, ibn = ib;")
+        E("if (ikn >= npk) { ikn = 0; ++ibn; }")
+        E("if (ibn >= npb) ibn = 0;")
+        E("const bool actn = t + 1 < trip;")
+        E("const int gbn = actn ? pb0 + ibn : 0, gkn = actn ? pk0 + ikn : 0;")
+        gan, kan = "gbn", "gkn"
+    else:
+        E.block("for (int p = pbeg; p < pend; ++p)")
+        E("int ikn = ik + 1, ibn = ib;")
+        E("if (ikn == npk) { ikn = 0; ++ibn; }")
+        E("if (ibn == npb) ibn = 0;  // past the end: any valid pair, result unused")
+        gan, kan = "pb0 + ibn", "pk0 + ikn"
+    E(f"const double4 Pgn = A.pp_P[{gan}], PAkn = A.pp_PA[{gan}];")
+    E(f"const double2 xbn = A.pp_aux[{gan}];")
+    E(f"const double4 Qgn = A.pp_P[{kan}], QCkn = A.pp_PA[{kan}];")
+    E(f"const double2 xkn = A.pp_aux[{kan}];")
     E("double pfacn;")
     if _ATT:
         E("double swn;")
@@ -640,7 +683,10 @@ def emit_prim_loop(E, plan, graph, gi, accs):
 
     def mid():
         E.block("")
-        _emit_stage2(E, "n", "ibn", "ikn", "p + 1 < pend && " if gi == 0 else None)
+        if _MFMA:
+            _emit_stage2(E, "n", gan, kan, "" if gi == 0 else None, act="actn")
+        else:
+            _emit_stage2(E, "n", gan, kan, "p + 1 < pend && " if gi == 0 else None)
         E.end()
 
     emit_vrr_group(E, plan, graph, gi, accs, mid=mid)
@@ -652,7 +698,7 @@ def emit_prim_loop(E, plan, graph, gi, accs):
     E.end()  # npq > 0
 
 
-def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', att=False):
+def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', att=False, mfma=False):
     la, lb, lc, ld = plan.la, plan.lb, plan.lc, plan.ld
     graph = VrrGraph()
     ne, nf = len(plan.ecomps), len(plan.fcomps)
@@ -662,9 +708,15 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
       f"{G} target group(s)")
     E(f"// group sizes (bra components x ket targets): {[(len(g), len(f)) for g, f in plan.groups]}")
     E(f"// group graph nodes: {plan.group_nodes}; estimated peak live doubles: {plan.group_peak}")
-    global _R, _ATT
+    global _R, _ATT, _MFMA
     _R = real
     _ATT = att
+    _MFMA = mfma and real == 'double'
+    if _MFMA:
+        E("// contraction on the matrix cores: every target accumulates its primitive")
+        E("// quartets through v_mfma_f64_4x4x4_4b_f64 with A = the lane's value, B a")
+        E("// one-hot selector that returns each lane's value to its own D slot, and")
+        E("// C/D the accumulator (AGPRs, no VALU moves); loops are wave-uniform")
     kname = f"l2g_{'erf' if att else mode}{'32' if real == 'float' else ''}_{plan.name}"
     attr = f" __attribute__((amdgpu_waves_per_eu({waves}, {waves})))" if waves else ""
     if mode == 'fock':
@@ -682,7 +734,26 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
     E.ind += 1
     E("const int tid = blockIdx.x * blockDim.x + threadIdx.x;")
     E("const int nthr = gridDim.x * blockDim.x;")
-    if mode in ('schwarz', 'eri'):
+    if _MFMA:
+        E("const int lane = threadIdx.x & 63;")
+        E(f"const double msel = {MFMA_SELECTOR};")
+        E(f"const int msrc = {MFMA_SOURCE};")
+        if mode in ('schwarz', 'eri'):
+            E("// A.coop: a wave per work item, each lane taking a contiguous 1/64 of the")
+            E("// primitive quartets; the group accumulators are summed over the wave")
+            E("const bool coop = A.coop != 0;")
+            E("// the wave walks the work list together: lanes past its end take the last")
+            E("// item as a stand-in (all work, no stores), so every loop stays uniform")
+            E.block("for (int qb = coop ? (tid >> 6) : tid - lane; qb < A.n; qb += coop ? (nthr >> 6) : nthr)")
+            E("const bool live = coop || qb + lane < A.n;")
+            E("const int qi = coop ? qb : (live ? qb + lane : A.n - 1);")
+        else:
+            E("// the wave walks the work list together: lanes past its end take the last")
+            E("// item as a stand-in (all work, no digest), so every loop stays uniform")
+            E.block("for (int qb = tid - lane; qb < A.n; qb += nthr)")
+            E("const bool live = qb + lane < A.n;")
+            E("const int qi = live ? qb + lane : A.n - 1;")
+    elif mode in ('schwarz', 'eri'):
         E("// A.coop: a wave per work item, each lane taking a contiguous 1/64 of the")
         E("// primitive quartets; the group accumulators are summed over the wave")
         E("const bool coop = A.coop != 0;")
@@ -713,6 +784,12 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
         E("pend = pbeg + (int)(((long long)len * (lane + 1)) >> 6);")
         E("pbeg = b0;")
         E.end()
+    if _MFMA:
+        E("// wave-uniform primitive loop: the longest range of the wave")
+        E("const int trip = pend - pbeg;")
+        E("int tmax = trip;")
+        E("for (int o = 32; o > 0; o >>= 1) tmax = max(tmax, __shfl_xor(tmax, o, 64));")
+        E("const int npk1 = npk > 0 ? npk : 1;")
     if mode == 'fock':
         E("const double lnprec = A.q_lnprec[qi];")
         E("const double qprec = A.method == L2G_SCREEN_ORIGINAL || A.method == L2G_SCREEN_CONSERVATIVE ? exp(lnprec) : 0.0;")
@@ -751,6 +828,16 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
         for a in accs:
             E(f"{_R} {a} = 0;")
         emit_prim_loop(E, plan, graph, gi, accs)
+        if _MFMA:
+            # the contracted values back to their own lanes (a wave sum is
+            # the same over any lane order)
+            if mode in ('schwarz', 'eri'):
+                E.block("if (!coop)")
+            else:
+                E.block("")
+            for a in accs:
+                E(f"{a} = __shfl({a}, msrc, 64);")
+            E.end()
         if mode in ('schwarz', 'eri'):
             E.block("if (coop)")
             for a in accs:
@@ -795,6 +882,8 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
             E.end()
         E.end()  # group block
     # bra HRR + bra sph + permuted store, column by column over the ket
+    if _MFMA:
+        E("if (!live) continue;  // a stand-in lane: no store, no digest")
     if mode == 'eri':
         E("if (coop) nsurv = l2g_wave_sum(nsurv);")
         E("if (A.nz && (!coop || lane == 0)) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;")
@@ -922,6 +1011,7 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
     E("}")
     _R = 'double'
     _ATT = False
+    _MFMA = False
     return kname, E.text(), in_regs
 
 
@@ -942,7 +1032,7 @@ def class_waves(plan, budget, waves):
 
 def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=None,
              modes=('eri', 'fock', 'schwarz', 'fp32', 'erf'), generic_lmax=3, max_nodes=0, fp32_lmax=2,
-             erf_lmax=2):
+             erf_lmax=2, mfma_acc=False):
     """Write the kernels and the dispatch tables.  `budgets` maps a class name
     ('2222') to its own (budget, waves).  Classes with an l above 2 whose
     partitioned recursion evaluates more than `max_nodes` nodes per primitive
@@ -999,7 +1089,7 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
             p, mode, real = jobs[i]
             wv = cfg((p.la, p.lb, p.lc, p.ld))[1]
             kname, code, in_regs = emit_class(p, mode='eri' if mode == 'erf' else mode, waves=wv, real=real,
-                                              att=mode == 'erf')
+                                              att=mode == 'erf', mfma=mfma_acc)
             parts.append(code)
             parts.append(f"void {kname}_launch(const L2gEriArgs& A, int grid, int block, hipStream_t s) {{")
             parts.append(f"  hipLaunchKernelGGL({kname}, dim3(grid), dim3(block), 0, s, A);")
@@ -1073,6 +1163,8 @@ if __name__ == '__main__':
     ap.add_argument("--max-nodes", type=int, default=0,
                     help="classes with l > 2 and more recursion nodes than this run on the generic kernel")
     ap.add_argument("--fp32-lmax", type=int, default=2, help="highest l with FP32 kernel variants")
+    ap.add_argument("--mfma-acc", type=int, default=0,
+                    help="1: FP64 kernels accumulate the contraction on the matrix cores")
     a = ap.parse_args()
     per = {}
     for item in filter(None, a.class_cfg.split(",")):
@@ -1081,6 +1173,6 @@ if __name__ == '__main__':
         per[k] = (int(b), int(w))
     files, plans = generate(a.outdir, a.lmax, a.budget, a.nfiles, a.waves,
                             set(filter(None, a.only.split(","))) or None, per, tuple(a.modes.split(",")),
-                            a.generic_lmax, a.max_nodes, a.fp32_lmax)
+                            a.generic_lmax, a.max_nodes, a.fp32_lmax, mfma_acc=bool(a.mfma_acc))
     for p in plans:
         print(p.name, 'groups', len(p.groups), 'nodes', sum(p.group_nodes), 'peak', max(p.group_peak))

@@ -66,8 +66,11 @@ __global__ void __launch_bounds__(kEnumThreads) enum_count(const l2g_impl::FockE
   }
 }
 
+// fills the items of blocks [block0, block0 + gridDim.x) of the count pass's
+// partition; base[k] is the first item of block block0 + k within the window
 __global__ void __launch_bounds__(kEnumThreads) enum_fill(const l2g_impl::FockEnumArgs E, long long total,
-                                                             long long chunk, const long long* __restrict__ base,
+                                                             long long chunk, long long block0,
+                                                             const long long* __restrict__ base,
                                                              l2g_impl::FockEnumOut O) {
   __shared__ unsigned int pos;
   __shared__ unsigned int hist[l2g_impl::kEnumClasses];
@@ -75,7 +78,7 @@ __global__ void __launch_bounds__(kEnumThreads) enum_fill(const l2g_impl::FockEn
   if (threadIdx.x == 0) pos = 0;
   for (int c = threadIdx.x; c < l2g_impl::kEnumClasses; c += kEnumThreads) hist[c] = 0, whist[c] = 0;
   __syncthreads();
-  const long long t0 = (long long)blockIdx.x * chunk, t1 = min(total, t0 + chunk);
+  const long long t0 = (block0 + blockIdx.x) * chunk, t1 = min(total, t0 + chunk);
   const long long b0 = base[blockIdx.x];
   for (long long t = t0 + threadIdx.x; t < t1; t += kEnumThreads) {
     long long p1, p2;
@@ -241,31 +244,42 @@ void fock_split_device(FockEnumState& S, const std::vector<int>& chunk, hipStrea
   S.n = (long long)total;
 }
 
-void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st) {
+void fock_count_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st) {
   const long long np = E.npairs;
-  const long long total = np * (np + 1) / 2;
+  S.total_t = np * (np + 1) / 2;
+  S.block_count.clear();
+  S.nblocks = 0;
+  S.tchunk = 0;
+  if (S.total_t == 0) return;
+  S.nblocks = (int)std::min<long long>(8192, (S.total_t + kEnumThreads * 64 - 1) / (kEnumThreads * 64));
+  S.tchunk = (S.total_t + S.nblocks - 1) / S.nblocks;
+  S.counts.alloc((size_t)S.nblocks * sizeof(long long));
+  hipLaunchKernelGGL(enum_count, dim3(S.nblocks), dim3(kEnumThreads), 0, st, E, S.total_t, S.tchunk,
+                     S.counts.as<long long>());
+  LAUNCHCHK("enum_count");
+  S.block_count.resize(S.nblocks);
+  HIPCHK(hipMemcpyAsync(S.block_count.data(), S.counts.p, (size_t)S.nblocks * sizeof(long long),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+void fock_fill_device(const FockEnumArgs& E, FockEnumState& S, int blk0, int blk1, hipStream_t st) {
   S.n = 0;
   S.cls_count.assign(kEnumClasses, 0ull);
   S.cls_nprim.assign(kEnumClasses, 0ull);
-  if (total == 0) return;
-  const int nblocks = (int)std::min<long long>(8192, (total + kEnumThreads * 64 - 1) / (kEnumThreads * 64));
-  const long long chunk = (total + nblocks - 1) / nblocks;
-  S.counts.alloc((size_t)nblocks * sizeof(long long));
-  hipLaunchKernelGGL(enum_count, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.counts.as<long long>());
-  LAUNCHCHK("enum_count");
-  std::vector<long long> cnt(nblocks), base(nblocks);
-  HIPCHK(hipMemcpyAsync(cnt.data(), S.counts.p, (size_t)nblocks * sizeof(long long), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (blk1 <= blk0) return;
+  const int nb = blk1 - blk0;
+  std::vector<long long> base(nb);
   long long n = 0;
-  for (int b = 0; b < nblocks; ++b) {
+  for (int b = 0; b < nb; ++b) {
     base[b] = n;
-    n += cnt[b];
+    n += S.block_count[blk0 + b];
   }
   S.n = n;
   if (n == 0) return;
-  if (n >= (1ll << 31)) throw L2gError(L2G_E_INVALID, "more than 2^31 screened quartets in one Fock build");
-  S.base.alloc((size_t)nblocks * sizeof(long long));
-  HIPCHK(hipMemcpyAsync(S.base.p, base.data(), (size_t)nblocks * sizeof(long long), hipMemcpyHostToDevice, st));
+  if (n >= (1ll << 31)) throw L2gError(L2G_E_INVALID, "more than 2^31 screened quartets in one window of a Fock build");
+  S.base.alloc((size_t)nb * sizeof(long long));
+  HIPCHK(hipMemcpyAsync(S.base.p, base.data(), (size_t)nb * sizeof(long long), hipMemcpyHostToDevice, st));
   // unsorted items, then sorted
   const size_t sz = (size_t)n;
   DevBuf u_bra, u_ket, u_meta, u_lnp, u_scale, u_bf, u_pr, u_key, u_idx, stats;
@@ -294,7 +308,8 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   O.idx = u_idx.as<unsigned>();
   O.cls_count = stats.as<unsigned long long>();
   O.cls_nprim = stats.as<unsigned long long>() + kEnumClasses;
-  hipLaunchKernelGGL(enum_fill, dim3(nblocks), dim3(kEnumThreads), 0, st, E, total, chunk, S.base.as<long long>(), O);
+  hipLaunchKernelGGL(enum_fill, dim3(nb), dim3(kEnumThreads), 0, st, E, S.total_t, S.tchunk, (long long)blk0,
+                     S.base.as<long long>(), O);
   LAUNCHCHK("enum_fill");
   // sort the item indices by (class, decreasing depth)
   S.perm.alloc(sz * sizeof(unsigned));
@@ -326,6 +341,11 @@ void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t 
   HIPCHK(hipStreamSynchronize(st));  // the temporaries are freed on return
   for (int c = 0; c < kEnumClasses; ++c) S.cls_count[c] = h[c], S.cls_nprim[c] = h[kEnumClasses + c];
   if (S.cls_count[kEnumClasses - 1]) throw L2gError(L2G_E_LMAX, "no Fock kernel for an angular momentum class");
+}
+
+void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st) {
+  fock_count_device(E, S, st);
+  fock_fill_device(E, S, 0, S.nblocks, st);
 }
 
 }  // namespace l2g_impl

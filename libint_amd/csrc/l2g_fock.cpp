@@ -593,11 +593,20 @@ void fock_launch_all(l2g_context_s* ctx, L2gEriArgs A, const ItemsView& items, b
 
 }  // namespace
 
+// quartets per window of a Fock build's list (about 200 bytes of device
+// memory each while the window is sorted and split)
+#ifndef L2G_FOCK_WINDOW_ITEMS
+#define L2G_FOCK_WINDOW_ITEMS (1ll << 26)
+#endif
+
 struct l2g_fock_s {
   FockSetup setup;
+  long long window_items = L2G_FOCK_WINDOW_ITEMS;
   double fp32_threshold = 0;  // FP32 selection threshold (fock_enumerate)
   int64_t last_nq = 0, last_nprim = 0, last_nq32 = 0;
+  int64_t last_windows = 0;  // windows the last build's quartet list came in
   float last_ms = 0;
+  std::vector<float> cls_ms;  // per class: kernel ms of the last build, over its windows
   PairDev pairs;  // resident shell-pair data
   DevBuf dens, jg, kg, mats, ybuf;
   // GPU quartet enumeration: pair list in (s1, s2) order, per-shell data,
@@ -748,37 +757,16 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     E.fp32_thr = f->fp32_threshold;
     E.hrr_factor = L2G_FP32_HRR_FACTOR;
     FockEnumState& es = f->enumst;
-    fock_enumerate_device(E, es, st);  // synchronises; Dn may go
+    fock_count_device(E, es, st);  // synchronises; Dn may go
     const int ncls = nfock_classes();
-    f->last_nq = es.n;
+    f->last_nq = 0;
     f->last_nq32 = 0;
     f->last_nprim = 0;
+    f->last_ms = 0;
+    f->last_windows = 0;
     f->cls_nq.assign(ncls, 0);
     f->cls_nprim.assign(ncls, 0);
-    std::vector<int> chunk(kEnumClasses, INT_MAX);
-    for (int c = 0; c < ncls; ++c) {
-      const int64_t m = (int64_t)es.cls_count[c];
-      f->cls_nq[c] = m;
-      f->cls_nprim[c] = (int64_t)es.cls_nprim[c];
-      f->last_nprim += f->cls_nprim[c];
-      if (c >= l2g_gen_nfock && !fock_is_generic(c)) f->last_nq32 += m;
-      chunk[c] = split_chunk(c, m, f->cls_nprim[c], ctx->ncu * 256);
-    }
-    // deep items cut by primitive range on the device (class-major order kept)
-    fock_split_device(es, chunk, st);
-    ItemsView big;
-    big.bra = es.bra.as<int>(), big.ket = es.ket.as<int>(), big.meta = es.meta.as<int>();
-    big.lnp = es.lnp.as<double>(), big.scale = es.scale.as<double>(), big.bf = es.bf.as<int4>();
-    big.pr = es.pr.as<int2>();
-    big.first.assign(ncls, 0);
-    big.count.assign(ncls, 0);
-    int64_t o = 0;
-    for (int c = 0; c < ncls; ++c) {
-      const int64_t m = es.n == 0 ? 0 : (int64_t)es.cls_pieces[c];
-      big.first[c] = o;
-      big.count[c] = m;
-      o += m;
-    }
+    f->cls_ms.assign(ncls, 0.0f);
     const size_t NN = (size_t)N * N;
     f->dens.alloc(std::max<size_t>(NN, 1) * sizeof(double));
     f->jg.alloc(std::max<size_t>(NN, 1) * sizeof(double));
@@ -788,44 +776,102 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       HIPCHK(hipMemsetAsync(f->jg.p, 0, NN * sizeof(double), st));
       HIPCHK(hipMemsetAsync(f->kg.p, 0, NN * sizeof(double), st));
     }
-    const int block = 128;
-    int64_t ymax = 0;
-    std::vector<int64_t> lanes(ncls, 0), yd(ncls, 0);
-    for (int c = 0; c < ncls; ++c) {
-      const int64_t m = big.count[c];
-      if (m == 0) continue;
-      // the generic kernel runs a block per item, no lane scratch, a few
-      // blocks per CU: up to 4 items per CU count as small (side streams)
-      lanes[c] = fock_is_generic(c) ? m * 64 : m;
-      yd[c] = fock_ybuf(c);
-      ymax = std::max<int64_t>(ymax, yd[c]);
-    }
-    std::vector<ClassSlot> slots;
-    const int64_t ytotal = plan_slots(slots, lanes, yd, ctx->ncu, block, l2g_context_s::kSide,
-                                      ymax * (int64_t)ctx->ncu * 8 * block + (int64_t(1) << 29));
-    f->ybuf.alloc(std::max<int64_t>(ytotal, 1) * sizeof(double));
-    const int grid = (int)ctx->ncu * 8;
-    const int64_t ystride = 0;  // per class, from its slot
-    const PairDev& p = f->pairs;
-    L2gEriArgs A = fock_args(ctx, fs.method, p.first.as<int>(), p.np.as<int>(), p.AB.as<double>(), p.P.as<double4>(),
-                             p.PA.as<double4>(), p.aux.as<double2>(), p.ns.as<double>(), f->dens.as<double>(),
-                             f->jg.as<double>(), f->kg.as<double>(), f->mats.as<L2gMatDesc>(), f->ybuf.as<double>(),
-                             ystride);
     if (f->ev.empty()) {
       f->ev.assign(2 * ncls + 2, nullptr);
       for (auto& e : f->ev) HIPCHK(hipEventCreate(&e));
     }
-    hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
-    HIPCHK(hipEventRecord(e0, st));
-    fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data(), &slots);
-    HIPCHK(hipEventRecord(e1, st));
+    const int block = 128;
+    const int grid = (int)ctx->ncu * 8;
+    // The quartet list is produced and digested in windows of consecutive
+    // count-pass blocks holding at most window_items quartets each (one
+    // block may exceed it alone), so that a large molecule's list (1e10
+    // quartets and more) never has to sit in device memory at once.
+    int blk = 0;
+    while (blk < es.nblocks) {
+      int blk1 = blk + 1;
+      long long items = es.block_count[blk];
+      while (blk1 < es.nblocks && items + es.block_count[blk1] <= f->window_items) items += es.block_count[blk1++];
+      fock_fill_device(E, es, blk, blk1, st);
+      blk = blk1;
+      if (es.n == 0) continue;
+      ++f->last_windows;
+      f->last_nq += es.n;
+      std::vector<int> chunk(kEnumClasses, INT_MAX);
+      for (int c = 0; c < ncls; ++c) {
+        const int64_t m = (int64_t)es.cls_count[c];
+        f->cls_nq[c] += m;
+        f->cls_nprim[c] += (int64_t)es.cls_nprim[c];
+        f->last_nprim += (int64_t)es.cls_nprim[c];
+        if (c >= l2g_gen_nfock && !fock_is_generic(c)) f->last_nq32 += m;
+        chunk[c] = split_chunk(c, m, (int64_t)es.cls_nprim[c], ctx->ncu * 256);
+      }
+      // deep items cut by primitive range on the device (class-major order kept)
+      fock_split_device(es, chunk, st);
+      ItemsView big;
+      big.bra = es.bra.as<int>(), big.ket = es.ket.as<int>(), big.meta = es.meta.as<int>();
+      big.lnp = es.lnp.as<double>(), big.scale = es.scale.as<double>(), big.bf = es.bf.as<int4>();
+      big.pr = es.pr.as<int2>();
+      big.first.assign(ncls, 0);
+      big.count.assign(ncls, 0);
+      int64_t o = 0;
+      for (int c = 0; c < ncls; ++c) {
+        const int64_t m = (int64_t)es.cls_pieces[c];
+        big.first[c] = o;
+        big.count[c] = m;
+        o += m;
+      }
+      int64_t ymax = 0;
+      std::vector<int64_t> lanes(ncls, 0), yd(ncls, 0);
+      for (int c = 0; c < ncls; ++c) {
+        const int64_t m = big.count[c];
+        if (m == 0) continue;
+        // the generic kernel runs a block per item, no lane scratch, a few
+        // blocks per CU: up to 4 items per CU count as small (side streams)
+        lanes[c] = fock_is_generic(c) ? m * 64 : m;
+        yd[c] = fock_ybuf(c);
+        ymax = std::max<int64_t>(ymax, yd[c]);
+      }
+      std::vector<ClassSlot> slots;
+      const int64_t ytotal = plan_slots(slots, lanes, yd, ctx->ncu, block, l2g_context_s::kSide,
+                                        ymax * (int64_t)ctx->ncu * 8 * block + (int64_t(1) << 29));
+      f->ybuf.alloc(std::max<int64_t>(ytotal, 1) * sizeof(double));
+      const int64_t ystride = 0;  // per class, from its slot
+      const PairDev& p = f->pairs;
+      L2gEriArgs A = fock_args(ctx, fs.method, p.first.as<int>(), p.np.as<int>(), p.AB.as<double>(),
+                               p.P.as<double4>(), p.PA.as<double4>(), p.aux.as<double2>(), p.ns.as<double>(),
+                               f->dens.as<double>(), f->jg.as<double>(), f->kg.as<double>(), f->mats.as<L2gMatDesc>(),
+                               f->ybuf.as<double>(), ystride);
+      hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
+      HIPCHK(hipEventRecord(e0, st));
+      fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data(), &slots);
+      HIPCHK(hipEventRecord(e1, st));
+      // the window's items and scratch are reused by the next window
+      hipError_t e = hipStreamSynchronize(st);
+      if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Fock kernels failed: ") + hipGetErrorString(e));
+      f->last_ms += event_ms_or(e0, e1, 0.0f);
+      for (int c = 0; c < ncls; ++c)
+        if (big.count[c]) f->cls_ms[c] += event_ms_or(f->ev[2 * c], f->ev[2 * c + 1], 0.0f);
+    }
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Fock kernels failed: ") + hipGetErrorString(e));
-    f->last_ms = event_ms_or(e0, e1, 0.0f);
     if (J && NN) HIPCHK(hipMemcpy(J, f->jg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (K && NN) HIPCHK(hipMemcpy(K, f->kg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (G && NN) HIPCHK(hipMemcpy(G, f->dens.p, NN * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int l2g_fock_set_window(l2g_fock f, int64_t items) {
+  return guarded([&] {
+    if (!f || items < 0) throw std::invalid_argument("l2g_fock_set_window: bad arguments");
+    f->window_items = items ? items : L2G_FOCK_WINDOW_ITEMS;
+  });
+}
+
+int l2g_fock_stats_windows(l2g_fock f, int64_t* windows) {
+  return guarded([&] {
+    if (!f || !windows) throw std::invalid_argument("null argument");
+    *windows = f->last_windows;
   });
 }
 
@@ -874,9 +920,7 @@ int l2g_fock_class_stats(l2g_fock f, int cap, int* nclasses, int* class_code, in
                           (fock_is_generic(c) ? 20000 : (c >= l2g_gen_nfock ? 10000 : 0));
         if (nquartets) nquartets[k] = f->cls_nq[c];
         if (nprim_quartets) nprim_quartets[k] = f->cls_nprim[c];
-        if (ms) {
-          ms[k] = event_ms_or(f->ev[2 * c], f->ev[2 * c + 1], 0.0f);
-        }
+        if (ms) ms[k] = c < (int)f->cls_ms.size() ? f->cls_ms[c] : 0.0f;
       }
       ++k;
     }

@@ -441,7 +441,12 @@ struct FockEnumOut {
 };
 // the sorted items (class-major, decreasing depth) of the last enumeration
 struct FockEnumState {
-  long long n = 0;  // items (after fock_split_device: pieces)
+  // count pass: the triangular range split into nblocks blocks of tchunk
+  // indices, with the surviving quartets of each
+  long long total_t = 0, tchunk = 0;
+  int nblocks = 0;
+  std::vector<long long> block_count;
+  long long n = 0;  // items of the filled window (after fock_split_device: pieces)
   std::vector<unsigned long long> cls_count, cls_nprim;  // kEnumClasses classes
   std::vector<unsigned long long> cls_pieces;            // per class, after the split
   DevBuf counts, base, perm, key, bra, ket, meta, lnp, scale, bf, pr;
@@ -457,6 +462,10 @@ struct FockEnumState {
   }
 };
 void fock_enumerate_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st);
+// the same in two steps: the count pass over the whole triangular range, then
+// the items of count-pass blocks [blk0, blk1) (a window), sorted
+void fock_count_device(const FockEnumArgs& E, FockEnumState& S, hipStream_t st);
+void fock_fill_device(const FockEnumArgs& E, FockEnumState& S, int blk0, int blk1, hipStream_t st);
 // Splits the enumerated items by primitive range, chunk[c] primitives at most
 // per piece of class c (kEnumClasses entries); pieces stay class-major.
 void fock_split_device(FockEnumState& S, const std::vector<int>& chunk, hipStream_t st);

@@ -56,8 +56,9 @@ def use_native_build(outdir: str) -> str:
     global _lib, _LIBPATH
     os.makedirs(outdir, exist_ok=True)
     path = os.path.join(outdir, "liboracle_native.so")
-    flags = ["-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c++17"]
-    subprocess.run(["g++", *flags, "-shared", "-o", path, os.path.join(_HERE, "eri_oracle.cpp")], check=True)
+    flags = ["-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c++17", "-fext-numeric-literals"]
+    subprocess.run(["g++", *flags, "-shared", "-o", path, os.path.join(_HERE, "eri_oracle.cpp"), "-lquadmath"],
+                   check=True)
     _lib = None
     _LIBPATH = path
     lib()
@@ -72,6 +73,7 @@ def lib():
         L = ctypes.CDLL(_LIBPATH)
         L.oracle_set_shg_ordering.argtypes = [ctypes.c_int]
         L.oracle_set_operator.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.oracle_set_tight_centre.argtypes = [ctypes.c_int]
         L.oracle_boys_cheb7.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p]
         L.oracle_boys_reference.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p]
         L.oracle_boys_table.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -86,6 +88,8 @@ def lib():
         L.oracle_eri_quartets.restype = ctypes.c_int64
         L.oracle_eri_quartets_ld.argtypes = L.oracle_eri_quartets.argtypes
         L.oracle_eri_quartets_ld.restype = ctypes.c_int64
+        L.oracle_eri_quartets_q.argtypes = L.oracle_eri_quartets.argtypes
+        L.oracle_eri_quartets_q.restype = ctypes.c_int64
         L.oracle_eri_brute.argtypes = basis_args + [ctypes.c_int64, _i32p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_eri_brute.restype = ctypes.c_int64
         L.oracle_fock_2e.argtypes = basis_args + [
@@ -159,7 +163,7 @@ COULOMB, ERF_COULOMB, ERFC_COULOMB = 0, 1, 2
 
 
 def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGINAL,
-                 uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0):
+                 uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0, tight_centre=False):
     """Restated Engine::compute2 over quartets (caller order), for
     Operator::coulomb, erf_coulomb or erfc_coulomb (omega: the attenuation).
 
@@ -171,6 +175,7 @@ def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGI
     L = lib()
     if L.oracle_set_operator(int(oper), float(omega)) != 0:
         raise ValueError("bad operator or omega")
+    L.oracle_set_tight_centre(int(bool(tight_centre)))
     try:
         total = L.oracle_eri_quartets(*ob.args(), nq, q.reshape(-1), precision, screening,
                                       int(uniform_cart), None, offs, None, nthreads)
@@ -180,15 +185,52 @@ def eri_quartets(basis, quartets, precision=np.finfo(float).eps, screening=ORIGI
                               out.ctypes.data, offs, nz.ctypes.data, nthreads)
     finally:
         L.oracle_set_operator(COULOMB, 0.0)
+        L.oracle_set_tight_centre(0)
+    return out[:total], offs, nz
+
+
+def eri_quartets_quad(basis, quartets, precision=np.finfo(float).eps, screening=ORIGINAL,
+                      uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0):
+    """eri_quartets in quad precision (__float128) throughout, rounded to
+    double: keeps ~20 digits where HGP's HRR amplifies rounding by 1e12 and
+    more (deeply contracted distant pairs of diffuse and tight primitives),
+    where the double and long-double restatements both fail.  Returns (flat
+    values, offsets, nonzero)."""
+    return _eri_quartets_with("oracle_eri_quartets_q", basis, quartets, precision, screening, uniform_cart,
+                              nthreads, oper, omega, False)
+
+
+def _eri_quartets_with(fn, basis, quartets, precision, screening, uniform_cart, nthreads, oper, omega,
+                       tight_centre):
+    ob = OBasis.of(basis)
+    q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+    nq = q.shape[0]
+    offs = np.zeros(nq, np.int64)
+    L = lib()
+    f = getattr(L, fn)
+    if L.oracle_set_operator(int(oper), float(omega)) != 0:
+        raise ValueError("bad operator or omega")
+    L.oracle_set_tight_centre(int(bool(tight_centre)))
+    try:
+        total = f(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart), None, offs, None,
+                  nthreads)
+        out = np.zeros(max(total, 1))
+        nz = np.zeros(nq, np.int8)
+        f(*ob.args(), nq, q.reshape(-1), precision, screening, int(uniform_cart), out.ctypes.data, offs,
+          nz.ctypes.data, nthreads)
+    finally:
+        L.oracle_set_operator(COULOMB, 0.0)
+        L.oracle_set_tight_centre(0)
     return out[:total], offs, nz
 
 
 def eri_quartets_ld(basis, quartets, precision=np.finfo(float).eps, screening=ORIGINAL,
-                    uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0):
+                    uniform_cart=False, nthreads=0, oper=COULOMB, omega=0.0, tight_centre=False):
     """eri_quartets evaluated in long double throughout (Boys function by
-    its series, pair data from the shells), rounded to double: the arbiter
-    for values where the double restatement loses digits to HRR
-    cancellation.  Returns (flat values, offsets, nonzero)."""
+    its series, pair data from the shells), rounded to double; with
+    tight_centre on the tighter shell of each equal-l pair: the arbiter for
+    values where the double restatement loses digits to HRR cancellation.
+    Returns (flat values, offsets, nonzero)."""
     ob = OBasis.of(basis)
     q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
     nq = q.shape[0]
@@ -196,6 +238,7 @@ def eri_quartets_ld(basis, quartets, precision=np.finfo(float).eps, screening=OR
     L = lib()
     if L.oracle_set_operator(int(oper), float(omega)) != 0:
         raise ValueError("bad operator or omega")
+    L.oracle_set_tight_centre(int(bool(tight_centre)))
     try:
         total = L.oracle_eri_quartets_ld(*ob.args(), nq, q.reshape(-1), precision, screening,
                                          int(uniform_cart), None, offs, None, nthreads)
@@ -205,6 +248,7 @@ def eri_quartets_ld(basis, quartets, precision=np.finfo(float).eps, screening=OR
                                  out.ctypes.data, offs, nz.ctypes.data, nthreads)
     finally:
         L.oracle_set_operator(COULOMB, 0.0)
+        L.oracle_set_tight_centre(0)
     return out[:total], offs, nz
 
 
@@ -225,25 +269,39 @@ def eri_brute(basis, quartets, nthreads=0, oper=COULOMB, omega=0.0):
     return out[:total]
 
 
-def fock_2e(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0):
+def fock_2e(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0, tight_centre=False):
     """Restated compute_2body_fock: returns (G, J, K, n_quartets) with
-    G = 2J - K (tests/hartree-fock/hartree-fock++.cc:1574-1772)."""
+    G = 2J - K (tests/hartree-fock/hartree-fock++.cc:1574-1772).
+    tight_centre: every integral (Schwarz factors included) built on the
+    tighter shell of an equal-l pair instead of the caller's first one: the
+    same quantities with far less HRR cancellation, the accurate reference
+    where the reference's own orientation loses digits."""
     ob = OBasis.of(basis)
     D = np.ascontiguousarray(D, np.float64)
     N = D.shape[0]
     G = np.zeros((N, N))
     JK = np.zeros((2, N, N))
     nq = ctypes.c_int64()
-    lib().oracle_fock_2e(*ob.args(), D.reshape(-1), precision, G.reshape(-1), screening, JK.reshape(-1),
+    L = lib()
+    L.oracle_set_tight_centre(int(bool(tight_centre)))
+    try:
+        L.oracle_fock_2e(*ob.args(), D.reshape(-1), precision, G.reshape(-1), screening, JK.reshape(-1),
                          nthreads, ctypes.byref(nq))
+    finally:
+        L.oracle_set_tight_centre(0)
     return G, JK[0], JK[1], nq.value
 
 
-def fock_pairs(basis, threshold=1e-12):
+def fock_pairs(basis, threshold=1e-12, tight_centre=False):
     """(significant-pair mask (s1 >= s2), Schwarz matrix)."""
     ob = OBasis.of(basis)
     n = len(ob.l)
     mask = np.zeros((n, n), np.int8)
     K = np.zeros((n, n))
-    lib().oracle_fock_pairs(*ob.args(), threshold, mask.reshape(-1), K.reshape(-1))
+    L = lib()
+    L.oracle_set_tight_centre(int(bool(tight_centre)))
+    try:
+        L.oracle_fock_pairs(*ob.args(), threshold, mask.reshape(-1), K.reshape(-1))
+    finally:
+        L.oracle_set_tight_centre(0)
     return mask, K

@@ -48,9 +48,11 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <quadmath.h>
 #endif
 
 namespace orc {
@@ -642,6 +644,13 @@ static void hrr_transfer(int l1, int l2, const Real AB[3], const Real* X, int re
 // erf_coulomb_gm_eval / erfc_coulomb_gm_eval (boys.h:1913-1985):
 //   erf:  G_m = s^(m+1/2) F_m(s T),  s = omega^2 / (omega^2 + rho)  (0 if omega = 0)
 //   erfc: G_m = F_m(T) - s^(m+1/2) F_m(s T)
+// HRR build centre of an equal-l pair: the reference keeps the caller's
+// order (engine.impl.h:1181-1208).  With g_tight_centre the restatement
+// builds instead on the shell the pair's charge sits nearer (a K-weighted
+// comparison of |P - A| and |P - B|): the same integrals, with far less HRR
+// cancellation for a distant pair of a diffuse and a tight shell.  Used by
+// the extended-precision arbiter, never by the reference restatement.
+static int g_tight_centre = 0;
 static int g_oper = 0;  // 0 coulomb, 1 erf_coulomb, 2 erfc_coulomb
 static double g_omega = 0;
 template <class Boys>
@@ -678,6 +687,47 @@ static void core_ints_ld(long double* Gm, long double rho, long double T, int mm
     for (int m = 0; m <= mmax; ++m, f *= s) Gm[m] += (g_oper == 2 ? -f : f) * Fs[m];
   }
 }
+// Quad precision (__float128, 113-bit significand): the Boys function by
+// its series below 117 and erf + upward recursion above, the attenuated
+// forms as in long double.  For the arbiter of values the double and long
+// double restatements both lose to cancellation.
+typedef __float128 quad;
+static quad boys_series_q(quad T, int m) {
+  quad denom = m + quad(0.5);
+  quad term = expq(-T) / (2 * denom);
+  quad sum = term, old;
+  const quad eps = FLT128_EPSILON / 10;
+  do {
+    denom += 1;
+    old = term;
+    term = old * T / denom;
+    sum += term;
+  } while (term > sum * eps || old < term);
+  return sum;
+}
+static void boys_reference_q(quad T, int mmax, quad* F) {
+  if (T < 117) {
+    for (int m = 0; m <= mmax; ++m) F[m] = boys_series_q(T, m);
+  } else {
+    const quad t2 = 2 * T, et = expq(-T), st = sqrtq(T);
+    F[0] = sqrtq(M_PIq) / 2 * erfq(st) / st;
+    for (int m = 0; m < mmax; ++m) F[m + 1] = ((2 * m + 1) * F[m] - et) / t2;
+  }
+}
+static inline void core_ints_t(quad* Gm, quad rho, quad T, int mmax) {
+  if (g_oper == 0 || g_oper == 2)
+    boys_reference_q(T, mmax, Gm);
+  else
+    for (int m = 0; m <= mmax; ++m) Gm[m] = 0;
+  if (g_omega > 0) {
+    const quad o2 = (quad)g_omega * g_omega, s = o2 / (o2 + rho);
+    quad Fs[64];
+    boys_reference_q(T * s, mmax, Fs);
+    quad f = sqrtq(s);
+    for (int m = 0; m <= mmax; ++m, f *= s) Gm[m] += (g_oper == 2 ? -f : f) * Fs[m];
+  }
+}
+static inline quad sqrt(quad x) { return sqrtq(x); }
 static inline void core_ints_t(double* Gm, double rho, double T, int mmax) { core_ints(cheb7(), Gm, rho, T, mmax); }
 static inline void core_ints_t(long double* Gm, long double rho, long double T, int mmax) {
   core_ints_ld(Gm, rho, T, mmax);
@@ -690,7 +740,7 @@ static inline void core_ints_t(long double* Gm, long double rho, long double T, 
 // inputs beyond the exponents, coefficients and centres themselves.
 template <class Real>
 static inline Real oo_gamma(double a1, double a2, double stored) {
-  if constexpr (sizeof(Real) == sizeof(double)) return stored;
+  if constexpr (std::is_same_v<Real, double>) return stored;
   return Real(1) / ((Real)a1 + (Real)a2);
 }
 template <class Real>
@@ -704,7 +754,7 @@ static inline Real pair_K(const struct Shell& s1, const struct Shell& s2, int p1
 // ----------------------------------------------------------------------------
 template <class Real>
 static inline void pair_center(const Shell& s1, const Shell& s2, int p1, int p2, const double* stored, Real* P) {
-  if constexpr (sizeof(Real) == sizeof(double)) {
+  if constexpr (std::is_same_v<Real, double>) {
     for (int i = 0; i < 3; ++i) P[i] = stored[i];
   } else {
     const Real a1 = s1.alpha[p1], a2 = s2.alpha[p2];
@@ -715,15 +765,34 @@ static inline void pair_center(const Shell& s1, const Shell& s2, int p1, int p2,
 }
 template <class Real>
 static inline Real pair_K(const Shell& s1, const Shell& s2, int p1, int p2, double stored) {
-  if constexpr (sizeof(Real) == sizeof(double)) {
+  if constexpr (std::is_same_v<Real, double>) {
     return stored;
   } else {
     const Real a1 = s1.alpha[p1], a2 = s2.alpha[p2], g = a1 + a2;
     Real AB2 = 0;
     for (int i = 0; i < 3; ++i) AB2 += ((Real)s1.O[i] - (Real)s2.O[i]) * ((Real)s1.O[i] - (Real)s2.O[i]);
     // sqrt(2) pi^(5/4) exp(-a1 a2 / g |AB|^2) / g (shell.h:1244)
-    return 5.914967172795612877839928522935L * expl(-a1 * a2 / g * AB2) / g;
+    if constexpr (std::is_same_v<Real, quad>)
+      return sqrtq(2 * powq(M_PIq, quad(2.5))) * expq(-a1 * a2 / g * AB2) / g;
+    else
+      return 5.914967172795612877839928522935L * expl(-a1 * a2 / g * AB2) / g;
   }
+}
+
+// true when the second shell of an equal-l pair is the better build centre
+static bool second_is_centre(const Shell& a, const Shell& b) {
+  if (a.l != b.l) return false;
+  double AB2 = 0;
+  for (int k = 0; k < 3; ++k) AB2 += (a.O[k] - b.O[k]) * (a.O[k] - b.O[k]);
+  if (AB2 == 0) return false;
+  double s = 0;
+  for (int p1 = 0; p1 < a.nprim(); ++p1)
+    for (int p2 = 0; p2 < b.nprim(); ++p2) {
+      const double a1 = a.alpha[p1], a2 = b.alpha[p2], g = a1 + a2;
+      const double w = std::fabs(a.coeff[p1] * b.coeff[p2]) * std::exp(-a1 * a2 / g * AB2) / g;
+      s += w * (a2 - a1) / g;
+    }
+  return s > 0;
 }
 
 template <class Real>
@@ -758,8 +827,8 @@ template <class Real>
 bool EngineT<Real>::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, const Shell& tk2,
                       const ShellPair* tspbra, const ShellPair* tspket) {
   // canonical reorder (engine.impl.h:1181-1208)
-  const bool swap_tbra = tb1.l < tb2.l;
-  const bool swap_tket = tk1.l < tk2.l;
+  const bool swap_tbra = tb1.l < tb2.l || (g_tight_centre && second_is_centre(tb1, tb2));
+  const bool swap_tket = tk1.l < tk2.l || (g_tight_centre && second_is_centre(tk1, tk2));
   const bool swap_braket = (tb1.l + tb2.l) > (tk1.l + tk2.l);
   const Shell& b1 = swap_braket ? (swap_tket ? tk2 : tk1) : (swap_tbra ? tb2 : tb1);
   const Shell& b2 = swap_braket ? (swap_tket ? tk1 : tk2) : (swap_tbra ? tb1 : tb2);
@@ -789,7 +858,7 @@ bool EngineT<Real>::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk
   }
   Real AB[3], CD[3];
   for (int i = 0; i < 3; ++i) {
-    if constexpr (sizeof(Real) == sizeof(double)) {
+    if constexpr (std::is_same_v<Real, double>) {
       AB[i] = spb_swapped ? -spb->AB[i] : spb->AB[i];
       CD[i] = spk_swapped ? -spk->AB[i] : spk->AB[i];
     } else {
@@ -1374,6 +1443,11 @@ Basis make_basis(int nshell, const int* l, const int* pure, const int* nprim, co
 
 extern "C" {
 
+int oracle_set_tight_centre(int on) {
+  g_tight_centre = on ? 1 : 0;
+  return 0;
+}
+
 int oracle_set_operator(int oper, double omega) {
   if (oper < 0 || oper > 2 || omega < 0) return -1;
   g_oper = oper;
@@ -1483,6 +1557,44 @@ int64_t oracle_eri_quartets_ld(int nshell, const int* l, const int* pure, const 
 #pragma omp parallel
   {
     EngineT<long double> e;
+    e.set_precision(precision);
+    e.screening = screening;
+    e.uniform_cart = uniform_cart != 0;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t q = 0; q < nq; ++q) {
+      const Shell& s1 = B.shells[quartets[4 * q + 0]];
+      const Shell& s2 = B.shells[quartets[4 * q + 1]];
+      const Shell& s3 = B.shells[quartets[4 * q + 2]];
+      const Shell& s4 = B.shells[quartets[4 * q + 3]];
+      const bool nz = e.compute2(s1, s2, s3, s4);
+      const int64_t sz = (int64_t)s1.size() * s2.size() * s3.size() * s4.size();
+      if (nonzero) nonzero[q] = nz ? 1 : 0;
+      for (int64_t k = 0; k < sz; ++k) out[offsets[q] + k] = nz ? (double)e.result[k] : 0.0;
+    }
+  }
+  return total;
+}
+
+// The same in quad precision (__float128).
+int64_t oracle_eri_quartets_q(int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                               const double* coeff, const double* origin, int64_t nq, const int32_t* quartets,
+                               double precision, int screening, int uniform_cart, double* out, int64_t* offsets,
+                               int8_t* nonzero, int nthreads) {
+  Basis B = make_basis(nshell, l, pure, nprim, alpha, coeff, origin);
+  int64_t total = 0;
+  for (int64_t q = 0; q < nq; ++q) {
+    offsets[q] = total;
+    int64_t sz = 1;
+    for (int k = 0; k < 4; ++k) sz *= B.shells[quartets[4 * q + k]].size();
+    total += sz;
+  }
+  if (!out) return total;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    EngineT<quad> e;
     e.set_precision(precision);
     e.screening = screening;
     e.uniform_cart = uniform_cart != 0;

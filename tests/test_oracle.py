@@ -228,6 +228,38 @@ def test_oracle_ld_resolves_hrr_cancellation():
     assert np.abs(vl - vb).max() <= 1e-15
 
 
+def test_oracle_quad_vs_brute_force_ill_conditioned():
+    """The parity arbiter: the restatement in quad precision against the
+    brute force on a distant (ff|ff) pair whose contractions mix diffuse and
+    tight primitives on both shells.  There the reference's FP64 algorithm is
+    off by more than 1e-6 (5e-4 of the largest integral) and long double by
+    more than 1e-9 in either build orientation; quad precision matches the
+    brute force to 1e-17."""
+    b = _Arr([(3, [(0.04, 0.6), (3000.0, 0.5)], (0, 0, 0), True),
+              (3, [(0.05, 0.7), (2500.0, 0.4)], (9.0, 0.7, -0.4), True)])
+    q = [[0, 1, 0, 1]]
+    vb = oracle.eri_brute(b, q)
+    vd, _, _ = oracle.eri_quartets(b, q, precision=0.0)
+    vl, _, _ = oracle.eri_quartets_ld(b, q, precision=0.0, tight_centre=True)
+    vq, _, _ = oracle.eri_quartets_quad(b, q, precision=0.0)
+    assert np.abs(vd - vb).max() > 1e-6
+    assert np.abs(vl - vb).max() > 1e-9
+    assert np.abs(vq - vb).max() <= 1e-17
+
+
+def test_oracle_quad_matches_double():
+    """On well-conditioned quartets quad and double agree to 1e-13, with the
+    same layout and screened-out flags."""
+    rng = np.random.default_rng(13)
+    b = rand_shells(rng, [0, 1, 2, 3, 2, 1], kmax=2)
+    q = np.array(list(itertools.product(range(6), repeat=4)), np.int32)
+    q = q[rng.choice(len(q), 60, replace=False)]
+    vd, od, nd = oracle.eri_quartets(b, q)
+    vq, oq, nq = oracle.eri_quartets_quad(b, q)
+    assert np.array_equal(od, oq) and np.array_equal(nd, nq)
+    assert np.abs(vd - vq).max() <= 1e-13
+
+
 def test_oracle_ld_matches_double():
     """On well-conditioned quartets (random shells near each other, K <= 3)
     the two precisions agree to 1e-13."""
