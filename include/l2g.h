@@ -422,6 +422,23 @@ int l2g_quartet_compute(l2g_quartet_engine engine, const int* l, const int* pure
                         const double* nonsph, double precision, int screening, double scale, int uniform_cart,
                         int oper, double omega, double* out, int8_t* nonzero);
 
+/* The same evaluation for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in libint: it is l2g_quartet_compute without an engine.  It
+ * calls no hipMalloc, hipFree or hipHostMalloc: its device memory is a
+ * fixed array of the library (about 9.5 MB per GPU, held until the process
+ * ends) that l2g_init sets up for its device, and its host staging is
+ * static per thread.  Limits: l <= 3 and at most L2G_NOALLOC_MAX_NPRIM
+ * primitives per shell; beyond them the call fails with an error naming
+ * the limit.  Calls on one GPU share its array and are serialised by the
+ * library; the array keeps the last call's inputs and integrals.  The
+ * arguments and results are those of l2g_quartet_compute. */
+#define L2G_NOALLOC_MAX_NPRIM 16
+int l2g_quartet_compute_noalloc(l2g_context ctx, const int* l, const int* pure, const int* nprim,
+                                const double* alpha, const double* coeff, const double* origin, const double* ln_scr,
+                                const double* nonsph, double precision, int screening, double scale, int uniform_cart,
+                                int oper, double omega, double* out, int8_t* nonzero);
+
 /* One-shot convenience: plan, execute, download, destroy.  offsets and
  * nonzero may be NULL. */
 int l2g_eri_compute(l2g_basis basis, int64_t nquartets, const int32_t* quartets, double precision, int screening,

@@ -110,6 +110,9 @@ ABI = {
     "l2g_quartet_compute": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp, ctypes.c_double,
                                            ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                            _f64p, _vp]),
+    "l2g_quartet_compute_noalloc": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _vp, _vp,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_double, _f64p, _vp]),
 }
 
 
@@ -671,6 +674,28 @@ class QuartetEngine:
             self._release()
         except Exception:
             pass
+
+
+NOALLOC_MAX_NPRIM = 16  # L2G_NOALLOC_MAX_NPRIM of include/l2g.h
+
+
+def quartet_compute_noalloc(ctx: Context, shells, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,
+                            scale=1.0, uniform_cart=False, oper=0, omega=0.0, ln_scr=None, nonsph=None):
+    """QuartetEngine.compute without an engine: l2g_quartet_compute_noalloc,
+    which allocates no device or pinned memory (the library's fixed scratch
+    array of the context's device; l <= 3, at most NOALLOC_MAX_NPRIM
+    primitives per shell).  Returns the integrals and the nonzero flag."""
+    a = BasisArrays.from_shells(shells)
+    n = int(np.prod(a.sizes()))
+    out = np.zeros(max(n, 1))
+    nz = np.zeros(1, np.int8)
+    ls = None if ln_scr is None else np.ascontiguousarray(ln_scr, np.float64)
+    ns = None if nonsph is None else np.ascontiguousarray(nonsph, np.float64)
+    _check(lib().l2g_quartet_compute_noalloc(ctx.handle, a.l, a.pure, a.nprim, a.alpha, a.coeff, a.origin,
+                                             None if ls is None else ls.ctypes.data,
+                                             None if ns is None else ns.ctypes.data, precision, screening, scale,
+                                             int(uniform_cart), oper, omega, out, nz.ctypes.data))
+    return out[:n], bool(nz[0])
 
 
 def compute_quartets(basis: Basis, quartets, precision=np.finfo(float).eps, screening=SCREEN_ORIGINAL,

@@ -310,6 +310,7 @@ int l2g_init(int device, l2g_context* out) {
           for (int y = l - x; y >= 0; --y, ++c) sh[(l * 7 + m + l) * 10 + c] = l2g_solid_harmonics_coeff(l, m, x, y, l - x - y);
       }
     generic_init(sh.data());
+    noalloc_setup(device);  // this device's address of the no-allocation scratch array
     *out = ctx.release();
   });
 }

@@ -588,4 +588,46 @@ struct PairBuilder {
   }
 };
 
+// l2g_quartet.cpp: one shell quartet per call.  The staged input block of a
+// call (byte offsets; np primitive pairs of both shell pairs together), then
+// one double for the screened-out flag and the integrals.
+inline constexpr size_t align32(size_t x) { return (x + 31) & ~size_t(31); }
+struct QuartetLayout {
+  size_t ints, lls, ab, P, PA, aux, ns, total;
+  constexpr explicit QuartetLayout(size_t np)
+      : ints(0),                                      // bra, ket, meta, pair_first[2], pair_np[2]
+        lls(align32(8 * sizeof(int))),                // out_off, qidx
+        ab(align32(lls + 2 * sizeof(long long))),     // AB of both pairs
+        P(align32(ab + 6 * sizeof(double))),
+        PA(align32(P + np * 4 * sizeof(double))),
+        aux(align32(PA + np * 4 * sizeof(double))),
+        ns(align32(aux + np * 2 * sizeof(double))),
+        total(align32(ns + np * sizeof(double))) {}
+};
+constexpr int kQuartetBlock = 128;  // threads of the one block (the kernels' launch bound)
+
+// Where one quartet evaluation works: the device block (inputs at most
+// in_bytes, then the flag slot and out_doubles integrals), the lane
+// scratch (ylane doubles per lane, kQuartetBlock lanes) and host staging
+// of the inputs (in_bytes + one double) and of the flag and integrals.
+struct QuartetBuffers {
+  int max_l = 0, max_nprim = 0;
+  unsigned char* d = nullptr;
+  double* ybuf = nullptr;
+  int64_t ylane = 0;
+  unsigned char* hin = nullptr;
+  double* hout = nullptr;
+  size_t in_bytes = 0, out_doubles = 0;
+  const char* owner = "quartet engine";  // names the limits in error messages
+};
+// lane scratch per lane of the largest register-kernel class with l <= max_l
+int64_t quartet_ylane(int max_l);
+void quartet_compute_impl(l2g_context_s* ctx, const QuartetBuffers& qb, const int* l, const int* pure,
+                          const int* nprim, const double* alpha, const double* coeff, const double* origin,
+                          const double* ln_scr, const double* nonsph, double precision, int screening, double scale,
+                          int uniform_cart, int oper, double omega, double* out, int8_t* nonzero);
+// l2g_noalloc.hip: resolves this device's copy of the no-allocation
+// entry point's scratch array (called by l2g_init with the device current)
+void noalloc_setup(int device);
+
 }  // namespace l2g_impl
