@@ -292,6 +292,20 @@ int l2g_fock_schwarz(l2g_fock fock, double* K);
  * J_ab = sum (ab|cd) D_cd, K_ab = sum (ac|bd) D_cd, G = 2J - K. */
 int l2g_fock_build(l2g_fock fock, const double* D, double precision, double* J, double* K, double* G);
 
+/* Deterministic J/K accumulation (on != 0): every contribution is added as
+ * a 128-bit fixed-point number (resolution 2^-80) with integer atomics, so
+ * the sum does not depend on the order in which the GPU's threads arrive
+ * and a build returns the same bits on every run with the same settings.
+ * The reference sums per-thread Fock matrices in a fixed order
+ * (tests/hartree-fock/hartree-fock++.cc:1752-1755); FP64 atomics (the
+ * default, on = 0) round in a run-dependent order.  Costs a second word
+ * plane of the J and K accumulators and about twice the atomics. */
+int l2g_fock_set_deterministic(l2g_fock fock, int on);
+
+/* Screened unique quartets a build with this density and precision would
+ * digest (the count pass of l2g_fock_build, without the build). */
+int l2g_fock_count(l2g_fock fock, const double* D, double precision, int64_t* nquartets);
+
 /* A build produces and digests its screened quartet list in windows of at
  * most `items` quartets (0: the default, 2^26), so that the list of a large
  * molecule never has to be held in device memory at once.  No counterpart
@@ -351,6 +365,8 @@ typedef struct {
   int screening;               /* primitive screening; 0 = L2G_SCREEN_SCHWARZINF */
   double pair_threshold;       /* significant-pair threshold; 0 = 1e-12 */
   double max_engine_precision; /* shell-pair precision; 0 = epsilon / 1e10 */
+  int deterministic;           /* 1: fixed-point J/K accumulation (as l2g_fock_set_deterministic),
+                                  so a request's result has the same bits alone and in any mix */
 } l2g_server_config;
 
 #define L2G_REQ_QUEUED 0   /* submitted, not yet admitted */

@@ -104,6 +104,8 @@ ABI = {
                                         ctypes.POINTER(ctypes.c_int64)]),
     "l2g_server_destroy": (ctypes.c_int, [_vp]),
     "l2g_fock_set_window": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "l2g_fock_set_deterministic": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_fock_count": (ctypes.c_int, [_vp, _f64p, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_fock_stats_windows": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_quartet_engine_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "l2g_quartet_engine_destroy": (ctypes.c_int, [_vp]),
@@ -120,7 +122,7 @@ class ServerConfig(ctypes.Structure):
     """l2g_server_config of include/l2g.h."""
     _fields_ = [("device", ctypes.c_int), ("hbm_bytes", ctypes.c_double), ("max_active", ctypes.c_int),
                 ("round_quartets", ctypes.c_int64), ("screening", ctypes.c_int), ("pair_threshold", ctypes.c_double),
-                ("max_engine_precision", ctypes.c_double)]
+                ("max_engine_precision", ctypes.c_double), ("deterministic", ctypes.c_int)]
 
 _lib = None
 
@@ -593,6 +595,20 @@ class FockBuilder:
         """Quartets with Schwarz bound below `threshold` run in FP32."""
         _check(lib().l2g_fock_set_fp32_threshold(self._h, threshold))
 
+    def set_deterministic(self, on: bool = True) -> None:
+        """Fixed-point J/K accumulation: the same bits on every run
+        (l2g_fock_set_deterministic)."""
+        _check(lib().l2g_fock_set_deterministic(self._h, int(bool(on))))
+
+    def count(self, D, precision: float = 1e-12) -> int:
+        """Screened unique quartets a build of D at this precision digests,
+        from the build's count pass alone (l2g_fock_count)."""
+        D = np.ascontiguousarray(D, np.float64)
+        assert D.shape == (self.nbf, self.nbf)
+        n = ctypes.c_int64()
+        _check(lib().l2g_fock_count(self._h, D.reshape(-1), precision, ctypes.byref(n)))
+        return n.value
+
     def set_window(self, items: int) -> None:
         """Quartets per window of the build's list (0: the default)."""
         _check(lib().l2g_fock_set_window(self._h, int(items)))
@@ -860,9 +876,10 @@ class Server:
     launch rounds."""
 
     def __init__(self, device: int = 0, hbm_bytes: float = 0.0, max_active: int = 0, round_quartets: int = 0,
-                 screening: int = 0, pair_threshold: float = 0.0, max_engine_precision: float = 0.0):
+                 screening: int = 0, pair_threshold: float = 0.0, max_engine_precision: float = 0.0,
+                 deterministic: bool = False):
         cfg = ServerConfig(device, hbm_bytes, max_active, round_quartets, screening, pair_threshold,
-                           max_engine_precision)
+                           max_engine_precision, int(bool(deterministic)))
         h = _vp()
         _check(lib().l2g_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -936,13 +936,13 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
                         E(f"jcol += d12_{i1}_{i2} * {v}; j12_{i1}_{i2} += d34 * {v};")
                         E(f"k13_{i1} += d24_{i2} * {v}; k14_{i1} += d23_{i2} * {v};")
                         E(f"k24_{i2} += d13_{i1} * {v}; k23_{i2} += d14_{i1} * {v};")
-                E("unsafeAtomicAdd(&jgm[bk3 * N + bk4], deg * jcol);")
+                E("l2g_acc(&jgm[bk3 * N + bk4], deg * jcol, A.fx_hoff);")
                 for i1 in range(d1):
-                    E(f"unsafeAtomicAdd(&kgm[ba1_{i1} * N + bk3], deg * k13_{i1});")
-                    E(f"unsafeAtomicAdd(&kgm[ba1_{i1} * N + bk4], deg * k14_{i1});")
+                    E(f"l2g_acc(&kgm[ba1_{i1} * N + bk3], deg * k13_{i1}, A.fx_hoff);")
+                    E(f"l2g_acc(&kgm[ba1_{i1} * N + bk4], deg * k14_{i1}, A.fx_hoff);")
                 for i2 in range(d2):
-                    E(f"unsafeAtomicAdd(&kgm[ba2_{i2} * N + bk4], deg * k24_{i2});")
-                    E(f"unsafeAtomicAdd(&kgm[ba2_{i2} * N + bk3], deg * k23_{i2});")
+                    E(f"l2g_acc(&kgm[ba2_{i2} * N + bk4], deg * k24_{i2}, A.fx_hoff);")
+                    E(f"l2g_acc(&kgm[ba2_{i2} * N + bk3], deg * k23_{i2}, A.fx_hoff);")
             else:
                 for i1 in range(d1):
                     for i2 in range(d2):
@@ -1004,8 +1004,8 @@ def emit_class(plan: ClassPlan, block=128, mode='eri', waves=0, real='double', a
     if mode == 'fock':
         for i1 in range(ncart(la)):
             for i2 in range(ncart(lb)):
-                E(f"if ({i1} < st.n1 && {i2} < st.n2) unsafeAtomicAdd(&jgm[(bf.x + st.pos(st.l1, st.p1, {i1})) * N + "
-                  f"bf.y + st.pos(st.l2, st.p2, {i2})], deg * j12_{i1}_{i2});")
+                E(f"if ({i1} < st.n1 && {i2} < st.n2) l2g_acc(&jgm[(bf.x + st.pos(st.l1, st.p1, {i1})) * N + "
+                  f"bf.y + st.pos(st.l2, st.p2, {i2})], deg * j12_{i1}_{i2}, A.fx_hoff);")
     E.end()  # quartet loop
     E.ind -= 1
     E("}")

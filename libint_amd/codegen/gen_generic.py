@@ -169,7 +169,9 @@ def class_program(la, lb, lc, ld):
         return min(n, 256, capped if capped >= min(n, 32) else (LDS_DOUBLES - start) // size)
 
     chk = per_pass(ne, w0, wrow)
-    dg = nab + (ncart(la) + ncart(lb)) * (ncart(lc) + ncart(ld))
+    # digest accumulators: a low and a high word each, for the fixed-point
+    # (deterministic) accumulation; the FP64 accumulation uses the first half
+    dg = 2 * (nab + (ncart(la) + ncart(lb)) * (ncart(lc) + ncart(ld)))
     wcol = odd(hb_nws + nab)
     wb0 = w0 + dg
     chb = per_pass(ncc, wb0, wcol)

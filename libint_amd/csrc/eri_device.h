@@ -27,6 +27,45 @@ struct L2gMatDesc {
   long long n;
 };
 
+// Deterministic J/K accumulation.  A contribution v enters as the 128-bit
+// fixed-point number floor(v 2^80), held as hi = floor(v 2^16) (two's
+// complement) and lo = frac(v 2^16) 2^64.  Integer additions commute, so the
+// accumulated pair does not depend on the order in which the contributions
+// arrive: a build gives the same bits on every run and in every batch.
+// Resolution 2^-80 per contribution; the sum must stay below 2^37 in
+// magnitude for the conversion back to be exact in its high word.
+__device__ __forceinline__ void l2g_fx_split(double v, unsigned long long& hi, unsigned long long& lo) {
+  const double y = v * 65536.0;  // exact: a power of two
+  const double h = floor(y);
+  hi = (unsigned long long)(long long)h;
+  lo = (unsigned long long)((y - h) * 18446744073709551616.0);  // y - h is exact, in [0, 1)
+}
+
+// (hi, lo) += v, with the carry out of the low word into the high word
+__device__ __forceinline__ void l2g_fx_add(unsigned long long* hi, unsigned long long* lo, double v) {
+  unsigned long long h, l;
+  l2g_fx_split(v, h, l);
+  const unsigned long long old = atomicAdd(lo, l);
+  h += (old + l < old) ? 1ull : 0ull;
+  if (h) atomicAdd(hi, h);
+}
+
+__device__ __forceinline__ double l2g_fx_value(unsigned long long hi, unsigned long long lo) {
+  return (double)(long long)hi * (1.0 / 65536.0) + (double)lo * 0x1p-80;
+}
+
+// J/K accumulator update: an FP64 atomic add (hoff == 0), or the fixed-point
+// pair whose low word is the 64 bits at p and whose high word is hoff words
+// further on (a second plane of the accumulator arena)
+__device__ __forceinline__ void l2g_acc(double* p, double v, long long hoff) {
+  if (hoff == 0) {
+    unsafeAtomicAdd(p, v);
+    return;
+  }
+  unsigned long long* lo = reinterpret_cast<unsigned long long*>(p);
+  l2g_fx_add(lo + hoff, lo, v);
+}
+
 // sum over the 64 lanes of the wave (butterfly), returned in every lane
 template <class T>
 __device__ __forceinline__ T l2g_wave_sum(T v) {
@@ -72,6 +111,7 @@ struct L2gEriArgs {
   const L2gMatDesc* __restrict__ mats;   // matrix block of each request
   const int* __restrict__ q_req;         // request of each work item (null: request 0)
   const int2* __restrict__ q_prange;     // primitive quartets [x, y) of each work item (null: all)
+  long long fx_hoff;                     // 0: FP64 atomics into jg / kg; else fixed point (l2g_acc)
   // Schwarz kernels: optional sqrt(max |[e0|e0]|) of the pre-HRR diagonal
   double* __restrict__ out2;
 };

@@ -11,14 +11,25 @@
 // Thread (i, j) with i <= j owns both mirrored elements, so no element is
 // read after another thread wrote it.
 __global__ void __launch_bounds__(256) l2g_fock_finish_kernel(double* __restrict__ jg, double* __restrict__ kg,
-                                                              double* __restrict__ g, long long N) {
+                                                              double* __restrict__ g, long long N, long long hoff) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= N * N) return;
   const long long i = idx / N, j = idx - i * N;
   if (j < i) return;
   const long long a = i * N + j, b = j * N + i;
-  const double Jv = 0.25 * (jg[a] + jg[b]);
-  const double Kv = 0.125 * (kg[a] + kg[b]);
+  double ja, jb, ka, kb;
+  if (hoff) {  // fixed-point accumulators: low words in place, high words hoff further on
+    const unsigned long long* jw = reinterpret_cast<const unsigned long long*>(jg);
+    const unsigned long long* kw = reinterpret_cast<const unsigned long long*>(kg);
+    ja = l2g_fx_value(jw[a + hoff], jw[a]);
+    jb = l2g_fx_value(jw[b + hoff], jw[b]);
+    ka = l2g_fx_value(kw[a + hoff], kw[a]);
+    kb = l2g_fx_value(kw[b + hoff], kw[b]);
+  } else {
+    ja = jg[a], jb = jg[b], ka = kg[a], kb = kg[b];
+  }
+  const double Jv = 0.25 * (ja + jb);
+  const double Kv = 0.125 * (ka + kb);
   jg[a] = Jv;
   jg[b] = Jv;
   kg[a] = Kv;
@@ -139,12 +150,12 @@ void launch_deriv1_combine(const double* raw, const long long* raw_off, const lo
   LAUNCHCHK("l2g_deriv1_combine_kernel");
 }
 
-void launch_fock_finish(double* jg, double* kg, double* g, long long N, hipStream_t s) {
+void launch_fock_finish(double* jg, double* kg, double* g, long long N, long long hoff, hipStream_t s) {
   if (N <= 0) return;
   const long long n = N * N;
   const int block = 256;
   const long long grid = (n + block - 1) / block;
-  hipLaunchKernelGGL(l2g_fock_finish_kernel, dim3((unsigned)grid), dim3(block), 0, s, jg, kg, g, N);
+  hipLaunchKernelGGL(l2g_fock_finish_kernel, dim3((unsigned)grid), dim3(block), 0, s, jg, kg, g, N, hoff);
   LAUNCHCHK("l2g_fock_finish_kernel");
 }
 

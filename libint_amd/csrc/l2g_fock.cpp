@@ -464,7 +464,8 @@ struct PairDev {
 // Kernel arguments common to every class of one Fock launch round.
 L2gEriArgs fock_args(l2g_context_s* ctx, int method, const int* pair_first, const int* pair_np, const double* pair_AB,
                      const double4* P, const double4* PA, const double2* aux, const double* ns, double* dens,
-                     double* jg, double* kg, const L2gMatDesc* mats, double* ybuf, int64_t ystride) {
+                     double* jg, double* kg, const L2gMatDesc* mats, double* ybuf, int64_t ystride,
+                     long long fx_hoff) {
   L2gEriArgs A;
   std::memset(&A, 0, sizeof(A));
   A.pair_first = pair_first;
@@ -484,6 +485,7 @@ L2gEriArgs fock_args(l2g_context_s* ctx, int method, const int* pair_first, cons
   A.mats = mats;
   A.ybuf = ybuf;
   A.ystride = ystride;
+  A.fx_hoff = fx_hoff;
   return A;
 }
 
@@ -603,6 +605,7 @@ struct l2g_fock_s {
   FockSetup setup;
   long long window_items = L2G_FOCK_WINDOW_ITEMS;
   double fp32_threshold = 0;  // FP32 selection threshold (fock_enumerate)
+  bool deterministic = false;  // fixed-point J/K accumulation (l2g_fx_add)
   int64_t last_nq = 0, last_nprim = 0, last_nq32 = 0;
   int64_t last_windows = 0;  // windows the last build's quartet list came in
   float last_ms = 0;
@@ -711,53 +714,83 @@ int l2g_fock_schwarz(l2g_fock f, double* K) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// The count pass of a build: shell-block norms of D, then the screened
+// unique quartets counted on the GPU per enumeration block (f->enumst).
+FockEnumArgs fock_count_pass(l2g_fock_s* f, const double* D, double precision) {
+  const FockSetup& fs = f->setup;
+  l2g_context_s* ctx = fs.B->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = fs.n, N = fs.nbf;
+  hipStream_t st = ctx->stream;
+  // shell-block infinity norms of D (compute_shellblock_norm)
+  std::vector<double> Dn((size_t)n * n, 0.0);
+  parallel_for(n, [&](int s1) {
+    for (int s2 = 0; s2 < n; ++s2) {
+      double m = 0;
+      for (int i = fs.bf1st[s1]; i < fs.bf1st[s1 + 1]; ++i)
+        for (int j = fs.bf1st[s2]; j < fs.bf1st[s2 + 1]; ++j) m = std::max(m, std::fabs(D[(size_t)i * N + j]));
+      Dn[(size_t)s1 * n + s2] = m;
+    }
+  });
+  double dmax = 0;
+  for (double v : Dn) dmax = std::max(dmax, v);
+  const double needed = dmax > 0 ? precision / dmax : precision;
+  if (needed < fs.max_engine_precision)
+    throw std::invalid_argument("requested Fock precision needs more engine precision than the shell-pair data hold");
+  upload(f->e_Dn, Dn, st);
+  // screened unique quartets on the GPU, sorted by class and depth
+  FockEnumArgs E;
+  E.ps1 = f->e_ps1.as<int>();
+  E.ps2 = f->e_ps2.as<int>();
+  E.pK = f->e_pK.as<double>();
+  E.pS = f->e_pS.as<double>();
+  E.pid = f->e_pid.as<int>();
+  E.pswap = f->e_pswap.as<unsigned char>();
+  E.l = f->e_l.as<int>();
+  E.pure = f->e_pure.as<int>();
+  E.bf1st = f->e_bf1st.as<int>();
+  E.np = f->e_np.as<int>();
+  E.Dn = f->e_Dn.as<double>();
+  E.n = n;
+  E.npairs = f->npairs;
+  E.clsmap = f->e_cls.as<int>();
+  E.cls32map = f->e_cls32.as<int>();
+  E.precision = precision;
+  E.needed = needed;
+  E.fp32_thr = f->fp32_threshold;
+  E.hrr_factor = L2G_FP32_HRR_FACTOR;
+  FockEnumState& es = f->enumst;
+  fock_count_device(E, es, st);  // synchronises; Dn may go
+  return E;
+}
+
+}  // namespace
+
+extern "C" {
+
+int l2g_fock_count(l2g_fock f, const double* D, double precision, int64_t* nquartets) {
+  return guarded([&] {
+    if (!f || !D || !nquartets) throw std::invalid_argument("l2g_fock_count: null argument");
+    (void)fock_count_pass(f, D, precision);
+    long long n = 0;
+    for (long long c : f->enumst.block_count) n += c;
+    *nquartets = n;
+  });
+}
+
 int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, double* K, double* G) {
   return guarded([&] {
     if (!f || !D) throw std::invalid_argument("l2g_fock_build: null argument");
     const FockSetup& fs = f->setup;
     l2g_context_s* ctx = fs.B->ctx;
-    HIPCHK(hipSetDevice(ctx->device));
-    const int n = fs.n, N = fs.nbf;
+    const int N = fs.nbf;
     hipStream_t st = ctx->stream;
-    // shell-block infinity norms of D (compute_shellblock_norm)
-    std::vector<double> Dn((size_t)n * n, 0.0);
-    parallel_for(n, [&](int s1) {
-      for (int s2 = 0; s2 < n; ++s2) {
-        double m = 0;
-        for (int i = fs.bf1st[s1]; i < fs.bf1st[s1 + 1]; ++i)
-          for (int j = fs.bf1st[s2]; j < fs.bf1st[s2 + 1]; ++j) m = std::max(m, std::fabs(D[(size_t)i * N + j]));
-        Dn[(size_t)s1 * n + s2] = m;
-      }
-    });
-    double dmax = 0;
-    for (double v : Dn) dmax = std::max(dmax, v);
-    const double needed = dmax > 0 ? precision / dmax : precision;
-    if (needed < fs.max_engine_precision)
-      throw std::invalid_argument("requested Fock precision needs more engine precision than the shell-pair data hold");
-    upload(f->e_Dn, Dn, st);
-    // screened unique quartets on the GPU, sorted by class and depth
-    FockEnumArgs E;
-    E.ps1 = f->e_ps1.as<int>();
-    E.ps2 = f->e_ps2.as<int>();
-    E.pK = f->e_pK.as<double>();
-    E.pS = f->e_pS.as<double>();
-    E.pid = f->e_pid.as<int>();
-    E.pswap = f->e_pswap.as<unsigned char>();
-    E.l = f->e_l.as<int>();
-    E.pure = f->e_pure.as<int>();
-    E.bf1st = f->e_bf1st.as<int>();
-    E.np = f->e_np.as<int>();
-    E.Dn = f->e_Dn.as<double>();
-    E.n = n;
-    E.npairs = f->npairs;
-    E.clsmap = f->e_cls.as<int>();
-    E.cls32map = f->e_cls32.as<int>();
-    E.precision = precision;
-    E.needed = needed;
-    E.fp32_thr = f->fp32_threshold;
-    E.hrr_factor = L2G_FP32_HRR_FACTOR;
+    FockEnumArgs E = fock_count_pass(f, D, precision);
     FockEnumState& es = f->enumst;
-    fock_count_device(E, es, st);  // synchronises; Dn may go
     const int ncls = nfock_classes();
     f->last_nq = 0;
     f->last_nq32 = 0;
@@ -769,12 +802,15 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     f->cls_ms.assign(ncls, 0.0f);
     const size_t NN = (size_t)N * N;
     f->dens.alloc(std::max<size_t>(NN, 1) * sizeof(double));
-    f->jg.alloc(std::max<size_t>(NN, 1) * sizeof(double));
-    f->kg.alloc(std::max<size_t>(NN, 1) * sizeof(double));
+    // deterministic digest: a plane of low words, then one of high words
+    const size_t planes = f->deterministic ? 2 : 1;
+    const long long hoff = f->deterministic ? (long long)NN : 0;
+    f->jg.alloc(std::max<size_t>(planes * NN, 1) * sizeof(double));
+    f->kg.alloc(std::max<size_t>(planes * NN, 1) * sizeof(double));
     if (NN) {
       HIPCHK(hipMemcpyAsync(f->dens.p, D, NN * sizeof(double), hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemsetAsync(f->jg.p, 0, NN * sizeof(double), st));
-      HIPCHK(hipMemsetAsync(f->kg.p, 0, NN * sizeof(double), st));
+      HIPCHK(hipMemsetAsync(f->jg.p, 0, planes * NN * sizeof(double), st));
+      HIPCHK(hipMemsetAsync(f->kg.p, 0, planes * NN * sizeof(double), st));
     }
     if (f->ev.empty()) {
       f->ev.assign(2 * ncls + 2, nullptr);
@@ -840,7 +876,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       L2gEriArgs A = fock_args(ctx, fs.method, p.first.as<int>(), p.np.as<int>(), p.AB.as<double>(),
                                p.P.as<double4>(), p.PA.as<double4>(), p.aux.as<double2>(), p.ns.as<double>(),
                                f->dens.as<double>(), f->jg.as<double>(), f->kg.as<double>(), f->mats.as<L2gMatDesc>(),
-                               f->ybuf.as<double>(), ystride);
+                               f->ybuf.as<double>(), ystride, hoff);
       hipEvent_t e0 = f->ev[2 * ncls], e1 = f->ev[2 * ncls + 1];
       HIPCHK(hipEventRecord(e0, st));
       fock_launch_all(ctx, A, big, false, grid, block, st, f->ev.data(), &slots);
@@ -852,12 +888,19 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       for (int c = 0; c < ncls; ++c)
         if (big.count[c]) f->cls_ms[c] += event_ms_or(f->ev[2 * c], f->ev[2 * c + 1], 0.0f);
     }
-    launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, st);
+    launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, hoff, st);
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) throw L2gError(L2G_E_LAUNCH, std::string("Fock kernels failed: ") + hipGetErrorString(e));
     if (J && NN) HIPCHK(hipMemcpy(J, f->jg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (K && NN) HIPCHK(hipMemcpy(K, f->kg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (G && NN) HIPCHK(hipMemcpy(G, f->dens.p, NN * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int l2g_fock_set_deterministic(l2g_fock f, int on) {
+  return guarded([&] {
+    if (!f) throw std::invalid_argument("l2g_fock_set_deterministic: null Fock builder");
+    f->deterministic = on != 0;
   });
 }
 
@@ -1021,6 +1064,7 @@ struct l2g_server_s {
   l2g_context admit = nullptr;  // Schwarz kernels and uploads of admissions
   // device arenas shared by all admitted requests
   DevBuf a_first, a_np, a_AB, a_P, a_PA, a_aux, a_ns, a_dens, a_jg, a_kg, a_mats, ybuf;
+  long long fx_hoff = 0;  // deterministic digest: words from a low word to its high word (0: FP64 atomics)
   FreeList fl_pair, fl_prim, fl_mat;
   int64_t ystride = 0;
   int grid = 1;
@@ -1138,6 +1182,10 @@ void upload_request(l2g_server_s* S, ServerRequest* r) {
   if (r->nmat) {
     HIPCHK(hipMemsetAsync(S->a_jg.as<double>() + r->mat_off, 0, r->nmat * sizeof(double), st));
     HIPCHK(hipMemsetAsync(S->a_kg.as<double>() + r->mat_off, 0, r->nmat * sizeof(double), st));
+    if (S->fx_hoff) {  // the high-word plane of the deterministic digest
+      HIPCHK(hipMemsetAsync(S->a_jg.as<double>() + S->fx_hoff + r->mat_off, 0, r->nmat * sizeof(double), st));
+      HIPCHK(hipMemsetAsync(S->a_kg.as<double>() + S->fx_hoff + r->mat_off, 0, r->nmat * sizeof(double), st));
+    }
   }
   for (auto& it : r->work.cls) {
     for (int& x : it.bra) x += (int)r->pair_off;
@@ -1281,7 +1329,7 @@ void run_round(l2g_server_s* S, const std::vector<ServerRequest*>& act) {
   L2gEriArgs A = fock_args(ctx, S->cfg.screening, S->a_first.as<int>(), S->a_np.as<int>(), S->a_AB.as<double>(),
                            S->a_P.as<double4>(), S->a_PA.as<double4>(), S->a_aux.as<double2>(), S->a_ns.as<double>(),
                            S->a_dens.as<double>(), S->a_jg.as<double>(), S->a_kg.as<double>(),
-                           S->rmats[b].as<L2gMatDesc>(), S->ybuf.as<double>(), S->ystride);
+                           S->rmats[b].as<L2gMatDesc>(), S->ybuf.as<double>(), S->ystride, S->fx_hoff);
   const ItemsView rv = S->rbuf[b].view();
   std::vector<int64_t> lanes(rv.count.size(), 0), yd(rv.count.size(), 0);
   for (size_t c = 0; c < rv.count.size(); ++c) {
@@ -1303,7 +1351,7 @@ void run_round(l2g_server_s* S, const std::vector<ServerRequest*>& act) {
     double* jg = S->a_jg.as<double>() + r->mat_off;
     double* kg = S->a_kg.as<double>() + r->mat_off;
     double* g = S->a_dens.as<double>() + r->mat_off;
-    launch_fock_finish(jg, kg, g, r->nbf, st);
+    launch_fock_finish(jg, kg, g, r->nbf, S->fx_hoff, st);
     if (NN) {
       HIPCHK(hipMemcpyAsync(r->result, jg, NN * sizeof(double), hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(r->result + NN, kg, NN * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1392,17 +1440,20 @@ int l2g_server_create(const l2g_server_config* config, l2g_server* out) {
       l2g_finalize(S->exec);
       throw L2gError(rc, g_err);
     }
-    // arenas: 60% matrices (D, jg, kg: 24 bytes per element), 35% primitive
+    // arenas: 60% matrices (D, jg, kg: 24 bytes per element; 40 with the
+    // deterministic digest's second plane of jg and kg), 35% primitive
     // pairs (88 bytes), 5% shell pairs (32 bytes)
     const double hb = c.hbm_bytes;
-    const int64_t cap_mat = (int64_t)(0.60 * hb / 24), cap_prim = (int64_t)(0.35 * hb / 88),
+    const int planes = c.deterministic ? 2 : 1;
+    const int64_t cap_mat = (int64_t)(0.60 * hb / (8 + 16 * planes)), cap_prim = (int64_t)(0.35 * hb / 88),
                   cap_pair = (int64_t)(0.05 * hb / 32);
     S->fl_mat.init(cap_mat);
     S->fl_prim.init(cap_prim);
     S->fl_pair.init(cap_pair);
+    S->fx_hoff = c.deterministic ? cap_mat : 0;
     S->a_dens.alloc(cap_mat * sizeof(double));
-    S->a_jg.alloc(cap_mat * sizeof(double));
-    S->a_kg.alloc(cap_mat * sizeof(double));
+    S->a_jg.alloc(planes * cap_mat * sizeof(double));
+    S->a_kg.alloc(planes * cap_mat * sizeof(double));
     S->a_P.alloc(cap_prim * sizeof(double4));
     S->a_PA.alloc(cap_prim * sizeof(double4));
     S->a_aux.alloc(cap_prim * sizeof(double2));

@@ -12,8 +12,9 @@
 //   2. ket HRR (CD) and ket solid harmonics, one thread per bra row;
 //   3. bra HRR (AB) and bra solid harmonics, one thread per ket column,
 //      followed by the caller-order store (ERI), the J/K digest into LDS
-//      accumulators flushed with FP64 atomics (Fock), or the norm reduction
-//      (Schwarz).
+//      accumulators flushed to the J/K arena (Fock; FP64 atomics, or
+//      fixed-point words for the deterministic digest), or the norm
+//      reduction (Schwarz).
 //
 // The maths is that of the generated kernels (engine.impl.h:1306-1985 for
 // the reference's steps); only the mapping onto the GPU differs.
@@ -28,6 +29,23 @@ constexpr int kModeEri = 0, kModeFock = 1, kModeSchwarz = 2;
 __constant__ double c_sh[4][7][10];
 
 __device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Fock digest accumulators in LDS: FP64 atomics, or (fx) fixed-point pairs
+// whose high words sit hoff words after the low words (l2g_fx_add)
+__device__ __forceinline__ void lds_acc(double* p, double v, bool fx, int hoff) {
+  if (!fx) {
+    atomicAdd(p, v);
+    return;
+  }
+  unsigned long long* lo = reinterpret_cast<unsigned long long*>(p);
+  l2g_fx_add(lo + hoff, lo, v);
+}
+
+__device__ __forceinline__ double lds_value(const double* p, bool fx, int hoff) {
+  if (!fx) return *p;
+  const unsigned long long* lo = reinterpret_cast<const unsigned long long*>(p);
+  return l2g_fx_value(lo[hoff], lo[0]);
+}
 
 // F_m(T) for one m: the Chebyshev-7 interpolant of the interval holding T,
 // or the asymptotic form above T_crit (boys.h:348-358)
@@ -257,6 +275,8 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
     int4 bf = int4{0, 0, 0, 0};
     double deg = 0.0;
     double *dJ12 = nullptr, *dK13 = nullptr, *dK14 = nullptr, *dK24 = nullptr, *dK23 = nullptr;
+    const bool fx = MODE == kModeFock && A.fx_hoff != 0;  // deterministic (fixed-point) digest
+    int ndg = 0;
     if (MODE == kModeFock) {
       const L2gMatDesc md = A.mats[A.q_req ? A.q_req[qi] : 0];
       N = md.n;
@@ -268,8 +288,9 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
       dK14 = dK13 + n1 * nk1;
       dK24 = dK14 + n1 * nk2;
       dK23 = dK24 + n2 * nk2;
-      const int ndg = n1 * n2 + (n1 + n2) * (nk1 + nk2);
-      for (int i = t; i < ndg; i += kThreads) dJ12[i] = 0.0;
+      ndg = n1 * n2 + (n1 + n2) * (nk1 + nk2);
+      // fixed-point accumulation: low words here, high words ndg further on
+      for (int i = t; i < (fx ? 2 * ndg : ndg); i += kThreads) dJ12[i] = 0.0;
       __syncthreads();
     }
     const double* Dm = A.dens + moff;
@@ -309,16 +330,16 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
               const long long ba2 = bf.y + st.pos(lb, pb2, i2);
               const double v = tmp[i1 * n2 + i2];
               jcol += Dm[ba1 * N + ba2] * v;
-              atomicAdd(&dJ12[i1 * n2 + i2], d34 * v);
+              lds_acc(&dJ12[i1 * n2 + i2], d34 * v, fx, ndg);
               k13 += Dm[ba2 * N + bk4] * v;
               k14 += Dm[ba2 * N + bk3] * v;
-              atomicAdd(&dK24[i2 * nk2 + id], d13 * v);
-              atomicAdd(&dK23[i2 * nk1 + ic], d14 * v);
+              lds_acc(&dK24[i2 * nk2 + id], d13 * v, fx, ndg);
+              lds_acc(&dK23[i2 * nk1 + ic], d14 * v, fx, ndg);
             }
-            atomicAdd(&dK13[i1 * nk1 + ic], k13);
-            atomicAdd(&dK14[i1 * nk2 + id], k14);
+            lds_acc(&dK13[i1 * nk1 + ic], k13, fx, ndg);
+            lds_acc(&dK14[i1 * nk2 + id], k14, fx, ndg);
           }
-          unsafeAtomicAdd(&A.jg[moff + bk3 * N + bk4], deg * jcol);
+          l2g_acc(&A.jg[moff + bk3 * N + bk4], deg * jcol, A.fx_hoff);
         }
       }
     }
@@ -326,25 +347,27 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
       __syncthreads();
       double* jg = A.jg + moff;
       double* kg = A.kg + moff;
+      const long long hoff = A.fx_hoff;
+      auto val = [&](const double* p) { return lds_value(p, fx, ndg); };
       for (int i = t; i < n1 * n2; i += kThreads) {
         const int i1 = i / n2, i2 = i - i1 * n2;
-        unsafeAtomicAdd(&jg[(bf.x + st.pos(la, pb1, i1)) * N + bf.y + st.pos(lb, pb2, i2)], deg * dJ12[i]);
+        l2g_acc(&jg[(bf.x + st.pos(la, pb1, i1)) * N + bf.y + st.pos(lb, pb2, i2)], deg * val(&dJ12[i]), hoff);
       }
       for (int i = t; i < n1 * nk1; i += kThreads) {
         const int i1 = i / nk1, ic = i - i1 * nk1;
-        unsafeAtomicAdd(&kg[(bf.x + st.pos(la, pb1, i1)) * N + bf.z + st.pos3(ic)], deg * dK13[i]);
+        l2g_acc(&kg[(bf.x + st.pos(la, pb1, i1)) * N + bf.z + st.pos3(ic)], deg * val(&dK13[i]), hoff);
       }
       for (int i = t; i < n1 * nk2; i += kThreads) {
         const int i1 = i / nk2, id = i - i1 * nk2;
-        unsafeAtomicAdd(&kg[(bf.x + st.pos(la, pb1, i1)) * N + bf.w + st.pos4(id)], deg * dK14[i]);
+        l2g_acc(&kg[(bf.x + st.pos(la, pb1, i1)) * N + bf.w + st.pos4(id)], deg * val(&dK14[i]), hoff);
       }
       for (int i = t; i < n2 * nk2; i += kThreads) {
         const int i2 = i / nk2, id = i - i2 * nk2;
-        unsafeAtomicAdd(&kg[(bf.y + st.pos(lb, pb2, i2)) * N + bf.w + st.pos4(id)], deg * dK24[i]);
+        l2g_acc(&kg[(bf.y + st.pos(lb, pb2, i2)) * N + bf.w + st.pos4(id)], deg * val(&dK24[i]), hoff);
       }
       for (int i = t; i < n2 * nk1; i += kThreads) {
         const int i2 = i / nk1, ic = i - i2 * nk1;
-        unsafeAtomicAdd(&kg[(bf.y + st.pos(lb, pb2, i2)) * N + bf.z + st.pos3(ic)], deg * dK23[i]);
+        l2g_acc(&kg[(bf.y + st.pos(lb, pb2, i2)) * N + bf.z + st.pos3(ic)], deg * val(&dK23[i]), hoff);
       }
     }
     if (MODE == kModeSchwarz) {

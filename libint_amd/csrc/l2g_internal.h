@@ -358,7 +358,9 @@ void schwarz_impl(l2g_basis B, int64_t npairs, const int32_t* pairs, double prec
 
 // l2g_aux.hip: in-place Fock symmetrisation of one N x N block (J into jg,
 // K into kg, G = 2J - K into g)
-void launch_fock_finish(double* jg, double* kg, double* g, long long N, hipStream_t s);
+// hoff: 0 for FP64 accumulators; else the fixed-point accumulators' offset
+// from a low word to its high word (l2g_fx_add)
+void launch_fock_finish(double* jg, double* kg, double* g, long long N, long long hoff, hipStream_t s);
 
 // Solid-harmonic rows in output order (Standard or Gaussian) for l <= 3: the
 // Cartesian components and coefficients of output function f of a pure

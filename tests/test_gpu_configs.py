@@ -11,7 +11,8 @@ of Engine::compute2 / compute_2body_fock.
 * 200-atom J/K recipe: the Fock build at the recipe's atom density and the
   largest size the oracle digests in about a minute, FP64 and mixed, <= 1e-8.
 * serving: Fock-build requests of f-recipe molecules of 10 and more atoms,
-  each equal alone and in the mix, and against the oracle.
+  each bit-for-bit equal alone and in the mix (deterministic digest), and
+  against the oracle.
 
 Per-integral bar: 1e-12 absolute / 1e-10 relative, failing only if both are
 exceeded (tests/eri/test.cc:77-88), against the exact integral where the
@@ -278,15 +279,16 @@ def test_config4_fock_recipe_vs_oracle(ctx, fock_f_reference):
 @pytest.mark.timeout(900)
 def test_config5_serving_f_recipe(fock_f_reference):
     """Fock-build requests of f-recipe molecules (10, 12 and 14 atoms at the
-    200-atom recipe's density), submitted together: each result equals the
-    same request alone, and the 10-atom one equals the oracle."""
+    200-atom recipe's density), submitted together to a server with the
+    deterministic digest: each result has the bits of the same request
+    alone, and the 10-atom one equals the oracle."""
     mol0, D0, (Go, Jo, Ko, _), _, _ = fock_f_reference
     reqs = [(mol0.basis, D0)]
     for k, n in enumerate((12, 14)):
         mol = molecule.make_molecule(4, natoms=n, seed=5000 + k, same_density=True)
         reqs.append((mol.basis, density(mol.nbf, 5100 + k)))
     say(f"case: serving, f-recipe requests of 10, 12, 14 atoms, nbf {[D.shape[0] for _, D in reqs]}")
-    srv = la.Server(0, round_quartets=1 << 18)
+    srv = la.Server(0, round_quartets=1 << 18, deterministic=True)
     try:
         alone = [srv.result(srv.submit(B, D, 1e-12)) for B, D in reqs]
         ids = [srv.submit(B, D, 1e-12) for B, D in reqs]
@@ -299,9 +301,8 @@ def test_config5_serving_f_recipe(fock_f_reference):
     assert st["max_requests_per_round"] >= 2
     for a, m in zip(alone, mix):
         for x, y in zip(a[:3], m[:3]):
-            # the digest's FP64 atomics sum in an order that varies from run
-            # to run: rounding of values up to ~3e2
-            assert np.abs(x - y).max() <= 1e-14 * max(1.0, np.abs(x).max())
+            # the deterministic (fixed-point) digest: the same bits
+            assert np.array_equal(x, y)
     dG = np.abs(mix[0][0] - Go).max()
     say(f"  10-atom request vs oracle: max|G-Go| {dG:.2e}")
     assert dG <= 1e-8

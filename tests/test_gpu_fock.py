@@ -85,3 +85,37 @@ def test_fock_dense_identity_small(ctx):
     assert np.abs(J - Jd).max() < 1e-9
     assert np.abs(K - Kd).max() < 1e-9
     assert np.abs(G - (2 * Jd - Kd)).max() < 1e-9
+
+
+def test_fock_deterministic_digest(ctx):
+    """Fixed-point J/K accumulation (l2g_fock_set_deterministic) on an
+    f-recipe molecule whose build runs register, split deep and generic
+    (f) classes: two builds give the same bits; the result meets the oracle
+    at the Fock bar and the FP64-atomic build to rounding; the count pass
+    alone gives the build's quartet count."""
+    mol = molecule.make_molecule(4, natoms=3, same_density=True)
+    D = density(mol.nbf, 31)
+    say(f"case: deterministic Fock digest, f recipe 3 atoms, nbf {mol.nbf}")
+    b = la.Basis(ctx, mol.basis)
+    fb = la.FockBuilder(b)
+    Gf, Jf, Kf = fb.build(D, 1e-12)
+    nq = fb.stats()["nquartets"]
+    assert fb.count(D, 1e-12) == nq
+    fb.set_deterministic(True)
+    runs = [fb.build(D, 1e-12) for _ in range(3)]
+    cls = fb.class_stats()
+    say(f"  {nq} quartets, classes {sorted({c['cls'] for c in cls})}")
+    for r in runs[1:]:
+        for x, y in zip(runs[0], r):
+            assert np.array_equal(x, y)
+    G, J, K = runs[0]
+    for x, y in ((G, Gf), (J, Jf), (K, Kf)):
+        assert np.abs(x - y).max() <= 1e-13 * max(1.0, np.abs(y).max())
+    Go, Jo, Ko, _ = oracle.fock_2e(mol.basis, D, 1e-12, tight_centre=True)
+    say(f"  deterministic vs oracle: max|G-Go| {np.abs(G - Go).max():.2e}")
+    assert np.abs(G - Go).max() <= 1e-8 and np.abs(J - Jo).max() <= 1e-8 and np.abs(K - Ko).max() <= 1e-8
+    # windows of a few hundred quartets: the same sum of the same contributions
+    fb.set_window(500)
+    Gw, _, _ = fb.build(D, 1e-12)
+    assert fb.stats()["windows"] > 1
+    assert np.abs(Gw - G).max() <= 1e-13 * max(1.0, np.abs(G).max())

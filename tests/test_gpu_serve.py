@@ -66,6 +66,27 @@ def test_serve_alone_equals_mix():
         assert np.abs(a[2] - Ko).max() <= 1e-8
 
 
+def test_serve_deterministic_alone_equals_mix_bitwise():
+    """With the deterministic digest a request's J, K and G have the same
+    bits alone and in a mix with others."""
+    reqs = requests()[:4]
+    say(f"case: deterministic server alone vs mix, {len(reqs)} requests")
+    srv = la.Server(0, round_quartets=256, deterministic=True)
+    try:
+        alone = [srv.result(srv.submit(B, D, 1e-12)) for B, D in reqs]
+        ids = [srv.submit(B, D, 1e-12) for B, D in reqs]
+        mix = [srv.result(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["max_requests_per_round"] >= 2
+    for a, m in zip(alone, mix):
+        for x, y in zip(a[:3], m[:3]):
+            assert np.array_equal(x, y)
+    Go, _, _, _ = oracle.fock_2e(reqs[0][0], reqs[0][1], 1e-12)
+    assert np.abs(alone[0][0] - Go).max() <= 1e-8
+
+
 def test_serve_equals_single_build(ctx):
     mol = molecule.make_molecule(2, natoms=3)
     D = density(mol.nbf, 7)
