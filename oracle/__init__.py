@@ -95,6 +95,10 @@ def lib():
         L.oracle_fock_2e.argtypes = basis_args + [
             _f64p, ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
         L.oracle_fock_2e.restype = ctypes.c_int
+        L.oracle_fock_2e_arbitrated.argtypes = basis_args + [
+            _f64p, ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int,
+            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.oracle_fock_2e_arbitrated.restype = ctypes.c_int
         L.oracle_fock_pairs.argtypes = basis_args + [ctypes.c_double, _i8p, _f64p]
         L.oracle_fock_pairs.restype = ctypes.c_int
         _lib = L
@@ -290,6 +294,31 @@ def fock_2e(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0, tight_
     finally:
         L.oracle_set_tight_centre(0)
     return G, JK[0], JK[1], nq.value
+
+
+def fock_2e_exact(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0, all_quad=False):
+    """fock_2e(tight_centre=True) with the ill-conditioned quartets settled
+    in quad precision: every quartet is evaluated in FP64 and in long
+    double, and one whose two evaluations differ by more than 1e-13
+    (absolute, or relative to its largest integral above 1) is digested from
+    its quad-precision integrals.  all_quad: every quartet in quad precision
+    (slow; for small molecules).  Returns (G, J, K, n_quartets,
+    n_quartets_in_quad)."""
+    ob = OBasis.of(basis)
+    D = np.ascontiguousarray(D, np.float64)
+    N = D.shape[0]
+    G = np.zeros((N, N))
+    JK = np.zeros((2, N, N))
+    nq = ctypes.c_int64()
+    narb = ctypes.c_int64()
+    L = lib()
+    L.oracle_set_tight_centre(1)
+    try:
+        L.oracle_fock_2e_arbitrated(*ob.args(), D.reshape(-1), precision, G.reshape(-1), screening, JK.reshape(-1),
+                                    nthreads, 2 if all_quad else 1, ctypes.byref(nq), ctypes.byref(narb))
+    finally:
+        L.oracle_set_tight_centre(0)
+    return G, JK[0], JK[1], nq.value, narb.value
 
 
 def fock_pairs(basis, threshold=1e-12, tight_centre=False):

@@ -802,6 +802,7 @@ struct EngineT {
   int screening = Original;
   double scale = 1.0;
   bool uniform_cart = false;
+  int tight = -1;  // HRR build centre of equal-l pairs: -1 as g_tight_centre, 0 caller's order, 1 tighter shell
   // scratch
   ShellPair spbra, spket;
   std::vector<Real> prim[16];
@@ -827,8 +828,9 @@ template <class Real>
 bool EngineT<Real>::compute2(const Shell& tb1, const Shell& tb2, const Shell& tk1, const Shell& tk2,
                       const ShellPair* tspbra, const ShellPair* tspket) {
   // canonical reorder (engine.impl.h:1181-1208)
-  const bool swap_tbra = tb1.l < tb2.l || (g_tight_centre && second_is_centre(tb1, tb2));
-  const bool swap_tket = tk1.l < tk2.l || (g_tight_centre && second_is_centre(tk1, tk2));
+  const bool tc = tight >= 0 ? tight != 0 : g_tight_centre != 0;
+  const bool swap_tbra = tb1.l < tb2.l || (tc && second_is_centre(tb1, tb2));
+  const bool swap_tket = tk1.l < tk2.l || (tc && second_is_centre(tk1, tk2));
   const bool swap_braket = (tb1.l + tb2.l) > (tk1.l + tk2.l);
   const Shell& b1 = swap_braket ? (swap_tket ? tk2 : tk1) : (swap_tbra ? tb2 : tb1);
   const Shell& b2 = swap_braket ? (swap_tket ? tk1 : tk2) : (swap_tbra ? tb1 : tb2);
@@ -1326,8 +1328,16 @@ static void fock_setup(const std::vector<Shell>& sh, double threshold, double ma
 
 // compute_2body_fock (hartree-fock++.cc:1574-1772): returns G = 2J - K and
 // optionally J and K (J_ab = sum (ab|cd) D_cd, K_ab = sum (ac|bd) D_cd).
+// arbitrate: 0 plain (the engine as configured); 1 every quartet is also
+// evaluated in long double (11 more significand bits, so the difference
+// measures the FP64 evaluation's own error), and a quartet whose two
+// evaluations differ by more than 1e-13 (absolute, or relative to its
+// largest integral above 1) is evaluated again in quad precision and
+// digested from that; 2 every quartet in quad precision.
+// narb_out (optional) receives the quartets evaluated in quad precision.
 static void fock_build(const std::vector<Shell>& sh, const FockSetup& F, const double* D, double precision,
-                       int method, double* G, double* J, double* K, int64_t* nq_out) {
+                       int method, double* G, double* J, double* K, int64_t* nq_out, int arbitrate = 0,
+                       int64_t* narb_out = nullptr) {
   const int n = (int)sh.size();
   const int N = F.nbf;
   std::vector<double> Dn((size_t)n * n, 0.0);
@@ -1342,11 +1352,16 @@ static void fock_build(const std::vector<Shell>& sh, const FockSetup& F, const d
   for (double v : Dn) dmax = std::max(dmax, v);
   const double needed = precision / dmax;
   std::vector<double> jg((size_t)N * N, 0.0), kg((size_t)N * N, 0.0);
-  int64_t nq = 0;
-#pragma omp parallel reduction(+ : nq)
+  int64_t nq = 0, narb = 0;
+#pragma omp parallel reduction(+ : nq, narb)
   {
     Engine e;
+    EngineT<long double> e2;
+    EngineT<quad> eq;
     e.screening = method;
+    e2.screening = method;
+    eq.screening = method;
+    std::vector<double> vq;
     std::vector<double> jl((size_t)N * N, 0.0), kl((size_t)N * N, 0.0);
 #pragma omp for schedule(dynamic, 1)
     for (int s1 = 0; s1 < n; ++s1) {
@@ -1369,10 +1384,31 @@ static void fock_build(const std::vector<Shell>& sh, const FockSetup& F, const d
             if (Dn1234 * F.K[(size_t)s1 * n + s2] * F.K[(size_t)s3 * n + s4] < precision) continue;
             const int b4 = F.bf1st[s4], n4 = sh[s4].size();
             const double deg = (s1 == s2 ? 1 : 2) * (s3 == s4 ? 1 : 2) * (s1 == s3 ? (s2 == s4 ? 1 : 2) : 2);
-            e.set_precision(Dn1234 != 0. ? precision / Dn1234 : needed);
+            const double qprec = Dn1234 != 0. ? precision / Dn1234 : needed;
+            e.set_precision(qprec);
             if (!e.compute2(sh[s1], sh[s2], sh[s3], sh[s4], sp12, sp34)) continue;
             ++nq;
             const double* buf = e.result.data();
+            const size_t nf = (size_t)n1 * n2 * n3 * n4;
+            bool redo = arbitrate == 2;
+            if (arbitrate == 1) {
+              e2.set_precision(qprec);
+              e2.compute2(sh[s1], sh[s2], sh[s3], sh[s4], sp12, sp34);
+              double dmax = 0, vmax = 1;
+              for (size_t f = 0; f < nf; ++f) {
+                dmax = std::max(dmax, (double)std::fabs((long double)buf[f] - e2.result[f]));
+                vmax = std::max(vmax, std::abs(buf[f]));
+              }
+              redo = dmax > 1e-13 * vmax;
+            }
+            if (redo) {
+              eq.set_precision(qprec);
+              eq.compute2(sh[s1], sh[s2], sh[s3], sh[s4], sp12, sp34);
+              vq.resize(nf);
+              for (size_t f = 0; f < nf; ++f) vq[f] = (double)eq.result[f];
+              buf = vq.data();
+              ++narb;
+            }
             for (int f1 = 0, f = 0; f1 < n1; ++f1)
               for (int f2 = 0; f2 < n2; ++f2)
                 for (int f3 = 0; f3 < n3; ++f3)
@@ -1408,6 +1444,7 @@ static void fock_build(const std::vector<Shell>& sh, const FockSetup& F, const d
       G[(size_t)i * N + j] = 2 * Jv - Kv;
     }
   if (nq_out) *nq_out = nq;
+  if (narb_out) *narb_out = narb;
 }
 
 }  // namespace orc
@@ -1699,6 +1736,24 @@ int oracle_fock_2e(int nshell, const int* l, const int* pure, const int* nprim, 
   fock_setup(B.shells, 1e-12, std::numeric_limits<double>::epsilon() / 1e10, method, F);
   const size_t NN = (size_t)F.nbf * F.nbf;
   fock_build(B.shells, F, D, precision, method, G, JK ? JK : nullptr, JK ? JK + NN : nullptr, nq);
+  return F.nbf;
+}
+
+// oracle_fock_2e with extended-precision arbitration of the ill-conditioned
+// quartets (fock_build's arbitrate 1) or every quartet in quad precision
+// (arbitrate 2); narb receives the quartets evaluated in quad precision.
+int oracle_fock_2e_arbitrated(int nshell, const int* l, const int* pure, const int* nprim, const double* alpha,
+                              const double* coeff, const double* origin, const double* D, double precision,
+                              double* G, int method, double* JK, int nthreads, int arbitrate, int64_t* nq,
+                              int64_t* narb) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  Basis B = make_basis(nshell, l, pure, nprim, alpha, coeff, origin);
+  FockSetup F;
+  fock_setup(B.shells, 1e-12, std::numeric_limits<double>::epsilon() / 1e10, method, F);
+  const size_t NN = (size_t)F.nbf * F.nbf;
+  fock_build(B.shells, F, D, precision, method, G, JK ? JK : nullptr, JK ? JK + NN : nullptr, nq, arbitrate, narb);
   return F.nbf;
 }
 

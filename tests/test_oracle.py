@@ -395,3 +395,41 @@ def test_oracle_range_separated_vs_brute_force():
             o, _, _ = oracle.eri_quartets(arr, q, precision=0.0, oper=oper, omega=w)
             bf = oracle.eri_brute(arr, q, oper=oper, omega=w)
             assert compare(o, bf, 5e-14, 1e-9)[0] == 0, (oper, w)
+
+
+def test_parity_arbitration_rule():
+    """tests.helpers.compare_arbitrated on the ill-conditioned (ff|ff) pair
+    above and on well-conditioned quartets: values with half the
+    reference's error on the ill-conditioned quartet pass, their misses
+    counted and bounded by the reference's; a value off the exact one by
+    more than the reference's worst error on that quartet fails, and so
+    does a value off the exact one beyond the bar on a quartet the
+    reference gets right."""
+    from tests.helpers import compare_arbitrated
+    b = _Arr([(3, [(0.04, 0.6), (3000.0, 0.5)], (0, 0, 0), True),
+              (3, [(0.05, 0.7), (2500.0, 0.4)], (9.0, 0.7, -0.4), True)])
+    q = np.array([[0, 1, 0, 1]], np.int32)
+    ref, off, _ = oracle.eri_quartets(b, q, precision=0.0)
+    exact, _, _ = oracle.eri_quartets_quad(b, q, precision=0.0)
+    scale = np.abs(ref - exact).max()
+    assert scale > 1e-6
+    half = exact + 0.5 * (ref - exact)
+    det = {}
+    nfail, narb, oerr = compare_arbitrated(b, q, off, half, ref, 1e-12, 1e-10, detail=det)
+    assert nfail == 0 and narb > 0 and oerr == scale
+    assert det["beyond_bar"] > 0 and det["beyond_bar_max"] <= 0.5 * det["ref_max"] * (1 + 1e-12)
+    worse = exact.copy()
+    worse[0] += 2 * scale
+    nfail, _, _ = compare_arbitrated(b, q, off, worse, ref, 1e-12, 1e-10)
+    assert nfail == 1
+    # a well-conditioned quartet: the bar itself
+    rng = np.random.default_rng(17)
+    b2 = rand_shells(rng, [1, 2, 1, 0], kmax=2)
+    q2 = np.array([[0, 1, 2, 3]], np.int32)
+    v2, off2, _ = oracle.eri_quartets(b2, q2)
+    bad = v2.copy()
+    k = int(np.argmax(np.abs(v2)))
+    bad[k] += 1e-9 + 1e-9 * abs(v2[k])
+    nfail, narb, _ = compare_arbitrated(b2, q2, off2, bad, v2, 1e-12, 1e-10)
+    assert nfail == 1 and narb == 1
+    assert compare_arbitrated(b2, q2, off2, v2, v2, 1e-12, 1e-10)[0] == 0
