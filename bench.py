@@ -14,11 +14,14 @@ Prints one JSON line.  `roofline` is the (dd|dd) kernel's model FLOP rate
 (libint_amd/flop_model.py) over its HIP-event duration on the library's
 stream, against the MI355X FP64 peak; `cpu_baseline` times the CPU oracle
 (a restatement of libint's Engine::compute2) on a seeded sample of the same
-quartets with every host thread.  Two more legs follow the ERI step (skip
-with --no-fock): `fock`, a J/K build (compute_2body_fock on the GPU) of a
-smaller molecule of the same recipe in FP64 and in mixed precision, and
-`serve`, a Poisson stream of Fock-build requests through the request server
-with its p50/p99 latency (`p99_fock_build_ms`).
+quartets with every host thread.  More legs follow the ERI step (skip with
+--no-fock): `config1`, the plain-method 10-atom workload batched, per call
+and on the CPU; `next_rows` (attenuated Coulomb, derivatives, 3-centre,
+one-body); `eri_f`, a batch of the f recipe; `fock` and `fock_f`, J/K builds
+(compute_2body_fock on the GPU) in FP64 and in mixed precision with their
+model FLOP rate; `serve` and `serve_f`, Poisson streams of Fock-build
+requests through the request server with their p50/p99 latency
+(`p99_fock_build_ms`).
 """
 from __future__ import annotations
 
@@ -113,7 +116,7 @@ def density(nbf, seed):
     return C @ C.T
 
 
-def fock_leg(la, molecule, ctx, natoms, reps, workload=2, same_density=False):
+def fock_leg(la, molecule, flop_model, ctx, natoms, reps, workload=2, same_density=False):
     """J/K build (compute_2body_fock on the GPU) for a molecule of a workload
     recipe, FP64 and mixed (FP32 where K12 K34 < 1e-6 and S12 S34 < 16e-6):
     median wall time per build, kernel time, and the largest difference of
@@ -138,10 +141,22 @@ def fock_leg(la, molecule, ctx, natoms, reps, workload=2, same_density=False):
             wall.append((time.perf_counter() - t0) * 1e3)
             kern.append(fb.stats()["kernel_ms"])
         st = fb.stats()
-        cls = sorted(fb.class_stats(), key=lambda c: -c["ms"])
-        out[name] = {"build_ms": float(np.median(wall)), "kernel_ms": float(np.median(kern)),
+        cs = fb.class_stats()
+        cls = sorted(cs, key=lambda c: -c["ms"])
+        # model FLOPs of the build: ERI work per class (flop_model) plus the
+        # J/K digest, over the kernels' event time
+        flops = sum(flop_model.class_flops(*(int(x) for x in c["cls"]), c["nquartets"], c["nprim"]) +
+                    c["nquartets"] * flop_model.fock_digest_flops(*(int(x) for x in c["cls"])) for c in cs)
+        kms = float(np.median(kern))
+        out[name] = {"build_ms": float(np.median(wall)), "kernel_ms": kms,
                      "quartets": st["nquartets"], "quartets_fp32": st["nquartets_fp32"],
                      "primitive_quartets": st["nprim"],
+                     "roofline": {"bound": "fp64_valu", "achieved": flops / (kms * 1e-3) / 1e12,
+                                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": flops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                                  "model_flops": flops,
+                                  "note": "HGP model FLOPs per class + 12 FLOP of J/K digest per integral, "
+                                          "over the sum of the Fock kernels' event times"},
                      "top_classes": [f"{c['cls']}{'/fp32' if c['fp32'] else ''}: {c['ms']:.1f} ms, "
                                      f"{c['nquartets']} quartets, {c['nprim']} primitive quartets"
                                      for c in cls[:6]]}
@@ -158,9 +173,9 @@ def eri_f_leg(la, molecule, flop_model, ctx, nquartets, reps=3, seed=21):
     cube, s4p3d2f1, K~U{1..8}): a seeded sample of quartets with
     K_ab K_cd >= 1e-12 over every class up to (ff|ff), FP64 and mixed
     (FP32 where K_ab K_cd < 1e-6 and S_ab S_cd < 16e-6, l <= the FP32 kernels'
-    l), and FP64 with the opt-in wave split (outside the per-integral parity
-    bar); quartets/s and model TFLOP/s of the whole batch, and the classes
-    that take the most time."""
+    l), and FP64 with the wave split off (a lane per quartet, or the generic
+    kernel for few deep high-l classes); quartets/s and model TFLOP/s of the
+    whole batch, and the classes that take the most time."""
     mol = molecule.make_molecule(3)
     B = mol.basis
     b = la.Basis(ctx, B)
@@ -182,10 +197,10 @@ def eri_f_leg(la, molecule, flop_model, ctx, nquartets, reps=3, seed=21):
     q = np.stack([i[p1], j[p1], i[p2], j[p2]], 1).astype(np.int32)
     out = {"workload": f"120 atoms, 20 A cube, s4p3d2f1/atom, K~U{{1..8}}, exponents [0.03,2e4]; "
                        f"{nquartets} quartets drawn (seed {seed}) with K_ab K_cd >= 1e-12, all classes to (ff|ff)"}
-    for name, thr, wave in (("fp64", 0.0, False), ("fp64_wave_split", 0.0, True), ("mixed", 1e-6, False)):
+    for name, thr, wave in (("fp64", 0.0, True), ("fp64_no_wave_split", 0.0, False), ("mixed", 1e-6, True)):
         plan = la.EriPlan(b, q, fp32_threshold=thr)
-        if wave:  # opt-in: reordered contraction sums for classes of few deep quartets
-            plan.set_wave_split(True)
+        if not wave:
+            plan.set_wave_split(False)
         d = plan.device_output()
         plan.execute(d)
         ctx.synchronize()
@@ -210,6 +225,97 @@ def eri_f_leg(la, molecule, flop_model, ctx, nquartets, reps=3, seed=21):
         del plan
     out["mixed_speedup"] = out["fp64"]["ms"] / out["mixed"]["ms"]
     del b
+    return out
+
+
+def config1_leg(la, molecule, ctx, reps=3, cpu_target_s=3.0, ncall=3000, seed=11):
+    """The plain-method workload (10 atoms in a 6 A cube; per atom s x3 with
+    K 6, 1, 1, p x2 with K 3, 1, d x1; exponents [0.1, 5e3]; spherical d):
+    every unique quartet with K_ab K_cd >= 1e-12, all classes to (dd|dd).
+    GPU: one batched plan over the whole list (median of `reps` executions,
+    HIP events), and the drop-in per-call path (the resident single-quartet
+    evaluator, one call per quartet as libint's Engine::compute loop makes
+    them) on the first `ncall` quartets.  CPU: the restatement of
+    Engine::compute2 on every host thread over the whole list, or a seeded
+    sample sized for about cpu_target_s per pass when the whole list would
+    take longer (warm-up, then `reps` passes, median), with the GPU values of
+    the same quartets checked against it at the per-integral bar."""
+    import oracle
+    mol = molecule.make_molecule(1)
+    B = mol.basis
+    b = la.Basis(ctx, B)
+    i, j = np.tril_indices(B.nshell)
+    pairs = np.stack([i, j], 1).astype(np.int32)
+    K = la.schwarz(b, pairs, precision=0.0)
+    p1, p2 = np.triu_indices(len(pairs))
+    keep = K[p1] * K[p2] >= 1e-12
+    q = np.stack([i[p1[keep]], j[p1[keep]], i[p2[keep]], j[p2[keep]]], 1).astype(np.int32)
+    plan = la.EriPlan(b, q)
+    offs = plan.offsets()
+    d = plan.device_output()
+    plan.execute(d)
+    ctx.synchronize()
+    ms = []
+    for _ in range(reps):
+        plan.execute(d)
+        ctx.synchronize()
+        ms.append(plan.elapsed_ms())
+    t_gpu = float(np.median(ms)) * 1e-3
+    g, _ = plan.download()
+    # the drop-in per-call path
+    qe = la.QuartetEngine(ctx, int(B.nprim.max()), int(B.l.max()))
+    off = np.concatenate([[0], np.cumsum(B.nprim)])
+    shells = [la.Shell(int(B.l[s]), list(zip(B.alpha[off[s]:off[s + 1]], B.coeff[off[s]:off[s + 1]])),
+                       B.origin[3 * s:3 * s + 3], pure=bool(B.pure[s]), renorm=False) for s in range(B.nshell)]
+    calls = q[:ncall]
+    qe.compute([shells[x] for x in calls[0]])
+    t0 = time.perf_counter()
+    for quad in calls:
+        qe.compute([shells[x] for x in quad])
+    t_call = (time.perf_counter() - t0) / len(calls)
+    qe._release()
+    # CPU restatement
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(seed)
+    cal = q[rng.choice(len(q), min(4096, len(q)), replace=False)]
+    oracle.eri_quartets(B, cal[:64], nthreads=nthreads)
+    t0 = time.perf_counter()
+    oracle.eri_quartets(B, cal, nthreads=nthreads)
+    rate = len(cal) / max(time.perf_counter() - t0, 1e-6)
+    if rate * cpu_target_s >= len(q):
+        idx = np.arange(len(q))
+    else:
+        idx = np.sort(rng.choice(len(q), int(rate * cpu_target_s), replace=False))
+    sample = q[idx]
+    o, ooff, _ = oracle.eri_quartets(B, sample, nthreads=nthreads)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.eri_quartets(B, sample, nthreads=nthreads)
+        ts.append(time.perf_counter() - t0)
+    t_cpu = float(np.median(ts))
+    ends = np.append(offs[1:], g.size)
+    gs = np.concatenate([g[offs[k]:ends[k]] for k in idx])
+    dd = np.abs(gs - o)
+    nfail = int(((dd > 1e-12) & (dd > 1e-10 * np.abs(o))).sum())
+    out = {"workload": f"10 atoms, 6 A cube, s3p2d1/atom (K 6,1,1 / 3,1 / 1), exponents [0.1,5e3], spherical d; "
+                       f"{len(q)} of {len(p1)} unique quartets with K_ab K_cd >= 1e-12, all classes",
+           "quartets": int(len(q)),
+           "gpu_batched": {"ms": t_gpu * 1e3, "quartets_per_s": len(q) / t_gpu},
+           "gpu_per_call": {"calls": int(len(calls)), "us_per_call": t_call * 1e6,
+                            "quartets_per_s": 1.0 / t_call,
+                            "note": "Python ctypes call per quartet into the resident single-quartet evaluator "
+                                    "(one copy in, one launch, one copy out, a wait)"},
+           "cpu": {"quartets": int(len(sample)), "full_list": bool(len(sample) == len(q)), "s": t_cpu,
+                   "quartets_per_s": len(sample) / t_cpu, "cores": nthreads,
+                   "timing": f"warm-up, then {reps} passes of {', '.join(f'{x:.2f}' for x in ts)} s, median"},
+           "gpu_batched_over_cpu": (len(q) / t_gpu) / (len(sample) / t_cpu),
+           "checked_integrals": int(gs.size), "max_abs_vs_cpu": float(dd.max(initial=0.0)),
+           "values_beyond_1e-12_abs_and_1e-10_rel": nfail}
+    log(f"config 1: {len(q)} quartets, GPU batched {t_gpu * 1e3:.1f} ms ({len(q) / t_gpu:.3e} q/s), "
+        f"per call {t_call * 1e6:.0f} us, CPU {len(sample) / t_cpu:.3e} q/s on {nthreads} threads, "
+        f"max |gpu - cpu| {dd.max(initial=0.0):.1e}")
+    del plan, b
     return out
 
 
@@ -289,20 +395,25 @@ def next_rows_leg(la, mol, B, q, ctx, nq=400000, nderiv=50000, seed=7):
     return out
 
 
-def serve_leg(la, molecule, nreq, rate_hz, seed=5000):
-    """Fock-build requests for molecules of 2-12 atoms (log-uniform; the two
-    recipes alternate) arriving as a Poisson stream, served with continuous
-    batching; latency = submit to result."""
+def serve_leg(la, molecule, nreq, rate_hz, seed=5000, atoms=(2, 12), f_recipe=False, deterministic=False):
+    """Fock-build requests arriving as a Poisson stream, served with
+    continuous batching; latency = submit to result.  Molecules of atoms[0]
+    to atoms[1] atoms (log-uniform): the two s-d recipes alternating, or
+    (f_recipe) the s4p3d2f1 recipe at the 200-atom molecule's density."""
     rng = np.random.default_rng(seed)
     mols = []
     for k in range(nreq):
-        n = int(round(np.exp(rng.uniform(np.log(2), np.log(12)))))
-        m = molecule.make_molecule(1 + k % 2, natoms=n)
+        n = int(round(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1])))))
+        if f_recipe:
+            m = molecule.make_molecule(4, natoms=n, seed=seed + k, same_density=True)
+        else:
+            m = molecule.make_molecule(1 + k % 2, natoms=n)
         mols.append((m, density(m.nbf, seed + k)))
     gaps = rng.exponential(1.0 / rate_hz, nreq)
-    srv = la.Server(0)
+    srv = la.Server(0, deterministic=deterministic)
     try:
         srv.result(srv.submit(mols[0][0].basis, mols[0][1], 1e-12))  # warm-up
+        st0 = srv.stats()
         t0 = time.perf_counter()
         ids = []
         for (m, D), g in zip(mols, gaps):
@@ -313,8 +424,11 @@ def serve_leg(la, molecule, nreq, rate_hz, seed=5000):
         st = srv.stats()
     finally:
         srv.close()
-    out = {"workload": f"{nreq} requests, Poisson {rate_hz:g}/s, 2-12 atoms of the two recipes (nbf "
-                       f"{min(m.nbf for m, _ in mols)}-{max(m.nbf for m, _ in mols)}), precision 1e-12",
+    recipe = "the f recipe at the 200-atom density" if f_recipe else "the two s-d recipes"
+    out = {"workload": f"{nreq} requests, Poisson {rate_hz:g}/s, {atoms[0]}-{atoms[1]} atoms of {recipe} (nbf "
+                       f"{min(m.nbf for m, _ in mols)}-{max(m.nbf for m, _ in mols)}), precision 1e-12"
+                       f"{', deterministic digest' if deterministic else ''}",
+           "work_items_per_s": (st["work_items"] - st0["work_items"]) / wall,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "requests_per_s": nreq / wall, "rounds": st["rounds"],
            "max_requests_per_round": st["max_requests_per_round"]}
@@ -340,6 +454,10 @@ def main():
                     help="quartets of the high-angular-momentum ERI leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=150, help="requests of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--fock-f-only", action="store_true",
+                    help="only the f-recipe J/K build leg (for profiling its kernels) and exit")
+    ap.add_argument("--serve-f-requests", type=int, default=24,
+                    help="requests of the f-recipe serving leg (10-20 atoms at the 200-atom density; 0: skip)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -358,6 +476,12 @@ def main():
 
     import libint_amd as la
     from libint_amd import flop_model, molecule
+
+    if args.fock_f_only:
+        ctx = la.Context(0)
+        print(json.dumps(fock_leg(la, molecule, flop_model, ctx, args.fock_f_atoms or 10, reps=2, workload=4,
+                                  same_density=True)), flush=True)
+        return
 
     t_setup = time.perf_counter()
     mol = molecule.make_molecule(2)
@@ -437,14 +561,19 @@ def main():
     }
     del plan
     if not args.no_fock:
+        out["config1"] = config1_leg(la, molecule, ctx)
         out["next_rows"] = next_rows_leg(la, mol, B, q, ctx)
     if not args.no_fock and args.eri_f_quartets:
         out["eri_f"] = eri_f_leg(la, molecule, flop_model, ctx, args.eri_f_quartets)
     if not args.no_fock:
-        out["fock"] = fock_leg(la, molecule, ctx, args.fock_atoms, reps=2)
+        out["fock"] = fock_leg(la, molecule, flop_model, ctx, args.fock_atoms, reps=2)
         if args.fock_f_atoms:
-            out["fock_f"] = fock_leg(la, molecule, ctx, args.fock_f_atoms, reps=2, workload=4, same_density=True)
+            out["fock_f"] = fock_leg(la, molecule, flop_model, ctx, args.fock_f_atoms, reps=2, workload=4,
+                                     same_density=True)
         out["serve"] = serve_leg(la, molecule, args.serve_requests, rate_hz=args.serve_rate)
+        if args.serve_f_requests:
+            out["serve_f"] = serve_leg(la, molecule, args.serve_f_requests, rate_hz=args.serve_rate, seed=7000,
+                                       atoms=(10, 20), f_recipe=True, deterministic=True)
         out["p99_fock_build_ms"] = out["serve"]["p99_ms"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol.basis, q)

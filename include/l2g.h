@@ -66,6 +66,23 @@ int l2g_max_l(void);
  * counterpart in the reference (FP64 only). */
 int l2g_fp32_max_l(void);
 
+/* Which classes mixed batches route to their FP32 variants (plans and Fock
+ * builders created afterwards on this context).  L2G_FP32_MEASURED (the
+ * default): only classes whose FP32 kernel the autotuner measured faster
+ * than their FP64 kernel (libint_amd/tuned.cfg), so that mixed precision is
+ * never slower than FP64; on MI355X, where an FP32 and an FP64 FMA issue at
+ * the same rate without packed FP32 math, no class is.  L2G_FP32_ALWAYS:
+ * every class with an FP32 variant (the FP32 path's tests and error
+ * tables).  No counterpart in the reference. */
+#define L2G_FP32_MEASURED 0
+#define L2G_FP32_ALWAYS 1
+int l2g_set_fp32_policy(l2g_context ctx, int policy);
+
+/* The classes (la lb lc ld as the decimal code la*1000+lb*100+lc*10+ld)
+ * whose FP32 ERI variant was measured faster than FP64, up to cap of them;
+ * *n receives how many there are. */
+int l2g_fp32_paying_classes(int cap, int* n, int* codes);
+
 /* Opens a device context: selects the HIP device, creates a stream, builds
  * and uploads the Boys-function table.  Replaces libint2::initialize()
  * (include/libint2/initialize.h:76-90) for this path. */
@@ -195,13 +212,21 @@ int l2g_eri_plan_set_graph(l2g_eri_plan plan, int on);
 #define L2G_OPER_ERFC_COULOMB 2
 int l2g_eri_plan_set_operator(l2g_eri_plan plan, int oper, double omega);
 
-/* Wave split (on != 0; off by default): classes of few, deeply contracted
- * quartets run a wave per quartet whose 64 lanes split the primitive loop,
- * instead of one lane per quartet, so that a handful of long contractions no
- * longer serialise on a few lanes.  The contraction is then summed in another
- *  order than libint's serial primitive loop (engine.impl.h:1306-1766), which
- * changes ill-conditioned integrals (large cancelling primitive terms) by up
- * to ~1e-10 relative: outside the 1e-12 parity bar of the default mode.
+/* Wave split (on by default; on = 0 turns it off): classes of few, deeply
+ * contracted quartets run a wave per quartet whose 64 lanes split the
+ * primitive loop, instead of one lane per quartet, so that a handful of
+ * long contractions no longer serialise on a few lanes.  Each contraction
+ * is then summed as 64 serial partial sums joined by a fixed butterfly,
+ * another order than libint's serial primitive loop
+ * (engine.impl.h:1306-1766).  Well-conditioned integrals agree with the
+ * lane-per-quartet order to rounding; on ill-conditioned ones (large
+ * cancelling terms) both orders stay within the error the reference's own
+ * FP64 algorithm makes against the exact integral.  Off: such classes of
+ * total angular momentum 7 and more run on the generic kernel (a block per
+ * quartet, serial order), the rest a lane per quartet.  Whether a class
+ * runs a wave per quartet depends on how many quartets of it the plan holds,
+ * so the same quartet in plans of another composition can differ in its last
+ * bits; with the split off a quartet's values depend on the quartet only.
  * No counterpart in the reference. */
 int l2g_eri_plan_set_wave_split(l2g_eri_plan plan, int on);
 
@@ -305,6 +330,13 @@ int l2g_fock_set_deterministic(l2g_fock fock, int on);
 /* Screened unique quartets a build with this density and precision would
  * digest (the count pass of l2g_fock_build, without the build). */
 int l2g_fock_count(l2g_fock fock, const double* D, double precision, int64_t* nquartets);
+
+/* Progress of long builds: after each window, fn(done, total, user) with
+ * the quartets digested so far and the build's total (fn = NULL: none).
+ * Called on the calling thread, between windows.  No counterpart in the
+ * reference. */
+typedef void (*l2g_progress_fn)(int64_t done, int64_t total, void* user);
+int l2g_fock_set_progress(l2g_fock fock, l2g_progress_fn fn, void* user);
 
 /* A build produces and digests its screened quartet list in windows of at
  * most `items` quartets (0: the default, 2^26), so that the list of a large

@@ -105,6 +105,9 @@ ABI = {
     "l2g_server_destroy": (ctypes.c_int, [_vp]),
     "l2g_fock_set_window": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "l2g_fock_set_deterministic": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_set_fp32_policy": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "l2g_fp32_paying_classes": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp]),
+    "l2g_fock_set_progress": (ctypes.c_int, [_vp, _vp, _vp]),
     "l2g_fock_count": (ctypes.c_int, [_vp, _f64p, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_fock_stats_windows": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "l2g_quartet_engine_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -116,6 +119,10 @@ ABI = {
                                                    ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_double, _f64p, _vp]),
 }
+
+
+_PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p)
+FP32_MEASURED, FP32_ALWAYS = 0, 1  # L2G_FP32_* of include/l2g.h
 
 
 class ServerConfig(ctypes.Structure):
@@ -204,6 +211,15 @@ def fp32_max_l() -> int:
     """Largest l with FP32 kernel variants (mixed precision runs quartets
     with a higher l in FP64)."""
     return lib().l2g_fp32_max_l()
+
+
+def fp32_paying_classes() -> list:
+    """Classes ('2021', ...) whose FP32 variant was measured faster than
+    their FP64 kernel: the ones mixed batches route to FP32 by default."""
+    n = ctypes.c_int()
+    codes = np.zeros(256, np.int32)
+    _check(lib().l2g_fp32_paying_classes(256, ctypes.byref(n), codes.ctypes.data))
+    return [f"{c:04d}" for c in codes[:min(n.value, 256)]]
 
 
 def shell_renorm(l: int, alpha, coeff, enforce_unit: bool = True):
@@ -314,6 +330,12 @@ class Context:
 
     def set_solid_harmonics_ordering(self, ordering: int) -> None:
         _check(lib().l2g_set_solid_harmonics_ordering(self._h, ordering))
+
+    def set_fp32_policy(self, policy: int) -> None:
+        """FP32_MEASURED (default): mixed batches use the FP32 variants of
+        the classes where they were measured faster; FP32_ALWAYS: of every
+        class that has one (l2g_set_fp32_policy)."""
+        _check(lib().l2g_set_fp32_policy(self._h, int(policy)))
 
     def set_generic_kernels(self, on: bool) -> None:
         """Route every class through the generic kernel (a testing aid)."""
@@ -443,8 +465,9 @@ class EriPlan:
 
     def set_wave_split(self, on=True):
         """Classes of few, deeply contracted quartets run a wave per quartet
-        (lanes split the primitive loop); sums in another order than libint,
-        so ill-conditioned integrals move by up to ~1e-10 relative."""
+        (lanes split the primitive loop; on by default): each contraction is
+        summed as 64 partial sums and a fixed butterfly instead of libint's
+        serial order."""
         _check(lib().l2g_eri_plan_set_wave_split(self._h, int(bool(on))))
 
     def elapsed_ms(self):
@@ -594,6 +617,12 @@ class FockBuilder:
     def set_fp32_threshold(self, threshold: float) -> None:
         """Quartets with Schwarz bound below `threshold` run in FP32."""
         _check(lib().l2g_fock_set_fp32_threshold(self._h, threshold))
+
+    def set_progress(self, fn) -> None:
+        """fn(done, total) after each window of a build (None: off)."""
+        self._progress = None if fn is None else _PROGRESS_FN(lambda d, tot, u: fn(int(d), int(tot)))
+        ptr = None if self._progress is None else ctypes.cast(self._progress, ctypes.c_void_p)
+        _check(lib().l2g_fock_set_progress(self._h, ptr, None))
 
     def set_deterministic(self, on: bool = True) -> None:
         """Fixed-point J/K accumulation: the same bits on every run

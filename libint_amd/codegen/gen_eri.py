@@ -1046,7 +1046,8 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
     budgets = budgets or {}
 
     def cfg(c):
-        return budgets.get("".join(map(str, c)), (budget, waves))
+        v = budgets.get("".join(map(str, c)), (budget, waves))
+        return tuple(v) + (0,) * (3 - len(v))  # (budget, waves, FP32 pays)
 
     plans = [plan_class(*c, budget=cfg(c)[0]) for c in classes]
     if max_nodes:
@@ -1119,10 +1120,11 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
                 (len(p.ecomps) * len(p.fcomps) if p.split_rows else 0)
             if not in_regs and name in ("l2g_gen_classes", "l2g_gen_classes32", "l2g_gen_classes_erf"):
                 ybuf += ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld)  # staged output
+            pays = cfg((p.la, p.lb, p.lc, p.ld))[2] if name in ("l2g_gen_classes32", "l2g_gen_fock32") else 0
             lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
-                         f"{sum(p.group_nodes)}, &{k}_launch}},")
+                         f"{sum(p.group_nodes)}, {pays}, &{k}_launch}},")
         if not tab:
-            lines.append("  {0, 0, 0, 0, 0, 0, 0, nullptr},")
+            lines.append("  {0, 0, 0, 0, 0, 0, 0, 0, nullptr},")
         lines.append("};")
     lines.append(f"extern const int l2g_gen_nclasses = {len(table)};")
     lines.append(f"extern const int l2g_gen_nschwarz = {len(stable)};")
@@ -1156,7 +1158,9 @@ if __name__ == '__main__':
     ap.add_argument("--budget", type=int, default=160)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--only", default="", help="comma-separated class names, e.g. 2222")
-    ap.add_argument("--class-cfg", default="", help="per class budget:waves, e.g. 2222=64:2,2122=80:2")
+    ap.add_argument("--class-cfg", default="",
+                    help="per class budget:waves[:fp32], e.g. 2222=64:2,2122=80:2:1 (fp32 1: the class's FP32 "
+                         "variant measured faster than its FP64 kernel, so mixed batches route quartets to it)")
     ap.add_argument("--nfiles", type=int, default=8)
     ap.add_argument("--modes", default="eri,fock,schwarz,fp32,erf")
     ap.add_argument("--generic-lmax", type=int, default=3, help="highest l of the generic kernel's classes")
@@ -1169,8 +1173,7 @@ if __name__ == '__main__':
     per = {}
     for item in filter(None, a.class_cfg.split(",")):
         k, v = item.split("=")
-        b, w = v.split(":")
-        per[k] = (int(b), int(w))
+        per[k] = tuple(int(x) for x in v.split(":"))
     files, plans = generate(a.outdir, a.lmax, a.budget, a.nfiles, a.waves,
                             set(filter(None, a.only.split(","))) or None, per, tuple(a.modes.split(",")),
                             a.generic_lmax, a.max_nodes, a.fp32_lmax, mfma_acc=bool(a.mfma_acc))

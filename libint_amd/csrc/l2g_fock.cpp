@@ -406,7 +406,7 @@ void fock_enumerate(const FockSetup& f, const double* D, double precision, doubl
             } else if (fp32_threshold > 0 && K12 * K34 < fp32_threshold &&
                 S12 * f.S[(size_t)s3 * n + s4] < L2G_FP32_HRR_FACTOR * fp32_threshold) {
               const int c32 = ctx->fock32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
-              if (c32 >= 0) {
+              if (c32 >= 0 && (ctx->fp32_policy || l2g_gen_fock32[c32].fp32_pays)) {
                 cls = l2g_gen_nfock + c32;
                 ++w.nq32;
               }
@@ -606,6 +606,8 @@ struct l2g_fock_s {
   long long window_items = L2G_FOCK_WINDOW_ITEMS;
   double fp32_threshold = 0;  // FP32 selection threshold (fock_enumerate)
   bool deterministic = false;  // fixed-point J/K accumulation (l2g_fx_add)
+  l2g_progress_fn progress = nullptr;  // called after each window
+  void* progress_user = nullptr;
   int64_t last_nq = 0, last_nprim = 0, last_nq32 = 0;
   int64_t last_windows = 0;  // windows the last build's quartet list came in
   float last_ms = 0;
@@ -660,7 +662,7 @@ void upload_enum_tables(l2g_fock_s& f, hipStream_t st) {
           const int g = ctx->generic_of[a][b][c][d];
           cls[k] = spec >= 0 ? spec : (g >= 0 ? nf64 + nf32 + g : -1);
           const int c32 = ctx->force_generic ? -1 : ctx->fock32_of[a][b][c][d];
-          cls32[k] = c32 >= 0 ? nf64 + c32 : -1;
+          cls32[k] = c32 >= 0 && (ctx->fp32_policy || l2g_gen_fock32[c32].fp32_pays) ? nf64 + c32 : -1;
         }
   if (nfock_classes() > kEnumClasses - 1) throw L2gError(L2G_E_STATE, "too many Fock kernel classes for the enumeration sort key");
   upload(f.e_ps1, ps1, st);
@@ -791,6 +793,8 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     hipStream_t st = ctx->stream;
     FockEnumArgs E = fock_count_pass(f, D, precision);
     FockEnumState& es = f->enumst;
+    long long total_nq = 0;
+    for (long long c : es.block_count) total_nq += c;
     const int ncls = nfock_classes();
     f->last_nq = 0;
     f->last_nq32 = 0;
@@ -887,6 +891,7 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
       f->last_ms += event_ms_or(e0, e1, 0.0f);
       for (int c = 0; c < ncls; ++c)
         if (big.count[c]) f->cls_ms[c] += event_ms_or(f->ev[2 * c], f->ev[2 * c + 1], 0.0f);
+      if (f->progress) f->progress(f->last_nq, total_nq, f->progress_user);
     }
     launch_fock_finish(f->jg.as<double>(), f->kg.as<double>(), f->dens.as<double>(), N, hoff, st);
     hipError_t e = hipStreamSynchronize(st);
@@ -894,6 +899,14 @@ int l2g_fock_build(l2g_fock f, const double* D, double precision, double* J, dou
     if (J && NN) HIPCHK(hipMemcpy(J, f->jg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (K && NN) HIPCHK(hipMemcpy(K, f->kg.p, NN * sizeof(double), hipMemcpyDeviceToHost));
     if (G && NN) HIPCHK(hipMemcpy(G, f->dens.p, NN * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int l2g_fock_set_progress(l2g_fock f, l2g_progress_fn fn, void* user) {
+  return guarded([&] {
+    if (!f) throw std::invalid_argument("l2g_fock_set_progress: null Fock builder");
+    f->progress = fn;
+    f->progress_user = user;
   });
 }
 

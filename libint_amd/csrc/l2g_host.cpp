@@ -110,7 +110,7 @@ struct l2g_eri_plan_s {
   DevBuf out, ybuf, nz;
   int64_t ymax = 0;  // largest scratch doubles per lane over the classes
   int block = 128;
-  bool wave_split = false;  // classes of few deep quartets run a wave per quartet
+  bool wave_split = true;  // classes of few deep quartets run a wave per quartet
   int oper = 0;             // 0 Coulomb, 1 erf-attenuated, 2 erfc-attenuated
   double omega = 0;
   // hipGraph mode: the class launches are captured once per output buffer
@@ -315,6 +315,29 @@ int l2g_init(int device, l2g_context* out) {
   });
 }
 
+int l2g_fp32_paying_classes(int cap, int* n, int* codes) {
+  return guarded([&] {
+    if (!n || (cap > 0 && !codes)) throw std::invalid_argument("l2g_fp32_paying_classes: null argument");
+    int k = 0;
+    for (int i = 0; i < l2g_gen_nclasses32; ++i) {
+      const L2gClassInfo& c = l2g_gen_classes32[i];
+      if (!c.launch || !c.fp32_pays) continue;
+      if (k < cap) codes[k] = c.la * 1000 + c.lb * 100 + c.lc * 10 + c.ld;
+      ++k;
+    }
+    *n = k;
+  });
+}
+
+int l2g_set_fp32_policy(l2g_context ctx, int policy) {
+  return guarded([&] {
+    if (!ctx) throw L2gError(L2G_E_STATE, "null context");
+    if (policy != L2G_FP32_MEASURED && policy != L2G_FP32_ALWAYS)
+      throw std::invalid_argument("l2g_set_fp32_policy: unknown policy");
+    ctx->fp32_policy = policy;
+  });
+}
+
 int l2g_finalize(l2g_context ctx) {
   return guarded([&] {
     if (!ctx || g_exiting) return;
@@ -469,7 +492,7 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
         cls = nc64 + nc32 + g;  // FP64 only
       } else if (fp32_threshold > 0 && use_fp32(s)) {
         const int c32 = ctx->class32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
-        if (c32 >= 0) cls = nc64 + c32;
+        if (c32 >= 0 && (ctx->fp32_policy || l2g_gen_classes32[c32].fp32_pays)) cls = nc64 + c32;
       }
       auto purebit = [&](int sh) { return (B->pure[sh] && B->l[sh] > 0) ? 1 : 0; };
       const int meta = (stb ? 1 : 0) | (stk ? 2 : 0) | (sbk ? 4 : 0) | (purebit(b1) << 3) | (purebit(b2) << 4) |
@@ -564,9 +587,10 @@ namespace {
 // of few quartets (fewer than a quarter of the chip's lanes at one wave per
 // SIMD) with deep primitive loops would, at one lane per quartet, leave most
 // of the chip idle while a few lanes run their loops serially.  With
-// wave_split such a class runs a wave per quartet whose lanes split the loop
-// (summing each contraction in another order than libint's serial loop);
-// without it, a class of total angular momentum 7 or more averaging at least
+// wave_split (the default) such a class runs a wave per quartet whose lanes
+// split the loop (summing each contraction as 64 serial partial sums and a
+// fixed butterfly, another order than libint's serial loop, no less
+// accurate); without it, a class of total angular momentum 7 or more averaging at least
 // 64 primitive quartets per quartet runs on the generic kernel, a block per
 // quartet whose threads split the targets and keep each contraction's
 // serial order.  Lower classes have too few targets per quartet to occupy

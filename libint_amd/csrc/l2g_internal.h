@@ -267,6 +267,10 @@ struct l2g_context_s {
   int schwarz_of[8][8];
   int generic_of[8][8][8][8];  // index into l2g_gen_generic
   bool force_generic = false;  // every class through the generic kernel (testing)
+  // FP32 routing of mixed batches: 0, only classes whose FP32 variant was
+  // measured faster than their FP64 kernel (L2gClassInfo::fp32_pays); 1,
+  // every class with an FP32 variant
+  int fp32_policy = 0;
   l2g_impl::DevBuf gprog, gidx;  // device copies of the generic programs
   // bases, plans and Fock builders created on this context and not yet
   // destroyed; l2g_finalize refuses while any is alive
@@ -310,8 +314,13 @@ struct l2g_basis_s {
   // the tight shell.  True when shell j is tighter, i.e. when the
   // K-weighted mean of |P - B_j| = alpha_i/gamma |AB| is below that of
   // |P - A_i| = alpha_j/gamma |AB|.
+  // Shells below f keep the caller's order, as the reference: when both
+  // shells of a pair mix tight and diffuse primitives neither centre is
+  // reliably better, and on d pairs the caller's order measured closer to
+  // the exact integrals; on f pairs the tighter shell cut the Fock error of
+  // an f-recipe build against the exact one from 2e-6 to 3e-11.
   bool second_is_centre(int i, int j) {
-    if (i == j || l[i] != l[j]) return false;
+    if (i == j || l[i] != l[j] || l[i] < 3) return false;
     const uint64_t key = (uint64_t)(uint32_t)i << 32 | (uint32_t)j;
     auto it = orient.find(key);
     if (it != orient.end()) return it->second;

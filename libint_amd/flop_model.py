@@ -115,6 +115,17 @@ def class_flops(la, lb, lc, ld, nquartets: int, nprim_quartets: int, pure=(True,
             float(nquartets) * quartet_flops(la, lb, lc, ld, pure))
 
 
+def fock_digest_flops(la: int, lb: int, lc: int, ld: int, pure=(True, True, True, True)) -> int:
+    """J/K digest FLOPs per contracted quartet: for every integral (ab|cd),
+    J_ab += D_cd v and J_cd += D_ab v, K_ac += D_bd v, K_ad += D_bc v,
+    K_bd += D_ac v, K_bc += D_ad v, as compute_2body_fock adds them
+    (tests/hartree-fock/hartree-fock++.cc:1709-1737): 6 FMA = 12 FLOP."""
+    n = 1
+    for l, p in zip((la, lb, lc, ld), pure):
+        n *= (2 * l + 1) if (p and l >= 2) else ncart(l)
+    return 12 * n
+
+
 def class_vrr_flops(la: int, lb: int, lc: int, ld: int) -> int:
     """VRR + contraction FLOPs per primitive quartet counted class by class:
     every (a0|c0)^(m) shell set the HGP targets reach, e in [la, la+lb] and

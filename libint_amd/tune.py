@@ -10,11 +10,18 @@ differs between classes and is measured, not modelled:
         kernels), under D/b<budget>/
   python -m libint_amd.tune measure --dir D
         (GPU) times every class's ERI kernel in every library on the same
-        seeded quartets of the workload-2 recipe (60 atoms), each library in
-        a child process; writes D/timings.json
+        seeded quartets (s-d classes: the 60-atom s4p3d2 recipe; f classes:
+        the 120-atom s4p3d2f1 recipe), each library in a child process;
+        writes D/timings.json
   python -m libint_amd.tune select  --dir D [--out libint_amd/tuned.cfg]
-        writes the fastest budget of each class as class=budget:0 entries,
-        which the Makefile passes to the generator (CLASSCFG)
+        writes the fastest budget of each measured class as a
+        class=budget:0:fp32 entry, merged into the file's other entries; the
+        Makefile passes them to the generator (CLASSCFG)
+  python -m libint_amd.tune fp32 [--classes ...] [--out libint_amd/tuned.cfg]
+        (GPU) times each class's FP64 kernel and its FP32 variant in the
+        built library on the same quartets and sets the fp32 field (1: the
+        FP32 variant is at least 5 % faster, so mixed batches route to it by
+        default); rebuild the library afterwards
 """
 from __future__ import annotations
 
@@ -26,6 +33,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_BUDGETS = (128, 144, 160)
+DEFAULT_BUDGET = 160  # the Makefile's BUDGET
 
 
 def heavy_classes():
@@ -42,7 +50,7 @@ def build(args):
         d = os.path.join(os.path.abspath(args.dir), f"b{b}")
         os.makedirs(d, exist_ok=True)
         cmd = ["make", "-s", "-C", HERE, f"GEN={d}/gen", f"OBJ={d}/obj", f"LIB={d}/libl2g.so",
-               f"ONLY={classes}", "MODES=eri", f"BUDGET={b}", "-j2"]
+               f"ONLY={classes}", "MODES=eri,fp32", f"BUDGET={b}", "CLASSCFG=", "-j2"]
         procs.append((b, subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)))
     for b, p in procs:
         _, err = p.communicate()
@@ -61,34 +69,55 @@ def time_library(args):
     import libint_amd as la
     from libint_amd import molecule
     classes = args.classes.split(",")
-    mol = molecule.make_molecule(2)
-    B = mol.basis
     ctx = la.Context(0)
-    b = la.Basis(ctx, B)
+    bases = {}
     out = {}
     rng = np.random.default_rng(11)
     for name in classes:
         la_, lb_, lc_, ld_ = (int(x) for x in name)
+        # s-d classes on the 60-atom recipe, f classes on the 120-atom one
+        w = 3 if max(la_, lb_, lc_, ld_) == 3 else 2
+        if w not in bases:
+            m = molecule.make_molecule(w)
+            bases[w] = (m.basis, la.Basis(ctx, m.basis))
+        B, b = bases[w]
         bra = [(i, j) for i in range(B.nshell) for j in range(B.nshell)
                if B.l[i] == la_ and B.l[j] == lb_ and (la_ != lb_ or i >= j)]
         ket = [(i, j) for i in range(B.nshell) for j in range(B.nshell)
                if B.l[i] == lc_ and B.l[j] == ld_ and (lc_ != ld_ or i >= j)]
-        bq = np.array(bra, np.int32)[rng.choice(len(bra), min(len(bra), 4000), replace=False)]
-        kq = np.array(ket, np.int32)[rng.choice(len(ket), min(len(ket), 4000), replace=False)]
-        q = np.concatenate([np.repeat(bq, len(kq), 0), np.tile(kq, (len(bq), 1))], 1)
-        q = q[rng.choice(len(q), min(len(q), 400000), replace=False)]
-        plan = la.EriPlan(b, q)
-        d = plan.device_output()
-        plan.execute(d)
-        ctx.synchronize()
-        ms = []
-        for _ in range(3):
+        # significant pairs (Schwarz factor >= 1e-9), quartets with
+        # K_ab K_cd >= 1e-12, as the workloads screen them
+        bra = np.array(bra, np.int32)
+        ket = np.array(ket, np.int32)
+        kb = la.schwarz(b, bra)
+        kk = kb if (la_, lb_) == (lc_, ld_) else la.schwarz(b, ket)
+        bra, kb = bra[kb >= 1e-9], kb[kb >= 1e-9]
+        ket, kk = ket[kk >= 1e-9], kk[kk >= 1e-9]
+        i1 = rng.integers(0, len(bra), 1600000)
+        i2 = rng.integers(0, len(ket), 1600000)
+        keep = kb[i1] * kk[i2] >= 1e-12
+        i1, i2 = i1[keep][:400000], i2[keep][:400000]
+        q = np.concatenate([bra[i1], ket[i2]], 1)
+        variants = [("", 0.0)]
+        if args.fp32 and max(la_, lb_, lc_, ld_) <= la.fp32_max_l():
+            variants.append(("/fp32", 1e300))  # every quartet below the bounds: all FP32
+        for suffix, thr in variants:
+            ctx.set_fp32_policy(la.FP32_ALWAYS)
+            plan = la.EriPlan(b, q, fp32_threshold=thr)
+            ctx.set_fp32_policy(la.FP32_MEASURED)
+            if thr and not all(s["fp32"] for s in plan.stats()):
+                raise SystemExit(f"{name}: the FP32 plan is not all FP32")
+            d = plan.device_output()
             plan.execute(d)
             ctx.synchronize()
-            ms.append(sum(s["ms"] for s in plan.stats()))
-        out[name] = float(np.median(ms))
-        del plan
-    del b
+            ms = []
+            for _ in range(3):
+                plan.execute(d)
+                ctx.synchronize()
+                ms.append(sum(s["ms"] for s in plan.stats()))
+            out[name + suffix] = float(np.median(ms))
+            del plan
+    bases.clear()
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -102,7 +131,7 @@ def measure(args):
             continue
         env = dict(os.environ, LIBINT_AMD_LIB=lib)
         print(f"budget {sub[1:]}: timing {classes}", flush=True)
-        r = subprocess.run([sys.executable, "-m", "libint_amd.tune", "_time", "--classes", classes],
+        r = subprocess.run([sys.executable, "-m", "libint_amd.tune", "_time", "--fp32", "--classes", classes],
                            env=env, cwd=os.path.dirname(HERE), capture_output=True, text=True, timeout=900)
         if r.returncode:
             raise SystemExit(f"budget {sub[1:]}: timing failed ({r.returncode})\n{r.stderr[-2000:]}")
@@ -112,29 +141,75 @@ def measure(args):
         json.dump(res, f, indent=1)
 
 
+def read_cfg(path):
+    """class -> [budget, waves, fp32] of a tuned.cfg file."""
+    out = {}
+    try:
+        text = open(path).read().strip()
+    except OSError:
+        return out
+    for item in filter(None, text.split(",")):
+        k, v = item.split("=")
+        f = [int(x) for x in v.split(":")]
+        out[k] = f + [0] * (3 - len(f))
+    return out
+
+
+def write_cfg(path, cfg):
+    with open(path, "w") as f:
+        f.write(",".join(f"{c}={v[0]}:{v[1]}:{v[2]}" for c, v in sorted(cfg.items())) + "\n")
+    print(f"wrote {path}")
+
+
+def fp32_pays(t64, t32):
+    return 1 if t32 is not None and t32 < 0.95 * t64 else 0
+
+
 def select(args):
     res = json.load(open(os.path.join(args.dir, "timings.json")))
-    classes = sorted({c for r in res.values() for c in r})
-    entries = []
+    classes = sorted({c for r in res.values() for c in r if "/" not in c})
+    cfg = read_cfg(args.out)
     for c in classes:
         best = min((r[c], int(b)) for b, r in res.items() if c in r)
-        entries.append(f"{c}={best[1]}:0")
-        print(f"{c}: budget {best[1]} ({best[0]:.2f} ms); " +
-              ", ".join(f"{b}: {r[c]:.2f}" for b, r in sorted(res.items(), key=lambda x: int(x[0])) if c in r))
-    with open(args.out, "w") as f:
-        f.write(",".join(entries) + "\n")
-    print(f"wrote {args.out}")
+        r = res[str(best[1])]
+        cfg[c] = [best[1], 0, fp32_pays(r[c], r.get(c + "/fp32"))]
+        print(f"{c}: budget {best[1]} ({best[0]:.2f} ms), fp32 {cfg[c][2]}; " +
+              ", ".join(f"{b}: {r_[c]:.2f}" for b, r_ in sorted(res.items(), key=lambda x: int(x[0])) if c in r_))
+    write_cfg(args.out, cfg)
+
+
+def fp32(args):
+    """FP64 kernel against FP32 variant of each class, with the built library."""
+    sys.path.insert(0, os.path.join(HERE, "codegen"))
+    import gen_eri
+    classes = args.classes or ",".join("".join(map(str, c)) for c in gen_eri.canonical_classes(2))
+    r = subprocess.run([sys.executable, "-m", "libint_amd.tune", "_time", "--fp32", "--classes", classes],
+                       cwd=os.path.dirname(HERE), capture_output=True, text=True, timeout=1200)
+    if r.returncode:
+        raise SystemExit(f"timing failed ({r.returncode})\n{r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    cfg = read_cfg(args.out)
+    for c in classes.split(","):
+        if c not in res:
+            continue
+        v = cfg.setdefault(c, [DEFAULT_BUDGET, 0, 0])
+        v[2] = fp32_pays(res[c], res.get(c + "/fp32"))
+        t32 = res.get(c + "/fp32")
+        print(f"{c}: fp64 {res[c]:.2f} ms, fp32 {t32 if t32 is None else round(t32, 2)} ms -> fp32 {v[2]}",
+              flush=True)
+    write_cfg(args.out, cfg)
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("step", choices=["build", "measure", "select", "_time"])
+    ap.add_argument("step", choices=["build", "measure", "select", "fp32", "_time"])
+    ap.add_argument("--fp32", action="store_true", help="_time: also the FP32 variants")
     ap.add_argument("--dir", default="tune_build")
     ap.add_argument("--budgets", default=",".join(map(str, DEFAULT_BUDGETS)))
     ap.add_argument("--classes", default="")
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.cfg"))
     args = ap.parse_args()
-    {"build": build, "measure": measure, "select": select, "_time": time_library}[args.step](args)
+    {"build": build, "measure": measure, "select": select, "fp32": fp32, "_time": time_library}[args.step](args)
 
 
 if __name__ == "__main__":

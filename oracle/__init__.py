@@ -296,6 +296,11 @@ def fock_2e(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0, tight_
     return G, JK[0], JK[1], nq.value
 
 
+def set_progress(on: bool) -> None:
+    """Long Fock builds print one line per shell row to stderr."""
+    lib().oracle_set_progress(int(bool(on)))
+
+
 def fock_2e_exact(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0, all_quad=False):
     """fock_2e(tight_centre=True) with the ill-conditioned quartets settled
     in quad precision: every quartet is evaluated in FP64 and in long

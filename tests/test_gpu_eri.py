@@ -14,7 +14,7 @@ import libint_amd as la
 import oracle
 import oracle.deriv1
 from libint_amd import molecule
-from tests.helpers import compare, random_shells
+from tests.helpers import compare_arbitrated, compare, random_shells, within_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -266,11 +266,11 @@ def test_graph_replay_matches_eager(ctx):
 
 
 def test_wave_split(ctx):
-    """Opt-in wave per quartet for classes of few deep quartets: classes left
-    in lane mode are bit-identical to the default plan; wave classes sum the
-    contraction in another order, so they are held to 1e-10 absolute or
-    1e-8 relative against the oracle (ill-conditioned f-shell quartets move
-    by up to ~1e-11 absolute), not to the per-integral parity bar."""
+    """Wave per quartet for classes of few deep quartets (the default):
+    classes left in lane mode are bit-identical to a plan with the wave
+    split off; wave classes sum the contraction in another order and meet
+    the per-integral bar with quad-precision arbitration, as the
+    lane-per-quartet order does."""
     mol = molecule.make_molecule(3, natoms=3, same_density=True)
     B = mol.basis
     b = la.Basis(ctx, B)
@@ -279,10 +279,10 @@ def test_wave_split(ctx):
     q = allq[rng.choice(len(allq), 12000, replace=False)]
     say(f"case: wave split, {len(q)} quartets of the workload-3 recipe (3 atoms)")
     ref = la.EriPlan(b, q)
+    ref.set_wave_split(False)
     ref.execute()
     r, rnz = ref.download()
     plan = la.EriPlan(b, q)
-    plan.set_wave_split(True)
     plan.execute()
     g, gnz = plan.download()
     st = plan.stats()
@@ -305,10 +305,14 @@ def test_wave_split(ctx):
     inw = np.isin(cls, list(wave))
     iw = np.repeat(inw, n)
     assert np.array_equal(g[~iw], r[~iw])
-    o, _, _ = oracle.eri_quartets(B, q)
-    nfail, mabs, mrel = compare(g[iw], o[iw], 1e-10, 1e-8)
-    say(f"  wave part ({int(inw.sum())} quartets): max abs {mabs:.2e}, fails {nfail}")
+    o, ooff, _ = oracle.eri_quartets(B, q)
+    det = {}
+    nfail, narb, _ = compare_arbitrated(B, q[inw], np.concatenate([[0], np.cumsum(n[inw])[:-1]]), g[iw], o[iw],
+                                        1e-12, 1e-10, detail=det)
+    say(f"  wave part ({int(inw.sum())} quartets): max abs {np.abs(g[iw] - o[iw]).max():.2e}, fails {nfail}, "
+        f"{narb} values arbitrated")
     assert nfail == 0
+    assert within_reference(det)
 
 
 def test_three_and_two_centre(ctx):
@@ -427,7 +431,10 @@ def test_first_derivatives(ctx):
                 l_ = int(arrays.l[sh])
                 std = [l_ + ((p_ + 1) // 2 if p_ % 2 else -(p_ // 2)) for p_ in range(2 * l_ + 1)]
                 blk = np.take(blk, std, axis=k + 1)
-        assert np.array_equal(gg[:, offs[i]:offs[i] + int(np.prod(dims))], blk.reshape(12, -1)), i
+        # another batch (40 quartets, not 300): classes may switch between a
+        # lane and a wave per quartet, which changes the rounding only
+        x = gg[:, offs[i]:offs[i] + int(np.prod(dims))]
+        assert np.abs(x - blk.reshape(12, -1)).max() <= 1e-14 * max(1.0, np.abs(blk).max()), i
     for d in range(3):
         s_ = g[d] + g[3 + d] + g[6 + d] + g[9 + d]
         assert np.abs(s_).max() < 1e-11

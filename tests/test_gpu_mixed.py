@@ -25,6 +25,37 @@ def say(*a):
     sys.stdout.flush()
 
 
+@pytest.fixture(autouse=True)
+def fp32_everywhere(ctx):
+    """The FP32 variants of every class (by default mixed batches use only
+    those measured faster than FP64)."""
+    ctx.set_fp32_policy(la.FP32_ALWAYS)
+    yield
+    ctx.set_fp32_policy(la.FP32_MEASURED)
+
+
+def test_mixed_default_routes_only_measured_classes(ctx):
+    """With the default routing no class of this build's FP32 variants was
+    measured faster than its FP64 kernel, so a mixed plan runs every quartet
+    in FP64 and gives the FP64 plan's values bit for bit."""
+    ctx.set_fp32_policy(la.FP32_MEASURED)
+    mol = molecule.make_molecule(2, natoms=4)
+    b = la.Basis(ctx, mol.basis)
+    q = molecule.unique_quartets(mol.basis.nshell)[:20000]
+    say(f"case: mixed plan, default FP32 routing, {len(q)} quartets")
+    m = la.EriPlan(b, q, fp32_threshold=THR)
+    m.execute()
+    gm, _ = m.download()
+    p = la.EriPlan(b, q)
+    p.execute()
+    g, _ = p.download()
+    paying = la.fp32_paying_classes()
+    say(f"  classes whose FP32 variant pays: {paying}")
+    if not paying:
+        assert sum(s["nquartets"] for s in m.stats() if s["fp32"]) == 0
+        assert np.array_equal(gm, g)
+
+
 @pytest.mark.parametrize("workload,natoms,nq", [(2, 6, 60000), (3, 3, 12000)])
 def test_mixed_eri_per_class_errors(ctx, workload, natoms, nq):
     """Workload 2 (s-d) and the high-angular-momentum recipe (s-f, K <= 8,

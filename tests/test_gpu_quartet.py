@@ -40,6 +40,7 @@ def test_quartet_engine_equals_plan(ctx):
             ctx.set_solid_harmonics_ordering(order)
             try:
                 P = la.EriPlan(b, q)
+                P.set_wave_split(False)  # a lane per quartet, as the quartet engine runs
                 if oper != la.COULOMB:
                     P.set_operator(oper, omega)
                 offs = P.offsets()
@@ -67,6 +68,7 @@ def test_quartet_engine_given_pairs(ctx):
     b = la.Basis(ctx, la.BasisArrays.from_shells(shells))
     q = np.array([[0, 1, 2, 3], [4, 2, 1, 0], [2, 5, 4, 4], [3, 3, 0, 5]], np.int32)
     P = la.EriPlan(b, q, precision=0.0)
+    P.set_wave_split(False)
     offs = P.offsets()
     P.execute()
     ref, _ = P.download()
