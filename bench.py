@@ -559,6 +559,30 @@ def main():
         },
         "setup_s": t_setup,
     }
+    # the same list in mixed precision (FP32 where K12 K34 < 1e-6 and
+    # S12 S34 < 16e-6, in the classes where FP32 was measured faster)
+    g64 = None
+    if not args.no_fock:
+        mplan = la.EriPlan(B, q, fp32_threshold=1e-6)
+        dm = mplan.device_output()
+        mplan.execute(dm)
+        ctx.synchronize()
+        mms = []
+        for _ in range(args.steps):
+            mplan.execute(dm)
+            ctx.synchronize()
+            mms.append(sum(s["ms"] for s in mplan.stats()))
+        n32 = sum(s["nquartets"] for s in mplan.stats() if s["fp32"])
+        gm, _ = mplan.download()
+        del mplan, dm
+        plan.execute(d_out)
+        g64, _ = plan.download()
+        t_m = float(np.mean(mms))
+        out["mixed"] = {"kernel_ms": t_m, "quartets_fp32": int(n32), "quartets_per_s": len(q) / (t_m * 1e-3),
+                        "speedup_vs_fp64": kms / t_m, "max_abs_vs_fp64": float(np.abs(gm - g64).max())}
+        log(f"mixed: {t_m:.1f} ms ({n32} of {len(q)} quartets in FP32), {kms / t_m:.2f}x the FP64 kernel, "
+            f"max |mixed - fp64| {out['mixed']['max_abs_vs_fp64']:.1e}")
+        del gm, g64
     del plan
     if not args.no_fock:
         out["config1"] = config1_leg(la, molecule, ctx)

@@ -66,14 +66,17 @@ int l2g_max_l(void);
  * counterpart in the reference (FP64 only). */
 int l2g_fp32_max_l(void);
 
-/* Which classes mixed batches route to their FP32 variants (plans and Fock
+/* Which quartets mixed batches route to FP32 variants (plans and Fock
  * builders created afterwards on this context).  L2G_FP32_MEASURED (the
- * default): only classes whose FP32 kernel the autotuner measured faster
- * than their FP64 kernel (libint_amd/tuned.cfg), so that mixed precision is
- * never slower than FP64; on MI355X, where an FP32 and an FP64 FMA issue at
- * the same rate without packed FP32 math, no class is.  L2G_FP32_ALWAYS:
- * every class with an FP32 variant (the FP32 path's tests and error
- * tables).  No counterpart in the reference. */
+ * default), so that mixed precision is not slower than FP64: in ERI plans,
+ * the classes whose FP32 kernel the autotuner measured faster than their
+ * FP64 kernel on FP32-eligible quartets (libint_amd/tuned.cfg), when the
+ * plan holds at least 4 x 256 lanes per CU of such quartets of the class
+ * (a smaller FP32 launch cannot fill the chip); in Fock builds, none (their
+ * FP32-eligible quartets are light, and mixed builds measured slower).
+ * L2G_FP32_ALWAYS: every eligible quartet of every class with an FP32
+ * variant (the FP32 path's tests and error tables).  No counterpart in the
+ * reference. */
 #define L2G_FP32_MEASURED 0
 #define L2G_FP32_ALWAYS 1
 int l2g_set_fp32_policy(l2g_context ctx, int policy);

@@ -406,7 +406,7 @@ void fock_enumerate(const FockSetup& f, const double* D, double precision, doubl
             } else if (fp32_threshold > 0 && K12 * K34 < fp32_threshold &&
                 S12 * f.S[(size_t)s3 * n + s4] < L2G_FP32_HRR_FACTOR * fp32_threshold) {
               const int c32 = ctx->fock32_of[B->l[b1]][B->l[b2]][B->l[k1]][B->l[k2]];
-              if (c32 >= 0 && (ctx->fp32_policy || l2g_gen_fock32[c32].fp32_pays)) {
+              if (c32 >= 0 && ctx->fp32_policy) {
                 cls = l2g_gen_nfock + c32;
                 ++w.nq32;
               }
@@ -662,7 +662,11 @@ void upload_enum_tables(l2g_fock_s& f, hipStream_t st) {
           const int g = ctx->generic_of[a][b][c][d];
           cls[k] = spec >= 0 ? spec : (g >= 0 ? nf64 + nf32 + g : -1);
           const int c32 = ctx->force_generic ? -1 : ctx->fock32_of[a][b][c][d];
-          cls32[k] = c32 >= 0 && (ctx->fp32_policy || l2g_gen_fock32[c32].fp32_pays) ? nf64 + c32 : -1;
+          // FP32 Fock kernels only when asked for every class: in mixed Fock
+          // builds the FP32-eligible quartets are the light ones (few
+          // primitive quartets survive), where FP32 saves nothing and the
+          // extra launches cost (measured 0.88-0.95x)
+          cls32[k] = c32 >= 0 && ctx->fp32_policy ? nf64 + c32 : -1;
         }
   if (nfock_classes() > kEnumClasses - 1) throw L2gError(L2G_E_STATE, "too many Fock kernel classes for the enumeration sort key");
   upload(f.e_ps1, ps1, st);

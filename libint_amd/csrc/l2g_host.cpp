@@ -510,6 +510,22 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
       buckets[cls].push_back(it);
     }
     plan->total = total;
+    // measured routing: an FP32 launch too small to fill the chip (its
+    // lanes waiting on a tail) costs more than FP32 saves, so below
+    // 4 x 256 lanes per CU a class's FP32 items run in its FP64 launch
+    if (!ctx->fp32_policy) {
+      const size_t min_items = (size_t)4 * ctx->ncu * 256;
+      for (int c32 = 0; c32 < nc32; ++c32) {
+        auto& v = buckets[nc64 + c32];
+        if (v.empty() || v.size() >= min_items) continue;
+        const L2gClassInfo& ci = l2g_gen_classes32[c32];
+        const int c64 = ctx->class_of[ci.la][ci.lb][ci.lc][ci.ld];
+        if (c64 < 0) continue;
+        auto& d = buckets[c64];
+        d.insert(d.end(), v.begin(), v.end());
+        v.clear();
+      }
+    }
     // flatten: classes in table order, each sorted by contraction depth
     std::vector<int> bra, ket, meta;
     std::vector<long long> off, qidx;

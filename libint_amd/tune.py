@@ -19,9 +19,10 @@ differs between classes and is measured, not modelled:
         Makefile passes them to the generator (CLASSCFG)
   python -m libint_amd.tune fp32 [--classes ...] [--out libint_amd/tuned.cfg]
         (GPU) times each class's FP64 kernel and its FP32 variant in the
-        built library on the same quartets and sets the fp32 field (1: the
-        FP32 variant is at least 5 % faster, so mixed batches route to it by
-        default); rebuild the library afterwards
+        built library on the quartets a mixed batch would run in FP32
+        (K K < 1e-6, S S < 16e-6) and sets the fp32 field (1: the FP32
+        variant is at least 5 % faster there, so mixed batches route to it
+        by default); rebuild the library afterwards
 """
 from __future__ import annotations
 
@@ -89,21 +90,26 @@ def time_library(args):
         # K_ab K_cd >= 1e-12, as the workloads screen them
         bra = np.array(bra, np.int32)
         ket = np.array(ket, np.int32)
-        kb = la.schwarz(b, bra)
-        kk = kb if (la_, lb_) == (lc_, ld_) else la.schwarz(b, ket)
-        bra, kb = bra[kb >= 1e-9], kb[kb >= 1e-9]
-        ket, kk = ket[kk >= 1e-9], kk[kk >= 1e-9]
+        kb, sb = la.precision_bounds(b, bra)
+        kk, sk = (kb, sb) if (la_, lb_) == (lc_, ld_) else la.precision_bounds(b, ket)
+        bra, kb, sb = bra[kb >= 1e-9], kb[kb >= 1e-9], sb[kb >= 1e-9]
+        ket, kk, sk = ket[kk >= 1e-9], kk[kk >= 1e-9], sk[kk >= 1e-9]
         i1 = rng.integers(0, len(bra), 1600000)
         i2 = rng.integers(0, len(ket), 1600000)
-        keep = kb[i1] * kk[i2] >= 1e-12
-        i1, i2 = i1[keep][:400000], i2[keep][:400000]
+        sig = kb[i1] * kk[i2] >= 1e-12
+        # the quartets a mixed batch at threshold 1e-6 runs in FP32
+        # (K K < 1e-6 and S S < 16e-6): where the FP32 variant is compared
+        elig = sig & (kb[i1] * kk[i2] < 1e-6) & (sb[i1] * sk[i2] < la.FP32_HRR_FACTOR * 1e-6)
+        qe = np.concatenate([bra[i1[elig][:400000]], ket[i2[elig][:400000]]], 1)
+        i1, i2 = i1[sig][:400000], i2[sig][:400000]
         q = np.concatenate([bra[i1], ket[i2]], 1)
-        variants = [("", 0.0)]
-        if args.fp32 and max(la_, lb_, lc_, ld_) <= la.fp32_max_l():
-            variants.append(("/fp32", 1e300))  # every quartet below the bounds: all FP32
-        for suffix, thr in variants:
+        variants = [("", 0.0, q)]
+        if args.fp32 and max(la_, lb_, lc_, ld_) <= la.fp32_max_l() and len(qe):
+            # the eligible quartets in FP64 and, every one below the bounds, in FP32
+            variants += [("/e64", 0.0, qe), ("/fp32", 1e300, qe)]
+        for suffix, thr, qq in variants:
             ctx.set_fp32_policy(la.FP32_ALWAYS)
-            plan = la.EriPlan(b, q, fp32_threshold=thr)
+            plan = la.EriPlan(b, qq, fp32_threshold=thr)
             ctx.set_fp32_policy(la.FP32_MEASURED)
             if thr and not all(s["fp32"] for s in plan.stats()):
                 raise SystemExit(f"{name}: the FP32 plan is not all FP32")
@@ -156,8 +162,10 @@ def read_cfg(path):
 
 
 def write_cfg(path, cfg):
+    """Entries that differ from the defaults (budget 160, waves 0, fp32 0)."""
+    keep = {c: v for c, v in cfg.items() if list(v) != [DEFAULT_BUDGET, 0, 0]}
     with open(path, "w") as f:
-        f.write(",".join(f"{c}={v[0]}:{v[1]}:{v[2]}" for c, v in sorted(cfg.items())) + "\n")
+        f.write(",".join(f"{c}={v[0]}:{v[1]}:{v[2]}" for c, v in sorted(keep.items())) + "\n")
     print(f"wrote {path}")
 
 
@@ -172,7 +180,7 @@ def select(args):
     for c in classes:
         best = min((r[c], int(b)) for b, r in res.items() if c in r)
         r = res[str(best[1])]
-        cfg[c] = [best[1], 0, fp32_pays(r[c], r.get(c + "/fp32"))]
+        cfg[c] = [best[1], 0, fp32_pays(r.get(c + "/e64", r[c]), r.get(c + "/fp32"))]
         print(f"{c}: budget {best[1]} ({best[0]:.2f} ms), fp32 {cfg[c][2]}; " +
               ", ".join(f"{b}: {r_[c]:.2f}" for b, r_ in sorted(res.items(), key=lambda x: int(x[0])) if c in r_))
     write_cfg(args.out, cfg)
@@ -193,10 +201,11 @@ def fp32(args):
         if c not in res:
             continue
         v = cfg.setdefault(c, [DEFAULT_BUDGET, 0, 0])
-        v[2] = fp32_pays(res[c], res.get(c + "/fp32"))
-        t32 = res.get(c + "/fp32")
-        print(f"{c}: fp64 {res[c]:.2f} ms, fp32 {t32 if t32 is None else round(t32, 2)} ms -> fp32 {v[2]}",
-              flush=True)
+        t64, t32 = res.get(c + "/e64"), res.get(c + "/fp32")
+        v[2] = fp32_pays(t64, t32) if t64 is not None else 0
+        print(f"{c}: all significant quartets fp64 {res[c]:.2f} ms; FP32-eligible quartets fp64 "
+              f"{t64 if t64 is None else round(t64, 2)} ms, fp32 {t32 if t32 is None else round(t32, 2)} ms "
+              f"-> fp32 {v[2]}", flush=True)
     write_cfg(args.out, cfg)
 
 

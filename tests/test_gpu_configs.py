@@ -208,13 +208,14 @@ def test_config3_f_sample_fp64_and_mixed(ctx):
     lplan.execute()
     gl, _ = lplan.download()
     n32 = sum(s["nquartets"] for s in mplan.stats() if s["fp32"])
-    Kv, Sv = la.precision_bounds(b, pairs)
-    Km = np.zeros((B.nshell, B.nshell))
-    Sm = np.zeros_like(Km)
-    Km[pairs[:, 0], pairs[:, 1]] = Km[pairs[:, 1], pairs[:, 0]] = Kv
-    Sm[pairs[:, 0], pairs[:, 1]] = Sm[pairs[:, 1], pairs[:, 0]] = Sv
-    KK = Km[q[:, 0], q[:, 1]] * Km[q[:, 2], q[:, 3]]
-    SS = Sm[q[:, 0], q[:, 1]] * Sm[q[:, 2], q[:, 3]]
+    # K and S of each quartet's two pairs in their caller order (S depends on
+    # the build centre, which for s-d pairs is the caller's first shell)
+    nq_ = len(q)
+    up, inv = np.unique(np.concatenate([q[:, :2], q[:, 2:]]).astype(np.int32), axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    Kv, Sv = la.precision_bounds(b, up)
+    KK = Kv[inv[:nq_]] * Kv[inv[nq_:]]
+    SS = Sv[inv[:nq_]] * Sv[inv[nq_:]]
     sel = (KK < THR) & (SS < la.FP32_HRR_FACTOR * THR) & (B.l[q].max(axis=1) <= la.fp32_max_l())
     assert n32 == int(sel.sum())
     sizes = B.sizes()

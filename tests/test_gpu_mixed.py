@@ -35,9 +35,10 @@ def fp32_everywhere(ctx):
 
 
 def test_mixed_default_routes_only_measured_classes(ctx):
-    """With the default routing no class of this build's FP32 variants was
-    measured faster than its FP64 kernel, so a mixed plan runs every quartet
-    in FP64 and gives the FP64 plan's values bit for bit."""
+    """With the default routing a mixed plan sends quartets to FP32 only in
+    the classes whose FP32 variant the autotuner measured faster
+    (fp32_paying_classes); every other class gives the FP64 plan's values
+    bit for bit."""
     ctx.set_fp32_policy(la.FP32_MEASURED)
     mol = molecule.make_molecule(2, natoms=4)
     b = la.Basis(ctx, mol.basis)
@@ -49,11 +50,43 @@ def test_mixed_default_routes_only_measured_classes(ctx):
     p = la.EriPlan(b, q)
     p.execute()
     g, _ = p.download()
-    paying = la.fp32_paying_classes()
-    say(f"  classes whose FP32 variant pays: {paying}")
-    if not paying:
-        assert sum(s["nquartets"] for s in m.stats() if s["fp32"]) == 0
-        assert np.array_equal(gm, g)
+    paying = set(la.fp32_paying_classes())
+    st = m.stats()
+    say(f"  classes whose FP32 variant pays: {sorted(paying)}; FP32 in the plan: "
+        f"{sorted((s['cls'], s['nquartets']) for s in st if s['fp32'])}")
+    assert all(s["cls"] in paying for s in st if s["fp32"])
+    # quartets of the other classes: the FP64 plan's bits
+    B = mol.basis
+    ls = B.l[q]
+    bra = np.sort(ls[:, :2], axis=1)[:, ::-1]
+    ket = np.sort(ls[:, 2:], axis=1)[:, ::-1]
+    swap = bra.sum(1) > ket.sum(1)
+    c1 = np.where(swap[:, None], ket, bra)
+    c2 = np.where(swap[:, None], bra, ket)
+    cls = np.array([f"{a}{b_}{c}{d}" for (a, b_), (c, d) in zip(c1, c2)])
+    sizes = B.sizes()
+    n = sizes[q[:, 0]] * sizes[q[:, 1]] * sizes[q[:, 2]] * sizes[q[:, 3]]
+    other = np.repeat(~np.isin(cls, list(paying)), n)
+    assert other.any()
+    assert np.array_equal(gm[other], g[other])
+    # a batch large enough to fill the chip with FP32 work: the (dd|dd)
+    # list of the 60-atom workload
+    mol = molecule.make_molecule(2)
+    b = la.Basis(ctx, mol.basis)
+    q, _ = molecule.headline_quartets(b)
+    say(f"case: mixed plan, default FP32 routing, the 60-atom (dd|dd) list ({len(q)} quartets)")
+    m = la.EriPlan(b, q, fp32_threshold=THR)
+    st = m.stats()
+    n32 = sum(s["nquartets"] for s in st if s["fp32"])
+    say(f"  {n32} of {len(q)} quartets in FP32")
+    if "2222" in paying:
+        assert n32 > 0
+    m.execute()
+    gm, _ = m.download()
+    p = la.EriPlan(b, q)
+    p.execute()
+    g, _ = p.download()
+    assert np.abs(gm - g).max() <= 1e-5 * THR
 
 
 @pytest.mark.parametrize("workload,natoms,nq", [(2, 6, 60000), (3, 3, 12000)])
