@@ -181,7 +181,13 @@ def class_program(la, lb, lc, ld):
     if lds > LDS_DOUBLES:
         raise ValueError(f"class ({la}{lb}|{lc}{ld}) needs {lds} doubles of LDS")
     f0 = w0  # digest accumulators
-    return dict(la=la, lb=lb, lc=lc, ld=ld, L=L, ne=ne, nf=nf, m0len=m0len, targ0=targ0,
+    # two primitive quartets per pass of the recursion when a second copy of
+    # the [zero][coefficients][level regions] block, pstride doubles further
+    # on, fits the LDS the class already takes: half the barriers per
+    # primitive quartet, and each thread's nodes of both in flight together
+    pstride = reg0 + 3 * region - zero
+    dual = 1 if zero + 2 * pstride <= lds else 0
+    return dict(la=la, lb=lb, lc=lc, ld=ld, L=L, ne=ne, nf=nf, m0len=m0len, targ0=targ0, dual=dual, pstride=pstride,
                 nvals=sum(need.values()),
                 base=base, zero=zero, coef=coef, acc=acc, y0=y0, w0=w0, wrow=wrow, chk=chk,
                 f0=f0, wb0=wb0, wcol=wcol, chb=chb, lds=lds, recs=recs, lev_first=lev_first,
@@ -193,7 +199,7 @@ def class_program(la, lb, lc, ld):
 FIELDS = ["la", "lb", "lc", "ld", "L", "ne", "nf", "m0len", "targ0", "base", "zero", "coef", "acc",
           "y0", "w0", "wrow", "chk", "f0", "wb0", "wcol", "chb", "lds", "vrr_first", "lev_first",
           "hk_first", "hk_nops", "hk_out", "hk_nws", "hb_first", "hb_nops", "hb_out", "hb_nws",
-          "diag_first", "ndiag", "nvals", "accs", "ys"]
+          "diag_first", "ndiag", "nvals", "accs", "ys", "dual", "pstride"]
 
 
 def emit_tables(classes, lmax):

@@ -114,6 +114,16 @@ struct L2gEriArgs {
   long long fx_hoff;                     // 0: FP64 atomics into jg / kg; else fixed point (l2g_acc)
   // Schwarz kernels: optional sqrt(max |[e0|e0]|) of the pre-HRR diagonal
   double* __restrict__ out2;
+  // generic kernel, ERI mode, quartets split over blocks by primitive
+  // range: 1 = a piece pass (work item = piece, q_prange its primitive
+  // quartets; writes its contracted [e0|f0] accumulators and survivor
+  // count to row gpart[item * gpart_stride]); 2 = the finish pass (work
+  // item = quartet; sums rows gpart_first[item] .. gpart_first[item + 1] - 1
+  // in order, then HRR and store); 0 = neither
+  int gpart_mode;
+  long long gpart_stride;
+  double* __restrict__ gpart;
+  const int* __restrict__ gpart_first;
 };
 
 typedef void (*L2gLaunchFn)(const L2gEriArgs&, int grid, int block, hipStream_t s);
@@ -135,6 +145,7 @@ struct L2gGenericClass {
   int diag_first, ndiag;
   int nvals;              // recursion values per primitive quartet
   int accs, ys;           // row strides of the accumulators and of Y (odd)
+  int dual, pstride;      // two primitive quartets per recursion pass; offset of the second's block
 };
 
 struct L2gClassInfo {

@@ -127,70 +127,101 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
       qprec = A.precision;
     }
     int pbeg = 0, pend = npb * npk;
-    if (MODE == kModeFock && A.q_prange) {
+    const int gpart = MODE == kModeEri ? A.gpart_mode : 0;
+    if ((MODE == kModeFock || gpart == 1) && A.q_prange) {
       const int2 pr = A.q_prange[qi];
       pbeg = pr.x;
       pend = pr.y;
     }
-    for (int i = t; i < ne * C.accs; i += kThreads) lds[C.acc + i] = 0.0;
-    if (t < 16) lds[C.zero + t] = 0.0;
-    __syncthreads();
     int nsurv = 0;
-    // ---- 1. primitive quartets: prefactors, F_m, VRR, contraction
-    for (int p = pbeg; p < pend; ++p) {
-      const int ib = p / npk, ik = p - ib * npk;
-      const double4 Pg = A.pp_P[pb0 + ib], PA = A.pp_PA[pb0 + ib];
-      const double2 xb = A.pp_aux[pb0 + ib];
-      const double4 Qg = A.pp_P[pk0 + ik], QC = A.pp_PA[pk0 + ik];
-      const double2 xk = A.pp_aux[pk0 + ik];
-      const double gp = Pg.w, gq = Qg.w;
-      const double rs = rsqrt(gp + gq);
-      const double oos = rs * rs;
-      const double pf = PA.w * QC.w * rs * A.scale;
-      bool ok = xb.x + xk.x > lnprec;
-      if (A.method == L2G_SCREEN_ORIGINAL) {
-        ok = ok && !(fabs(pf) < qprec);
-      } else if (A.method == L2G_SCREEN_CONSERVATIVE) {
-        const double nsb = A.pp_nonsph[pb0 + ib], nsk = A.pp_nonsph[pk0 + ik];
-        ok = ok && !(fabs(pf) * fmax(1.0, nsb * nsk) * npbk < qprec);
+    if (gpart == 2) {
+      // the quartet's pieces, summed in piece order; no primitive loop here
+      const int r0 = A.gpart_first[qi], r1 = A.gpart_first[qi + 1];
+      const int nacc = ne * C.accs;
+      for (int i = t; i < nacc; i += kThreads) {
+        double s = 0.0;
+        for (int r = r0; r < r1; ++r) s += A.gpart[(long long)r * A.gpart_stride + i];
+        lds[C.acc + i] = s;
       }
-      if (!ok) continue;  // block-uniform
-      ++nsurv;
-      const double PQx = Pg.x - Qg.x, PQy = Pg.y - Qg.y, PQz = Pg.z - Qg.z;
-      const double T = gp * gq * oos * (PQx * PQx + PQy * PQy + PQz * PQz);
-      if (t < 17) {
-        const double goq = gq * oos, gop = gp * oos;
-        double v = 0.0;
-        switch (t) {
-          case 0: v = PA.x; break;
-          case 1: v = PA.y; break;
-          case 2: v = PA.z; break;
-          case 3: v = QC.x; break;
-          case 4: v = QC.y; break;
-          case 5: v = QC.z; break;
-          case 6: v = -goq * PQx; break;
-          case 7: v = -goq * PQy; break;
-          case 8: v = -goq * PQz; break;
-          case 9: v = gop * PQx; break;
-          case 10: v = gop * PQy; break;
-          case 11: v = gop * PQz; break;
-          case 12: v = xb.y; break;   // 1/(2 zeta)
-          case 13: v = xk.y; break;   // 1/(2 eta)
-          case 14: v = goq; break;    // rho/zeta
-          case 15: v = gop; break;    // rho/eta
-          default: v = 0.5 * oos;     // 1/(2 (zeta + eta))
+      for (int r = r0; r < r1; ++r) nsurv += (int)A.gpart[(long long)r * A.gpart_stride + nacc];
+      pend = pbeg;
+    } else {
+      for (int i = t; i < ne * C.accs; i += kThreads) lds[C.acc + i] = 0.0;
+    }
+    if (t < 16) {
+      lds[C.zero + t] = 0.0;
+      if (C.dual) lds[C.zero + C.pstride + t] = 0.0;  // the second block's zero vector
+    }
+    __syncthreads();
+    // ---- 1. primitive quartets: prefactors, F_m, VRR, contraction.  With
+    // C.dual the next two surviving primitive quartets go through the
+    // recursion together, the second one's coefficients and level regions
+    // C.pstride doubles after the first's; each node task computes both and
+    // adds them to the accumulator in primitive order.
+    const int slots = C.dual ? 2 : 1;
+    int p = pbeg;
+    while (p < pend) {
+      int ns = 0;  // surviving primitive quartets gathered for this pass (block-uniform)
+      for (; p < pend && ns < slots; ++p) {
+        const int ib = p / npk, ik = p - ib * npk;
+        const double4 Pg = A.pp_P[pb0 + ib], PA = A.pp_PA[pb0 + ib];
+        const double2 xb = A.pp_aux[pb0 + ib];
+        const double4 Qg = A.pp_P[pk0 + ik], QC = A.pp_PA[pk0 + ik];
+        const double2 xk = A.pp_aux[pk0 + ik];
+        const double gp = Pg.w, gq = Qg.w;
+        const double rs = rsqrt(gp + gq);
+        const double oos = rs * rs;
+        const double pf = PA.w * QC.w * rs * A.scale;
+        bool ok = xb.x + xk.x > lnprec;
+        if (A.method == L2G_SCREEN_ORIGINAL) {
+          ok = ok && !(fabs(pf) < qprec);
+        } else if (A.method == L2G_SCREEN_CONSERVATIVE) {
+          const double nsb = A.pp_nonsph[pb0 + ib], nsk = A.pp_nonsph[pk0 + ik];
+          ok = ok && !(fabs(pf) * fmax(1.0, nsb * nsk) * npbk < qprec);
         }
-        lds[C.coef + t] = v;
-      }
-      if (t < C.m0len) {
-        // erf_coulomb (boys.h:1913-1944): G_m = s^(m+1/2) F_m(s T), s = w^2/(w^2 + rho)
-        if (A.omega > 0) {
-          const double w2 = A.omega * A.omega, s = w2 / (w2 + gp * gq * oos);
-          lds[C.base + t] = pf * pow(s, t + 0.5) * boys_one(T * s, t, A.boys);
-        } else {
-          lds[C.base + t] = pf * boys_one(T, t, A.boys);
+        if (!ok) continue;  // block-uniform
+        ++nsurv;
+        const int so = ns * C.pstride;  // this primitive quartet's block
+        ++ns;
+        const double PQx = Pg.x - Qg.x, PQy = Pg.y - Qg.y, PQz = Pg.z - Qg.z;
+        const double T = gp * gq * oos * (PQx * PQx + PQy * PQy + PQz * PQz);
+        if (t < 17) {
+          const double goq = gq * oos, gop = gp * oos;
+          double v = 0.0;
+          switch (t) {
+            case 0: v = PA.x; break;
+            case 1: v = PA.y; break;
+            case 2: v = PA.z; break;
+            case 3: v = QC.x; break;
+            case 4: v = QC.y; break;
+            case 5: v = QC.z; break;
+            case 6: v = -goq * PQx; break;
+            case 7: v = -goq * PQy; break;
+            case 8: v = -goq * PQz; break;
+            case 9: v = gop * PQx; break;
+            case 10: v = gop * PQy; break;
+            case 11: v = gop * PQz; break;
+            case 12: v = xb.y; break;   // 1/(2 zeta)
+            case 13: v = xk.y; break;   // 1/(2 eta)
+            case 14: v = goq; break;    // rho/zeta
+            case 15: v = gop; break;    // rho/eta
+            default: v = 0.5 * oos;     // 1/(2 (zeta + eta))
+          }
+          lds[C.coef + so + t] = v;
+        }
+        if (t < C.m0len) {
+          // erf_coulomb (boys.h:1913-1944): G_m = s^(m+1/2) F_m(s T), s = w^2/(w^2 + rho)
+          if (A.omega > 0) {
+            const double w2 = A.omega * A.omega, s = w2 / (w2 + gp * gq * oos);
+            lds[C.base + so + t] = pf * pow(s, t + 0.5) * boys_one(T * s, t, A.boys);
+          } else {
+            lds[C.base + so + t] = pf * boys_one(T, t, A.boys);
+          }
         }
       }
+      if (ns == 0) break;
+      const bool two = ns == 2;
+      const int s2 = C.pstride;
       __syncthreads();
       // levels; a thread's next record is loaded (at a clamped, always valid
       // index) while it computes the current one, and its first record of
@@ -216,10 +247,26 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
           const double* Y = lds + (rc.y >> 16);
           const double* Z = lds + (rc.z & 0xFFFF);
           double* O = lds + out;
-#pragma unroll 2
-          for (int m = 0; m < mlen; ++m) O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
           const int tg = (rc.z >> 16) - 1;
-          if (tg >= 0) lds[C.acc + tg] += O[0];
+          if (two) {
+            const double cA1 = lds[C.coef + s2 + slot], cW1 = lds[C.coef + s2 + 6 + slot];
+            const double g1 = (double)na * lds[C.coef + s2 + 12 + kind], r1 = lds[C.coef + s2 + 14 + kind];
+            const double h1 = (double)nc * lds[C.coef + s2 + 16];
+#pragma unroll 2
+            for (int m = 0; m < mlen; ++m) {
+              O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
+              O[s2 + m] = cA1 * X[s2 + m] + cW1 * X[s2 + m + 1] + g1 * (Y[s2 + m] - r1 * Y[s2 + m + 1]) +
+                          h1 * Z[s2 + m + 1];
+            }
+            if (tg >= 0) {
+              lds[C.acc + tg] += O[0];
+              lds[C.acc + tg] += O[s2];
+            }
+          } else {
+#pragma unroll 2
+            for (int m = 0; m < mlen; ++m) O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
+            if (tg >= 0) lds[C.acc + tg] += O[0];
+          }
           rc = rn;
           nd = ndn;
         }
@@ -229,9 +276,20 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
         if (k < C.L) rc = prog[C.vrr_first + min(nd, max(nend - 1, 0))];
         __syncthreads();
       }
-      if (C.targ0 >= 0 && t == 0) lds[C.acc + C.targ0] += lds[C.base];
+      if (C.targ0 >= 0 && t == 0) {
+        lds[C.acc + C.targ0] += lds[C.base];
+        if (two) lds[C.acc + C.targ0] += lds[C.base + s2];
+      }
     }
     __syncthreads();
+    if (gpart == 1) {
+      // a piece: its contracted accumulators and survivor count to its row
+      double* row = A.gpart + (long long)qi * A.gpart_stride;
+      const int nacc = ne * C.accs;
+      for (int i = t; i < nacc; i += kThreads) row[i] = lds[C.acc + i];
+      if (t == 0) row[nacc] = (double)nsurv;
+      continue;  // block-uniform
+    }
     if (MODE == kModeEri && t == 0 && A.nz) A.nz[A.qidx[qi]] = nsurv > 0 ? 1 : 0;
     if (MODE == kModeFock && nsurv == 0) continue;  // nothing to digest (libint's nullptr)
     double smax = 0.0, ssq = 0.0, hmax = 0.0;

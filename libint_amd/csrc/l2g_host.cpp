@@ -91,6 +91,9 @@ struct ClassWork {
   int64_t first, count;
   ClassSlot slot;     // grid, scratch region and stream of the launch
   int64_t nprimq;
+  // generic classes of few deep quartets: the quartets split into pieces of
+  // at most `chunk` primitive quartets, a block each, then summed per quartet
+  int64_t npieces = 0, piece_off = 0, pfirst_off = 0, gpart_off = 0, gstride = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float ms = 0;
 };
@@ -108,6 +111,8 @@ struct l2g_eri_plan_s {
   DevBuf bra, ket, off, meta, qidx;
   DevBuf pair_first, pair_np, pair_AB, ppP, ppPA, ppAux, ppNs;
   DevBuf out, ybuf, nz;
+  std::vector<int> itemw;  // primitive quartets of each work item (host)
+  DevBuf gp_bra, gp_ket, gp_meta, gp_qidx, gp_range, gp_first, gpart;  // generic-class pieces
   int64_t ymax = 0;  // largest scratch doubles per lane over the classes
   int block = 128;
   bool wave_split = true;  // classes of few deep quartets run a wave per quartet
@@ -565,6 +570,7 @@ static int plan_create_impl(l2g_basis B, int64_t nq, const int32_t* quartets, do
         meta.push_back(it.meta);
         off.push_back(it.off);
         qidx.push_back(it.q);
+        plan->itemw.push_back(it.w);
         cw.nprimq += it.w;
       }
       HIPCHK(hipEventCreate(&cw.ev0));
@@ -627,9 +633,72 @@ int class_l(const ClassWork& c) {
   return ci.la + ci.lb + ci.lc + ci.ld;
 }
 
+// Generic-kernel classes of few quartets (up to 16 per CU) whose deepest
+// quartet has more primitive quartets than a piece of about 1/(4 ncu) of
+// the class's (at least 32): each quartet splits into pieces of at most
+// that many, a block each, that leave their contracted [e0|f0] values in
+// gpart; a second pass sums each quartet's pieces in piece order and runs
+// the HRR and store.  A block per quartet would otherwise walk the deepest
+// quartet's primitive loop alone while the rest of the chip idles.
+void plan_pieces(l2g_eri_plan_s* p, l2g_context_s* ctx) {
+  std::vector<int> pb, pk, pm, pf;
+  std::vector<long long> pq;
+  std::vector<int2> pr;
+  std::vector<int> hb((size_t)p->nq), hk((size_t)p->nq), hm((size_t)p->nq);
+  std::vector<long long> hq((size_t)p->nq);
+  bool have = false;
+  int64_t gtotal = 0;
+  for (auto& c : p->classes) {
+    c.npieces = 0;
+    if (!c.generic || c.count > (int64_t)ctx->ncu * 16) continue;
+    int wmax = 0;
+    for (int64_t i = c.first; i < c.first + c.count; ++i) wmax = std::max(wmax, p->itemw[(size_t)i]);
+    int64_t chunk = std::max<int64_t>(32, (c.nprimq + 4 * ctx->ncu - 1) / (4 * ctx->ncu));
+    chunk += chunk & 1;  // even: the two-primitive pass stays full
+    if (wmax <= chunk) continue;
+    if (!have) {  // the plan's work-item arrays, read back once
+      HIPCHK(hipMemcpy(hb.data(), p->bra.p, hb.size() * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(hk.data(), p->ket.p, hk.size() * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(hm.data(), p->meta.p, hm.size() * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(hq.data(), p->qidx.p, hq.size() * sizeof(long long), hipMemcpyDeviceToHost));
+      have = true;
+    }
+    const L2gGenericClass& g = l2g_gen_generic[c.cls];
+    c.gstride = (int64_t)g.ne * g.accs + 1;
+    c.piece_off = (int64_t)pb.size();
+    c.pfirst_off = (int64_t)pf.size();
+    for (int64_t i = c.first; i < c.first + c.count; ++i) {
+      pf.push_back((int)((int64_t)pb.size() - c.piece_off));
+      const int w = p->itemw[(size_t)i];
+      for (int a = 0; a < w || a == 0; a += (int)chunk) {
+        pb.push_back(hb[(size_t)i]);
+        pk.push_back(hk[(size_t)i]);
+        pm.push_back(hm[(size_t)i]);
+        pq.push_back(hq[(size_t)i]);
+        pr.push_back(make_int2(a, (int)std::min<int64_t>(w, a + chunk)));
+      }
+    }
+    pf.push_back((int)((int64_t)pb.size() - c.piece_off));
+    c.npieces = (int64_t)pb.size() - c.piece_off;
+    c.gpart_off = gtotal;
+    gtotal += c.npieces * c.gstride;
+  }
+  if (pb.empty()) return;
+  hipStream_t st = ctx->stream;
+  upload(p->gp_bra, pb, st);
+  upload(p->gp_ket, pk, st);
+  upload(p->gp_meta, pm, st);
+  upload(p->gp_qidx, pq, st);
+  upload(p->gp_range, pr, st);
+  upload(p->gp_first, pf, st);
+  p->gpart.alloc(gtotal * sizeof(double));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
 void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
   p->block = 128;
   std::vector<int64_t> lanes, yd;
+  std::vector<double> cost;
   for (auto& c : p->classes) {
     c.generic = c.native_generic;
     c.cls = c.spec_cls;
@@ -646,11 +715,22 @@ void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
     // on the side streams
     lanes.push_back(c.generic ? c.count * 64 : (c.coop ? 64 * c.count : c.count));
     yd.push_back(c.generic ? 0 : (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].ybuf_doubles);
+    // expected time for the side-stream packing: recursion values per
+    // primitive quartet times primitive quartets, at each kernel kind's rate
+    // (ns per value, measured on an s-f batch: generic 0.03, a wave per
+    // quartet 0.005, a lane per quartet 0.02), plus 100 ns per quartet
+    const double vals = c.generic ? (double)l2g_gen_generic[c.cls].nvals
+                                  : (double)(c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].nodes;
+    const double rate = c.generic ? 0.03 : (c.coop ? 0.005 : 0.02);
+    cost.push_back((double)c.nprimq * vals * rate + 100.0 * (double)c.count);
   }
+  plan_pieces(p, ctx);
+  for (size_t i = 0; i < p->classes.size(); ++i)
+    if (p->classes[i].npieces) lanes[i] = p->classes[i].npieces * 64;
   std::vector<ClassSlot> slots;
   // side regions up to 4 GiB of scratch beyond the shared one
   const int64_t total = plan_slots(slots, lanes, yd, ctx->ncu, p->block, l2g_context_s::kSide,
-                                   p->ymax * (int64_t)ctx->ncu * 8 * p->block + (int64_t(1) << 29));
+                                   p->ymax * (int64_t)ctx->ncu * 8 * p->block + (int64_t(1) << 29), &cost);
   for (size_t i = 0; i < p->classes.size(); ++i) p->classes[i].slot = slots[i];
   p->ybuf.alloc(std::max<int64_t>(total, 1) * sizeof(double));
 }
@@ -783,8 +863,31 @@ int l2g_eri_plan_execute(l2g_eri_plan p, double* d_out) {
       } else if (c.generic || A.omega > 0) {  // the attenuated pass of erf / erfc
         // one block per quartet, persistent over the class
         if (c.gidx < 0) throw L2gError(L2G_E_LMAX, "no generic-kernel program for this class");
-        const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
-        launch_generic(kGenericEri, A, l2g_gen_generic[c.gidx], ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid, cs);
+        const L2gGenericClass& gc = l2g_gen_generic[c.gidx];
+        if (c.generic && c.npieces) {  // pieces of the primitive loops, then the sum per quartet
+          L2gEriArgs P = A;
+          P.bra_pair = p->gp_bra.as<int>() + c.piece_off;
+          P.ket_pair = p->gp_ket.as<int>() + c.piece_off;
+          P.meta = p->gp_meta.as<int>() + c.piece_off;
+          P.qidx = p->gp_qidx.as<long long>() + c.piece_off;
+          P.q_prange = p->gp_range.as<int2>() + c.piece_off;
+          P.n = (int)c.npieces;
+          P.gpart_mode = 1;
+          P.gpart = p->gpart.as<double>() + c.gpart_off;
+          P.gpart_stride = c.gstride;
+          launch_generic(kGenericEri, P, gc, ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
+                         (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.npieces), cs);
+          P = A;
+          P.gpart_mode = 2;
+          P.gpart = p->gpart.as<double>() + c.gpart_off;
+          P.gpart_stride = c.gstride;
+          P.gpart_first = p->gp_first.as<int>() + c.pfirst_off;
+          launch_generic(kGenericEri, P, gc, ctx->gprog.as<int4>(), ctx->gidx.as<int>(),
+                         (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count), cs);
+        } else {
+          const int grid = (int)std::min<int64_t>((int64_t)ctx->ncu * 8, c.count);
+          launch_generic(kGenericEri, A, gc, ctx->gprog.as<int4>(), ctx->gidx.as<int>(), grid, cs);
+        }
       } else {
         const L2gClassInfo& ci = (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls];
         A.coop = c.coop ? 1 : 0;

@@ -199,13 +199,17 @@ void parallel_for(int n, F&& f) {
 // doubles per lane; a class launches exactly slots[i].grid blocks of `block`
 // threads (with ystride = grid * block).  Returns the scratch doubles needed;
 // when the side regions would exceed cap_doubles every class runs on the
-// main stream (side = -1) in the shared region.
+// main stream (side = -1) in the shared region.  cost (optional): each
+// class's expected time; the small classes then go to the side streams
+// longest first, each to the stream with the least expected time so far (so
+// a long class starts at once and the others pack beside it), else in turn.
 struct ClassSlot {
   int64_t yoff = 0, ystride = 0;
   int grid = 1, side = -1;
 };
 inline int64_t plan_slots(std::vector<ClassSlot>& slots, const std::vector<int64_t>& lanes,
-                          const std::vector<int64_t>& ydoubles, int ncu, int block, int nside, int64_t cap_doubles) {
+                          const std::vector<int64_t>& ydoubles, int ncu, int block, int nside, int64_t cap_doubles,
+                          const std::vector<double>* cost = nullptr) {
   const size_t n = lanes.size();
   slots.assign(n, ClassSlot());
   const int64_t maxblocks = (int64_t)ncu * 8, layer = (int64_t)ncu * 256;
@@ -217,7 +221,7 @@ inline int64_t plan_slots(std::vector<ClassSlot>& slots, const std::vector<int64
     s.grid = (int)std::max<int64_t>(1, std::min<int64_t>(maxblocks, (lanes[i] + block - 1) / block));
     s.ystride = (int64_t)s.grid * block;
     if (lanes[i] <= layer && nside > 0) {
-      s.side = nsmall++ % nside;
+      s.side = nsmall++ % nside;  // reassigned below when costs are given
       s.yoff = -1;  // placed below
       sides += ydoubles[i] * s.ystride;
     } else {
@@ -231,6 +235,18 @@ inline int64_t plan_slots(std::vector<ClassSlot>& slots, const std::vector<int64
       shared = std::max(shared, ydoubles[i] * slots[i].ystride);
     }
     return shared;
+  }
+  if (cost && nsmall) {
+    std::vector<size_t> order;
+    for (size_t i = 0; i < n; ++i)
+      if (lanes[i] > 0 && slots[i].side >= 0) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return (*cost)[a] > (*cost)[b]; });
+    std::vector<double> load(nside, 0.0);
+    for (size_t i : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      slots[i].side = k;
+      load[k] += (*cost)[i];
+    }
   }
   int64_t off = shared;
   for (size_t i = 0; i < n; ++i)

@@ -23,6 +23,7 @@ Parity status per function (how each is pinned is listed in the tests):
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -99,6 +100,7 @@ def lib():
             _f64p, ctypes.c_double, _f64p, ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_int,
             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.oracle_fock_2e_arbitrated.restype = ctypes.c_int
+        L.oracle_set_fock_rows.argtypes = [ctypes.c_int, ctypes.c_void_p]
         L.oracle_fock_pairs.argtypes = basis_args + [ctypes.c_double, _i8p, _f64p]
         L.oracle_fock_pairs.restype = ctypes.c_int
         _lib = L
@@ -324,6 +326,39 @@ def fock_2e_exact(basis, D, precision=1e-12, screening=SCHWARZ_INF, nthreads=0, 
     finally:
         L.oracle_set_tight_centre(0)
     return G, JK[0], JK[1], nq.value, narb.value
+
+
+def fock_rows_exact(basis, D, shells, precision=1e-12, screening=SCHWARZ_INF, nthreads=0):
+    """Rows of fock_2e_exact's G, J, K for the functions of `shells`, from
+    only the quartets that touch those shells (all that add to their rows).
+    Returns (rows, G_rows, J_rows, K_rows, n_quartets, n_quartets_in_quad)."""
+    rows = shell_rows(basis, shells)
+    with fock_rows_only(basis, shells):
+        G, J, K, nq, nquad = fock_2e_exact(basis, D, precision, screening, nthreads)
+    return rows, G[rows], J[rows], K[rows], nq, nquad
+
+
+def shell_rows(basis, shells):
+    """Basis-function indices of `shells`, in order."""
+    ob = OBasis.of(basis)
+    sizes = np.where(ob.pure != 0, 2 * ob.l + 1, (ob.l + 1) * (ob.l + 2) // 2)
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    return np.concatenate([np.arange(first[s], first[s + 1]) for s in shells])
+
+
+@contextlib.contextmanager
+def fock_rows_only(basis, shells):
+    """Within the block, the Fock builds digest only the quartets touching
+    `shells`: the rows (and columns) of their functions come out whole, the
+    rest of G, J, K is partial."""
+    mask = np.zeros(len(OBasis.of(basis).l), np.int8)
+    mask[np.asarray(shells, np.int64)] = 1
+    L = lib()
+    L.oracle_set_fock_rows(len(mask), mask.ctypes.data)
+    try:
+        yield
+    finally:
+        L.oracle_set_fock_rows(0, None)
 
 
 def fock_pairs(basis, threshold=1e-12, tight_centre=False):

@@ -653,6 +653,11 @@ static void hrr_transfer(int l1, int l2, const Real AB[3], const Real* X, int re
 // a diffuse and a tight shell.  Never used by the reference restatement.
 static int g_tight_centre = 0;
 static int g_progress = 0;  // fock_build prints its progress to stderr (long runs)
+// fock_build digests only the quartets with at least one shell in g_rows
+// (when not empty): every quartet that adds to a row or column of those
+// shells' functions, so their rows of G, J and K come out whole (the
+// sampled-row check of large Fock builds)
+static std::vector<int8_t> g_rows;
 static int g_oper = 0;  // 0 coulomb, 1 erf_coulomb, 2 erfc_coulomb
 static double g_omega = 0;
 template <class Boys>
@@ -1393,6 +1398,7 @@ static void fock_build(const std::vector<Shell>& sh, const FockSetup& F, const d
           for (size_t i4 = 0; i4 < F.splist[s3].size(); ++i4) {
             const int s4 = F.splist[s3][i4];
             if (s4 > s4max) break;
+            if (!g_rows.empty() && !(g_rows[s1] | g_rows[s2] | g_rows[s3] | g_rows[s4])) continue;
             const ShellPair* sp34 = F.sp[s3][i4].get();
             const double Dn1234 =
                 std::max(std::max(Dn[(size_t)s1 * n + s4], Dn[(size_t)s2 * n + s4]), std::max(Dn[(size_t)s3 * n + s4], Dn123));
@@ -1786,6 +1792,15 @@ int oracle_fock_2e_arbitrated(int nshell, const int* l, const int* pure, const i
   const size_t NN = (size_t)F.nbf * F.nbf;
   fock_build(B.shells, F, D, precision, method, G, JK ? JK : nullptr, JK ? JK + NN : nullptr, nq, arbitrate, narb);
   return F.nbf;
+}
+
+// Restricts the Fock builds to quartets touching the shells with mask[s]
+// != 0 (mask NULL or nshell 0: every quartet).
+void oracle_set_fock_rows(int nshell, const int8_t* mask) {
+  if (!mask || nshell <= 0)
+    g_rows.clear();
+  else
+    g_rows.assign(mask, mask + nshell);
 }
 
 // Significant shell pairs (overlap-norm >= threshold or same centre) and the

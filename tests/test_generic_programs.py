@@ -15,9 +15,13 @@ def cart_shell(l, prims, origin):
     return la.Shell(l, prims, origin, pure=False)
 
 
-def emulate(prog, shells):
+def emulate(prog, shells, dual=False):
     """Contracted Cartesian (ab|cd) of the canonical class `prog` for four
-    renormalised Cartesian shells, walking the program records."""
+    renormalised Cartesian shells, walking the program records.  dual: the
+    kernel's two-primitive pass (prog["dual"] classes): primitive quartets
+    in pairs, the second one's zero vector, coefficients and level regions
+    prog["pstride"] doubles after the first's, both added per node in
+    primitive order."""
     A, B, C, D = (np.array(s.O) for s in shells)
     la_, lb_, lc_, ld_ = prog["la"], prog["lb"], prog["lc"], prog["ld"]
     ne, nf, L = prog["ne"], prog["nf"], prog["L"]
@@ -38,6 +42,7 @@ def emulate(prog, shells):
                 out.append((g, P, K))
         return out
 
+    prims = []
     for (gp, P, Kb) in pairs(shells[0], shells[1], A, B):
         for (gq, Q, Kk) in pairs(shells[2], shells[3], C, D):
             rho = gp * gq / (gp + gq)
@@ -49,24 +54,35 @@ def emulate(prog, shells):
             coef[6:9], coef[9:12] = -gq / (gp + gq) * PQ, gp / (gp + gq) * PQ
             coef[12], coef[13] = 0.5 / gp, 0.5 / gq
             coef[14], coef[15], coef[16] = gq / (gp + gq), gp / (gp + gq), 0.5 / (gp + gq)
-            lds[prog["zero"]:prog["zero"] + 16] = 0
             F = oracle.boys_reference(T, L)[0]
-            lds[prog["base"]:prog["base"] + prog["m0len"]] = pf * F[:prog["m0len"]]
-            for k in range(1, L + 1):
-                for nd in range(lev[k - 1], lev[k]):
-                    x, y, z, _ = rec[nd]
-                    out, mlen, slot = x & 0x7FFF, (x >> 15) & 15, (x >> 19) & 7
-                    na, nc, kind = (x >> 22) & 7, (x >> 25) & 7, (x >> 28) & 1
-                    ox, oy, oz, tg = y & 0xFFFF, y >> 16, z & 0xFFFF, (z >> 16) - 1
+            prims.append((coef, pf * F[:prog["m0len"]]))
+    S = prog["pstride"] if dual else 0
+    step = 2 if dual else 1
+    for i0 in range(0, len(prims), step):
+        group = prims[i0:i0 + step]
+        for s, (_, base) in enumerate(group):
+            lds[prog["zero"] + s * S:prog["zero"] + s * S + 16] = 0
+            lds[prog["base"] + s * S:prog["base"] + s * S + prog["m0len"]] = base
+        for k in range(1, L + 1):
+            for nd in range(lev[k - 1], lev[k]):
+                x, y, z, _ = rec[nd]
+                out, mlen, slot = x & 0x7FFF, (x >> 15) & 15, (x >> 19) & 7
+                na, nc, kind = (x >> 22) & 7, (x >> 25) & 7, (x >> 28) & 1
+                ox, oy, oz, tg = y & 0xFFFF, y >> 16, z & 0xFFFF, (z >> 16) - 1
+                for s, (coef, _) in enumerate(group):
+                    o = s * S
                     cA, cW = coef[slot], coef[6 + slot]
                     g, r, h = na * coef[12 + kind], coef[14 + kind], nc * coef[16]
                     for m in range(mlen):
-                        lds[out + m] = (cA * lds[ox + m] + cW * lds[ox + m + 1]
-                                        + g * (lds[oy + m] - r * lds[oy + m + 1]) + h * lds[oz + m + 1])
-                    if tg >= 0:
-                        acc[tg] += lds[out]
-            if prog["targ0"] >= 0:
-                acc[prog["targ0"]] += lds[prog["base"]]
+                        lds[o + out + m] = (cA * lds[o + ox + m] + cW * lds[o + ox + m + 1]
+                                            + g * (lds[o + oy + m] - r * lds[o + oy + m + 1])
+                                            + h * lds[o + oz + m + 1])
+                if tg >= 0:
+                    for s in range(len(group)):
+                        acc[tg] += lds[s * S + out]
+        if prog["targ0"] >= 0:
+            for s in range(len(group)):
+                acc[prog["targ0"]] += lds[prog["base"] + s * S]
     CD, AB = C - D, A - B
     ncc = gen_eri.ncart(lc_) * gen_eri.ncart(ld_)
     Y = np.zeros((ne, ncc))
@@ -104,6 +120,29 @@ def test_generic_program_matches_oracle(cls):
     assert v.shape == o.shape
     d = np.abs(v - o)
     assert (d <= 1e-12 + 1e-10 * np.abs(o)).all(), d.max()
+
+
+@pytest.mark.parametrize("cls", [(2, 2, 3, 3), (3, 1, 3, 3), (3, 2, 3, 3)])
+def test_generic_dual_pass(cls):
+    """The kernel's two-primitive pass on classes whose LDS holds it: the
+    second block stays inside the class's LDS and off the first block and
+    the accumulators, and walking the records in pairs (with an odd count,
+    so the last pass has one) gives the single-pass values bit for bit."""
+    prog = gen_generic.class_program(*cls)
+    assert prog["dual"] == 1
+    zero, ps = prog["zero"], prog["pstride"]
+    assert prog["acc"] + prog["ne"] * prog["accs"] <= zero and zero + 2 * ps <= prog["lds"]
+    for (x, y, z, _) in prog["recs"]:
+        mlen = (x >> 15) & 15
+        for o, n in ((x & 0x7FFF, mlen), (y & 0xFFFF, mlen + 1), (y >> 16, mlen + 1), (z & 0xFFFF, mlen + 1)):
+            assert zero <= o and o + n <= zero + ps
+    rng = np.random.default_rng(sum(cls))
+    ks = [1, 3, 1, 1]  # 3 primitive quartets
+    shells = [cart_shell(l, list(zip(rng.uniform(0.3, 2.0, k), rng.uniform(0.2, 1.0, k))),
+                         tuple(rng.uniform(-0.8, 0.8, 3))) for l, k in zip(cls, ks)]
+    v1 = emulate(prog, shells)
+    v2 = emulate(prog, shells, dual=True)
+    assert np.array_equal(v1, v2)
 
 
 def test_generic_programs_fit_lds():

@@ -178,3 +178,33 @@ def test_few_deep_quartets_route_to_generic(ctx):
     shallow = random_shells(rng, [2, 2, 2, 2], kmax=1)
     st2 = la.EriPlan(la.Basis(ctx, la.BasisArrays.from_shells(shallow)), q).stats()
     assert [(s["cls"], s["generic"]) for s in st2] == [("2222", False)], st2
+
+
+def test_deep_quartets_split_over_blocks(ctx):
+    """Few (fd|ff) and (ff|ff) quartets of 8-primitive shells (4096 primitive
+    quartets each): the generic kernel splits each quartet's primitive loop
+    into pieces run by separate blocks and sums them per quartet in piece
+    order.  At the integral bar against the oracle, and the same bits on a
+    second execution (the sum has a fixed order)."""
+    rng = np.random.default_rng(77)
+    shells = [la.Shell(l, list(zip(np.exp(rng.uniform(np.log(0.05), np.log(5e3), 8)), rng.uniform(0.1, 1.0, 8))),
+                       tuple(rng.uniform(-0.6, 0.6, 3)), pure=p)
+              for l, p in ((3, True), (2, True), (3, False), (3, True))]
+    arrays = la.BasisArrays.from_shells(shells)
+    q = np.array([(0, 1, 2, 3), (2, 3, 0, 1), (0, 2, 3, 0), (3, 2, 2, 0), (0, 3, 2, 3), (1, 0, 3, 2),
+                  (0, 0, 3, 3), (2, 0, 3, 1)], np.int32)
+    say(f"case: {len(q)} deep f quartets (K = 8 on every shell), split primitive loops")
+    b = la.Basis(ctx, arrays)
+    plan = la.EriPlan(b, q)
+    plan.execute()
+    g1, nz1 = plan.download()
+    plan.execute()
+    g2, _ = plan.download()
+    assert np.array_equal(g1, g2)
+    o, off, onz = oracle.eri_quartets(arrays, q)
+    assert np.array_equal(plan.offsets(), off)
+    g1 = g1[:len(o)]
+    nfail, narb, oerr = compare_arbitrated(arrays, q, off, g1, o, ABS, REL)
+    say(f"  classes {[c['cls'] + ('/generic' if c['generic'] else '') for c in plan.stats()]}; "
+        f"max abs {np.abs(g1 - o).max():.2e}; fails {nfail}, arbitrated {narb}")
+    assert nfail == 0

@@ -434,3 +434,25 @@ def test_parity_arbitration_rule():
     nfail, narb, _ = compare_arbitrated(b2, q2, off2, bad, v2, 1e-12, 1e-10)
     assert nfail == 1 and narb == 1
     assert compare_arbitrated(b2, q2, off2, v2, v2, 1e-12, 1e-10)[0] == 0
+
+
+def test_fock_rows_only_matches_full_build():
+    """The row-restricted Fock build (quartets touching the sampled shells
+    only) gives the full build's rows of G, J, K for those shells' functions,
+    and leaves others partial, so the filter is not a no-op."""
+    from libint_amd import molecule
+    mol = molecule.make_molecule(2, natoms=3, same_density=True)
+    rng = np.random.default_rng(3)
+    C = rng.normal(0.0, 0.1, (mol.nbf, mol.nbf // 5))
+    D = C @ C.T
+    G, J, K, nq = oracle.fock_2e(mol.basis, D, 1e-12, nthreads=4)
+    shells = [0, 7, int(mol.basis.nshell) - 1]
+    rows = oracle.shell_rows(mol.basis, shells)
+    with oracle.fock_rows_only(mol.basis, shells):
+        Gr, Jr, Kr, nqr = oracle.fock_2e(mol.basis, D, 1e-12, nthreads=4)
+    assert 0 < nqr < nq
+    for X, Y in ((G, Gr), (J, Jr), (K, Kr)):
+        assert np.allclose(X[rows], Y[rows], rtol=0, atol=1e-13)
+        assert np.allclose(X[:, rows], Y[:, rows], rtol=0, atol=1e-13)
+    others = np.setdiff1d(np.arange(mol.nbf), rows)
+    assert np.abs(G[np.ix_(others, others)] - Gr[np.ix_(others, others)]).max() > 1e-3
