@@ -168,6 +168,56 @@ def fock_leg(la, molecule, flop_model, ctx, natoms, reps, workload=2, same_densi
     return out
 
 
+def config4_leg(la, molecule, mode, rows_out=None, natoms=200):
+    """One J/K build of the 200-atom f recipe (nbf 6000, D = C C^T with C ~
+    N(0, 0.1) of nbf/5 columns, seed 12, precision 1e-12, SchwarzInf), FP64
+    or mixed (every class's FP32 variant where K_ab K_cd < 1e-6): build and
+    kernel time, the classes that take the most time, and (rows_out) the
+    rows of G, J, K of molecule.sampled_row_shells plus every 50th row of G
+    and its diagonal, for the check against the oracle's exact rows
+    (tests/test_config4_rows.py)."""
+    mol = molecule.make_molecule(4, natoms=natoms, same_density=natoms != 200)
+    shells = molecule.sampled_row_shells(mol)
+    sizes = mol.basis.sizes()
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    rows = np.concatenate([np.arange(first[s], first[s + 1]) for s in shells])
+    every = np.arange(0, mol.nbf, 50)
+    D = density(mol.nbf, 12)
+    ctx = la.Context(0)
+    if mode == "mixed":
+        ctx.set_fp32_policy(la.FP32_ALWAYS)
+    b = la.Basis(ctx, mol.basis)
+    fb = la.FockBuilder(b)
+    fb.set_fp32_threshold(1e-6 if mode == "mixed" else 0.0)
+    state = {"t0": time.perf_counter(), "last": time.perf_counter()}
+
+    def progress(done, total):
+        now = time.perf_counter()
+        if now - state["last"] > 20 or done == total:
+            log(f"  {done:.4e} / {total:.4e} quartets, {now - state['t0']:.0f} s")
+            state["last"] = now
+
+    fb.set_progress(progress)
+    log(f"config 4: {natoms} atoms, nbf {mol.nbf}, {mode}")
+    state["t0"] = time.perf_counter()
+    G, J, K = fb.build(D, 1e-12)
+    dt = time.perf_counter() - state["t0"]
+    st = fb.stats()
+    top = sorted(fb.class_stats(), key=lambda c: -c["ms"])[:8]
+    out = {"workload": f"{natoms} atoms of the f recipe (s4p3d2f1, K~U{{1..8}}, exponents [0.03,2e4]), nbf {mol.nbf}, "
+                       "D = C C^T, C ~ N(0, 0.1) (seed 12), precision 1e-12, SchwarzInf",
+           "mode": mode, "build_s": dt, "kernel_s": st["kernel_ms"] / 1e3, "quartets": st["nquartets"],
+           "quartets_fp32": st["nquartets_fp32"], "quartets_per_s": st["nquartets"] / dt,
+           "top_classes": [f"{c['cls']}{'/fp32' if c['fp32'] else ''}{'/generic' if c['generic'] else ''}: "
+                           f"{c['ms'] / 1e3:.1f} s, {c['nquartets']:.3e} quartets, {c['nprim']:.3e} primitive quartets"
+                           for c in top],
+           "symmetric": bool(np.allclose(G, G.T, rtol=0, atol=1e-10))}
+    if rows_out:
+        np.savez(rows_out, shells=np.array(shells), rows=rows, G=G[rows], J=J[rows], K=K[rows], every=every,
+                 Ge=G[every], diag=np.diag(G), build_s=dt, mode=mode)
+    return out
+
+
 def eri_f_leg(la, molecule, flop_model, ctx, nquartets, reps=3, seed=21):
     """Batched ERIs of the high-angular-momentum recipe (120 atoms in a 20 A
     cube, s4p3d2f1, K~U{1..8}): a seeded sample of quartets with
@@ -456,6 +506,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--fock-f-only", action="store_true",
                     help="only the f-recipe J/K build leg (for profiling its kernels) and exit")
+    ap.add_argument("--config4", choices=["fp64", "mixed"],
+                    help="only the 200-atom f-recipe J/K build (about 16 min on one GPU) and exit")
+    ap.add_argument("--rows-out", help="with --config4: .npz of the sampled rows for tests/test_config4_rows.py")
     ap.add_argument("--serve-f-requests", type=int, default=24,
                     help="requests of the f-recipe serving leg (10-20 atoms at the 200-atom density; 0: skip)")
     args = ap.parse_args()
@@ -476,6 +529,10 @@ def main():
 
     import libint_amd as la
     from libint_amd import flop_model, molecule
+
+    if args.config4:
+        print(json.dumps(config4_leg(la, molecule, args.config4, args.rows_out)), flush=True)
+        return
 
     if args.fock_f_only:
         ctx = la.Context(0)

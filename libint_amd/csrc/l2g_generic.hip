@@ -247,24 +247,35 @@ __global__ void __launch_bounds__(NT) l2g_generic_kernel(const L2gEriArgs A, con
           const double* Y = lds + (rc.y >> 16);
           const double* Z = lds + (rc.z & 0xFFFF);
           double* O = lds + out;
+          // X[m] and Y[m] carried from the previous m (three LDS reads per
+          // value, not five): O lies in level k's region, X, Y and Z in
+          // levels k - 1 and k - 2 (or the zero vector), so no store of O
+          // changes a value read later
           const int tg = (rc.z >> 16) - 1;
           if (two) {
             const double cA1 = lds[C.coef + s2 + slot], cW1 = lds[C.coef + s2 + 6 + slot];
             const double g1 = (double)na * lds[C.coef + s2 + 12 + kind], r1 = lds[C.coef + s2 + 14 + kind];
             const double h1 = (double)nc * lds[C.coef + s2 + 16];
+            double x0 = X[0], y0 = Y[0], u0 = X[s2], v0 = Y[s2];
 #pragma unroll 2
             for (int m = 0; m < mlen; ++m) {
-              O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
-              O[s2 + m] = cA1 * X[s2 + m] + cW1 * X[s2 + m + 1] + g1 * (Y[s2 + m] - r1 * Y[s2 + m + 1]) +
-                          h1 * Z[s2 + m + 1];
+              const double x1 = X[m + 1], y1 = Y[m + 1], u1 = X[s2 + m + 1], v1 = Y[s2 + m + 1];
+              O[m] = cA * x0 + cW * x1 + g * (y0 - r * y1) + h * Z[m + 1];
+              O[s2 + m] = cA1 * u0 + cW1 * u1 + g1 * (v0 - r1 * v1) + h1 * Z[s2 + m + 1];
+              x0 = x1, y0 = y1, u0 = u1, v0 = v1;
             }
             if (tg >= 0) {
               lds[C.acc + tg] += O[0];
               lds[C.acc + tg] += O[s2];
             }
           } else {
+            double x0 = X[0], y0 = Y[0];
 #pragma unroll 2
-            for (int m = 0; m < mlen; ++m) O[m] = cA * X[m] + cW * X[m + 1] + g * (Y[m] - r * Y[m + 1]) + h * Z[m + 1];
+            for (int m = 0; m < mlen; ++m) {
+              const double x1 = X[m + 1], y1 = Y[m + 1];
+              O[m] = cA * x0 + cW * x1 + g * (y0 - r * y1) + h * Z[m + 1];
+              x0 = x1, y0 = y1;
+            }
             if (tg >= 0) lds[C.acc + tg] += O[0];
           }
           rc = rn;

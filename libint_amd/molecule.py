@@ -200,3 +200,19 @@ def stratified_quartets(arrays: BasisArrays, pairs: np.ndarray, K: np.ndarray, p
         bk = rng.integers(0, 2, len(q)).astype(bool)
         q[bk] = q[bk][:, [2, 3, 0, 1]]
     return q
+
+
+def sampled_row_shells(mol: Molecule) -> list[int]:
+    """Shells whose rows of a large J/K build are kept for checking: the
+    first s, d and f shells of the atom nearest the centre of the molecule,
+    then the first f shell of the atom farthest from it (the fewest
+    quartets touch it, so its exact rows are the cheapest to compute)."""
+    B = mol.basis
+    c = mol.coords.mean(0)
+    d2 = ((mol.coords - c) ** 2).sum(1)
+    near, far = int(np.argmin(d2)), int(np.argmax(d2))
+    pick = []
+    for l in (0, 2, 3):
+        pick += [s for s in range(B.nshell) if mol.atom_of_shell[s] == near and B.l[s] == l][:1]
+    pick += [s for s in range(B.nshell) if mol.atom_of_shell[s] == far and B.l[s] == 3][:1]
+    return pick

@@ -150,3 +150,22 @@ def test_generic_programs_fit_lds():
         p = gen_generic.class_program(*c)
         assert p["lds"] <= gen_generic.LDS_DOUBLES
         assert p["chk"] >= 1 and p["chb"] >= 1
+
+
+def test_generic_levels_write_apart_from_their_reads():
+    """Within a level, no record's output range meets any record's operand
+    ranges (X, Y, Z with one more m): the kernel's threads write a level
+    while others read, and each thread carries X[m], Y[m] over its m-loop in
+    registers, both sound only when a level's writes and reads are apart."""
+    for c in gen_eri.canonical_classes(3):
+        p = gen_generic.class_program(*c)
+        recs, lev = p["recs"], p["lev_first"]
+        for k in range(len(lev) - 1):
+            w = np.zeros(p["lds"], bool)
+            r = np.zeros(p["lds"], bool)
+            for (x, y, z, _) in recs[lev[k]:lev[k + 1]]:
+                mlen = (x >> 15) & 15
+                w[x & 0x7FFF:(x & 0x7FFF) + mlen] = True
+                for o in (y & 0xFFFF, y >> 16, z & 0xFFFF):
+                    r[o:o + mlen + 1] = True
+            assert not (w & r).any(), (c, k)
