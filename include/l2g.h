@@ -218,7 +218,11 @@ int l2g_eri_plan_set_operator(l2g_eri_plan plan, int oper, double omega);
 /* Wave split (on by default; on = 0 turns it off): classes of few, deeply
  * contracted quartets run a wave per quartet whose 64 lanes split the
  * primitive loop, instead of one lane per quartet, so that a handful of
- * long contractions no longer serialise on a few lanes.  Each contraction
+ * long contractions no longer serialise on a few lanes.  With on = 1 a
+ * class measured faster there on the generic kernel (its quartets split
+ * over blocks by primitive range, summed per quartet in a fixed order) runs
+ * on it instead when it holds at least half of the plan's quartets; on = 2
+ * keeps every such class on its register kernel.  Each contraction
  * is then summed as 64 serial partial sums joined by a fixed butterfly,
  * another order than libint's serial primitive loop
  * (engine.impl.h:1306-1766).  Well-conditioned integrals agree with the

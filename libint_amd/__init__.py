@@ -467,8 +467,10 @@ class EriPlan:
         """Classes of few, deeply contracted quartets run a wave per quartet
         (lanes split the primitive loop; on by default): each contraction is
         summed as 64 partial sums and a fixed butterfly instead of libint's
-        serial order."""
-        _check(lib().l2g_eri_plan_set_wave_split(self._h, int(bool(on))))
+        serial order.  True (1) routes the classes measured faster on the
+        generic kernel there instead; 2 keeps them all on their register
+        kernels; False (0) turns the split off."""
+        _check(lib().l2g_eri_plan_set_wave_split(self._h, int(on)))
 
     def elapsed_ms(self):
         """Duration of the last execution (all classes), waiting for it."""

@@ -1047,7 +1047,7 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
 
     def cfg(c):
         v = budgets.get("".join(map(str, c)), (budget, waves))
-        return tuple(v) + (0,) * (3 - len(v))  # (budget, waves, FP32 pays)
+        return tuple(v) + (0,) * (4 - len(v))  # (budget, waves, FP32 pays, generic for few deep quartets)
 
     plans = [plan_class(*c, budget=cfg(c)[0]) for c in classes]
     if max_nodes:
@@ -1121,10 +1121,11 @@ def generate(outdir, lmax=2, budget=160, nfiles=8, waves=0, only=None, budgets=N
             if not in_regs and name in ("l2g_gen_classes", "l2g_gen_classes32", "l2g_gen_classes_erf"):
                 ybuf += ncart(p.la) * ncart(p.lb) * ncart(p.lc) * ncart(p.ld)  # staged output
             pays = cfg((p.la, p.lb, p.lc, p.ld))[2] if name in ("l2g_gen_classes32", "l2g_gen_fock32") else 0
+            fewgen = cfg((p.la, p.lb, p.lc, p.ld))[3] if name == "l2g_gen_classes" else 0
             lines.append(f"  {{{p.la}, {p.lb}, {p.lc}, {p.ld}, {len(p.groups)}, {ybuf}, "
-                         f"{sum(p.group_nodes)}, {pays}, &{k}_launch}},")
+                         f"{sum(p.group_nodes)}, {pays}, {fewgen}, &{k}_launch}},")
         if not tab:
-            lines.append("  {0, 0, 0, 0, 0, 0, 0, 0, nullptr},")
+            lines.append("  {0, 0, 0, 0, 0, 0, 0, 0, 0, nullptr},")
         lines.append("};")
     lines.append(f"extern const int l2g_gen_nclasses = {len(table)};")
     lines.append(f"extern const int l2g_gen_nschwarz = {len(stable)};")

@@ -154,6 +154,7 @@ struct L2gClassInfo {
   int ybuf_doubles;  // per lane; 0 = rows kept in registers
   int nodes;         // VRR graph nodes per primitive quartet (all groups)
   int fp32_pays;     // FP32 variants: measured faster than the class's FP64 kernel
+  int few_generic;   // FP64 ERI kernels: few deep quartets measured faster on the generic kernel
   L2gLaunchFn launch;
 };
 

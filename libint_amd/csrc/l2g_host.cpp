@@ -116,6 +116,7 @@ struct l2g_eri_plan_s {
   int64_t ymax = 0;  // largest scratch doubles per lane over the classes
   int block = 128;
   bool wave_split = true;  // classes of few deep quartets run a wave per quartet
+  bool few_generic = true;  // ... or the generic kernel, where measured faster
   int oper = 0;             // 0 Coulomb, 1 erf-attenuated, 2 erfc-attenuated
   double omega = 0;
   // hipGraph mode: the class launches are captured once per output buffer
@@ -703,6 +704,16 @@ void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
     c.generic = c.native_generic;
     c.cls = c.spec_cls;
     const bool few = c.count * 4 <= (int64_t)ctx->ncu * 256;
+    // few deep quartets of a class whose register kernel was measured slower
+    // there than the generic kernel (with its quartets split over blocks),
+    // in a plan that is mostly this class: the split generic passes fill
+    // the chip's LDS, and beside other classes (a mixed batch) the wave per
+    // quartet, which leaves most of the chip to them, measured faster overall
+    if (p->wave_split && p->few_generic && !c.generic && !c.fp32 && few && c.nprimq >= 16 * c.count &&
+        c.gidx >= 0 && l2g_gen_classes[c.spec_cls].few_generic && c.count * 2 >= p->nq) {
+      c.generic = true;
+      c.cls = c.gidx;
+    }
     c.coop = p->wave_split && !c.generic && few && c.nprimq >= 16 * c.count;
     if (!c.generic && !c.coop && !c.fp32 && c.gidx >= 0 && few && c.nprimq >= L2G_GENERIC_MIN_DEPTH * c.count &&
         class_l(c) >= L2G_GENERIC_MIN_L) {
@@ -715,18 +726,21 @@ void plan_layout(l2g_eri_plan_s* p, l2g_context_s* ctx) {
     // on the side streams
     lanes.push_back(c.generic ? c.count * 64 : (c.coop ? 64 * c.count : c.count));
     yd.push_back(c.generic ? 0 : (c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].ybuf_doubles);
-    // expected time for the side-stream packing: recursion values per
-    // primitive quartet times primitive quartets, at each kernel kind's rate
-    // (ns per value, measured on an s-f batch: generic 0.03, a wave per
-    // quartet 0.005, a lane per quartet 0.02), plus 100 ns per quartet
-    const double vals = c.generic ? (double)l2g_gen_generic[c.cls].nvals
-                                  : (double)(c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].nodes;
-    const double rate = c.generic ? 0.03 : (c.coop ? 0.005 : 0.02);
-    cost.push_back((double)c.nprimq * vals * rate + 100.0 * (double)c.count);
   }
   plan_pieces(p, ctx);
   for (size_t i = 0; i < p->classes.size(); ++i)
     if (p->classes[i].npieces) lanes[i] = p->classes[i].npieces * 64;
+  // expected time for the side-stream packing: recursion values per
+  // primitive quartet times primitive quartets, at each kernel kind's rate
+  // (ns per value, measured on an s-f batch: the generic kernel 0.005 with
+  // its quartets split over blocks, 0.03 without; a wave per quartet 0.0025;
+  // a lane per quartet 0.02), plus 100 ns per quartet
+  for (const auto& c : p->classes) {
+    const double vals = c.generic ? (double)l2g_gen_generic[c.cls].nvals
+                                  : (double)(c.fp32 ? l2g_gen_classes32 : l2g_gen_classes)[c.cls].nodes;
+    const double rate = c.generic ? (c.npieces ? 0.005 : 0.03) : (c.coop ? 0.0025 : 0.02);
+    cost.push_back((double)c.nprimq * vals * rate + 100.0 * (double)c.count);
+  }
   std::vector<ClassSlot> slots;
   // side regions up to 4 GiB of scratch beyond the shared one
   const int64_t total = plan_slots(slots, lanes, yd, ctx->ncu, p->block, l2g_context_s::kSide,
@@ -967,7 +981,9 @@ int l2g_eri_plan_set_wave_split(l2g_eri_plan p, int on) {
     l2g_context_s* ctx = p->basis->ctx;
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (on < 0 || on > 2) throw std::invalid_argument("wave split mode must be 0, 1 or 2");
     p->wave_split = on != 0;
+    p->few_generic = on != 2;
     p->drop_graph();
     plan_layout(p, ctx);
   });

@@ -23,6 +23,12 @@ differs between classes and is measured, not modelled:
         (K K < 1e-6, S S < 16e-6) and sets the fp32 field (1: the FP32
         variant is at least 5 % faster there, so mixed batches route to it
         by default); rebuild the library afterwards
+  python -m libint_amd.tune fewgen [--out libint_amd/tuned.cfg]
+        (GPU) for each class whose quartets of the f-recipe batch run a wave
+        per quartet, times them on its register kernel and on the generic
+        kernel (quartets split over blocks) and sets the fewgen field (1: the
+        generic kernel is at least 10 % faster, so such quartets go there);
+        rebuild the library afterwards
 """
 from __future__ import annotations
 
@@ -148,7 +154,7 @@ def measure(args):
 
 
 def read_cfg(path):
-    """class -> [budget, waves, fp32] of a tuned.cfg file."""
+    """class -> [budget, waves, fp32, fewgen] of a tuned.cfg file."""
     out = {}
     try:
         text = open(path).read().strip()
@@ -157,15 +163,17 @@ def read_cfg(path):
     for item in filter(None, text.split(",")):
         k, v = item.split("=")
         f = [int(x) for x in v.split(":")]
-        out[k] = f + [0] * (3 - len(f))
+        out[k] = f + [0] * (4 - len(f))
     return out
 
 
 def write_cfg(path, cfg):
-    """Entries that differ from the defaults (budget 160, waves 0, fp32 0)."""
-    keep = {c: v for c, v in cfg.items() if list(v) != [DEFAULT_BUDGET, 0, 0]}
+    """Entries that differ from the defaults (budget 160, waves 0, fp32 0,
+    fewgen 0); the fewgen field only where set."""
+    keep = {c: v for c, v in cfg.items() if list(v) != [DEFAULT_BUDGET, 0, 0, 0]}
     with open(path, "w") as f:
-        f.write(",".join(f"{c}={v[0]}:{v[1]}:{v[2]}" for c, v in sorted(keep.items())) + "\n")
+        f.write(",".join(f"{c}={v[0]}:{v[1]}:{v[2]}" + (f":{v[3]}" if v[3] else "")
+                         for c, v in sorted(keep.items())) + "\n")
     print(f"wrote {path}")
 
 
@@ -180,7 +188,7 @@ def select(args):
     for c in classes:
         best = min((r[c], int(b)) for b, r in res.items() if c in r)
         r = res[str(best[1])]
-        cfg[c] = [best[1], 0, fp32_pays(r.get(c + "/e64", r[c]), r.get(c + "/fp32"))]
+        cfg[c] = [best[1], 0, fp32_pays(r.get(c + "/e64", r[c]), r.get(c + "/fp32")), cfg.get(c, [0] * 4)[3]]
         print(f"{c}: budget {best[1]} ({best[0]:.2f} ms), fp32 {cfg[c][2]}; " +
               ", ".join(f"{b}: {r_[c]:.2f}" for b, r_ in sorted(res.items(), key=lambda x: int(x[0])) if c in r_))
     write_cfg(args.out, cfg)
@@ -200,7 +208,7 @@ def fp32(args):
     for c in classes.split(","):
         if c not in res:
             continue
-        v = cfg.setdefault(c, [DEFAULT_BUDGET, 0, 0])
+        v = cfg.setdefault(c, [DEFAULT_BUDGET, 0, 0, 0])
         t64, t32 = res.get(c + "/e64"), res.get(c + "/fp32")
         v[2] = fp32_pays(t64, t32) if t64 is not None else 0
         print(f"{c}: all significant quartets fp64 {res[c]:.2f} ms; FP32-eligible quartets fp64 "
@@ -209,16 +217,88 @@ def fp32(args):
     write_cfg(args.out, cfg)
 
 
+def canonical_class(ls):
+    """Class name of a quartet's four angular momenta as the plans name
+    it: each pair high-l first, bra and ket swapped when the bra's total
+    angular momentum is the larger."""
+    p1 = tuple(sorted(ls[:2], reverse=True))
+    p2 = tuple(sorted(ls[2:], reverse=True))
+    if sum(p1) > sum(p2):
+        p1, p2 = p2, p1
+    return "%d%d%d%d" % (p1 + p2)
+
+
+def fewgen(args):
+    """Few deep quartets: each class's register kernel (a wave per quartet)
+    against the generic kernel (quartets split over blocks) on that class's
+    quartets of the f-recipe batch (400k quartets of the 120-atom molecule,
+    seed 21, as bench.py's eri_f leg), each class alone, median of 3."""
+    import numpy as np
+
+    import libint_amd as la
+    from libint_amd import molecule
+    ctx = la.Context(0)
+    mol = molecule.make_molecule(3)
+    B = mol.basis
+    b = la.Basis(ctx, B)
+    i, j = np.tril_indices(B.nshell)
+    K = la.schwarz(b, np.stack([i, j], 1))
+    keep = K >= 1e-9
+    i, j, K = i[keep], j[keep], K[keep]
+    rng = np.random.default_rng(21)
+    n, picks, got = 400000, [], 0
+    while got < n:
+        p1 = rng.integers(0, len(K), 4 * n)
+        p2 = rng.integers(0, len(K), 4 * n)
+        ok = K[p1] * K[p2] >= 1e-12
+        picks.append((p1[ok], p2[ok]))
+        got += int(ok.sum())
+    p1 = np.concatenate([p[0] for p in picks])[:n]
+    p2 = np.concatenate([p[1] for p in picks])[:n]
+    q = np.stack([i[p1], j[p1], i[p2], j[p2]], 1).astype(np.int32)
+    names = np.array([canonical_class(tuple(int(B.l[s]) for s in row)) for row in q])
+    cfg = read_cfg(args.out)
+
+    def timed(qc, mode):
+        plan = la.EriPlan(b, qc)
+        plan.set_wave_split(mode)
+        d = plan.device_output()
+        plan.execute(d)
+        ms = []
+        for _ in range(3):
+            plan.execute(d)
+            ctx.synchronize()
+            ms.append(plan.elapsed_ms())
+        kinds = {("generic" if c["generic"] else "wave" if c["wave"] else "lane") for c in plan.stats()}
+        return float(np.median(ms)), kinds
+
+    for c in sorted(set(names)):
+        qc = q[names == c]
+        tw, kw = timed(qc, 2)
+        if kw != {"wave"}:
+            continue  # not a class of few deep quartets here
+        ctx.set_generic_kernels(True)
+        try:
+            tg, _ = timed(qc, 2)
+        finally:
+            ctx.set_generic_kernels(False)
+        v = cfg.setdefault(c, [DEFAULT_BUDGET, 0, 0, 0])
+        v[3] = 1 if tg < 0.9 * tw else 0
+        print(f"{c}: {len(qc)} quartets, wave {tw:.2f} ms, generic {tg:.2f} ms -> fewgen {v[3]}", flush=True)
+    write_cfg(args.out, cfg)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("step", choices=["build", "measure", "select", "fp32", "_time"])
+    ap.add_argument("step", choices=["build", "measure", "select", "fp32", "fewgen", "_time"])
     ap.add_argument("--fp32", action="store_true", help="_time: also the FP32 variants")
     ap.add_argument("--dir", default="tune_build")
     ap.add_argument("--budgets", default=",".join(map(str, DEFAULT_BUDGETS)))
     ap.add_argument("--classes", default="")
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.cfg"))
     args = ap.parse_args()
-    {"build": build, "measure": measure, "select": select, "fp32": fp32, "_time": time_library}[args.step](args)
+    {"build": build, "measure": measure, "select": select, "fp32": fp32, "fewgen": fewgen,
+     "_time": time_library}[args.step](args)
 
 
 if __name__ == "__main__":

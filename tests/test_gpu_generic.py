@@ -151,8 +151,9 @@ def test_few_deep_quartets_route_to_generic(ctx):
     L >= 7, few quartets and deep contractions runs on the generic kernel (a
     block per quartet, each contraction in its serial order), at the
     integral bar; with the split on (the default) it runs a wave per
-    quartet; the same class with shallow contractions stays on its register
-    kernel."""
+    quartet, or the generic kernel where that was measured faster for the
+    class (mode 2 keeps it on the wave); the same class with shallow
+    contractions stays on its register kernel."""
     rng = np.random.default_rng(41)
     deep = []
     for k in range(4):
@@ -163,6 +164,9 @@ def test_few_deep_quartets_route_to_generic(ctx):
     say(f"case: {len(q)} (dd|dd) quartets with K = 6 on every shell")
     b = la.Basis(ctx, arrays)
     plan = la.EriPlan(b, q)
+    st = plan.stats()
+    assert [(s["cls"], s["generic"] or s["wave"]) for s in st] == [("2222", True)], st
+    plan.set_wave_split(2)
     st = plan.stats()
     assert [(s["cls"], s["generic"], s["wave"]) for s in st] == [("2222", False, True)], st
     plan.set_wave_split(False)
