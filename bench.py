@@ -506,6 +506,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--fock-f-only", action="store_true",
                     help="only the f-recipe J/K build leg (for profiling its kernels) and exit")
+    ap.add_argument("--eri-f-only", action="store_true",
+                    help="only the f-recipe ERI batch leg (for profiling its kernels) and exit")
     ap.add_argument("--config4", choices=["fp64", "mixed"],
                     help="only the 200-atom f-recipe J/K build (about 16 min on one GPU) and exit")
     ap.add_argument("--rows-out", help="with --config4: .npz of the sampled rows for tests/test_config4_rows.py")
@@ -532,6 +534,11 @@ def main():
 
     if args.config4:
         print(json.dumps(config4_leg(la, molecule, args.config4, args.rows_out)), flush=True)
+        return
+
+    if args.eri_f_only:
+        ctx = la.Context(0)
+        print(json.dumps(eri_f_leg(la, molecule, flop_model, ctx, args.eri_f_quartets)), flush=True)
         return
 
     if args.fock_f_only:
